@@ -1,0 +1,150 @@
+/* bte.h -- C ABI of the MI355X (gfx950) steady-state phonon BTE source iteration.
+ *
+ * This is the drop-in boundary below the reference's C++ API.  The reference has no C ABI: its
+ * public surface is the C++ class StaticBTESolver (includes/StaticBTESolver/StaticBTESolver.h:81-95)
+ * whose solve() runs _preprocess (StaticBTESolver.cpp:372-688) and then the outer source
+ * iteration _iteration (StaticBTESolver.cpp:690-808) with a per-(band, direction) inner solve
+ * (_solve_matrix :270-308 on the CPU, class BICGSTAB src/bicgstab.hpp:33-321 on the GPU).  The
+ * entry points below replace that iteration as a whole: the host keeps the geometric and angular
+ * preprocessing, hands the arrays over once, and bte_solve() runs the whole iteration resident on
+ * one GPU.  The C++ facade (include/StaticBTESolver/StaticBTESolver.h) calls them.
+ *
+ * Conventions: plain C, no exceptions cross the boundary, every call returns BTE_OK (0) or a
+ * negative status and leaves a message for bte_last_error().  Arrays are caller-owned host
+ * memory and are copied on upload.  One context owns one HIP stream on one device; a context is
+ * not thread-safe, distinct contexts are independent.
+ */
+#ifndef STATICBTE_AMD_BTE_H
+#define STATICBTE_AMD_BTE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bte_ctx bte_ctx;
+
+enum bte_status {
+    BTE_OK = 0,
+    BTE_ERR_ARG = -1,      /* bad argument or size */
+    BTE_ERR_HIP = -2,      /* HIP runtime error */
+    BTE_ERR_STATE = -3,    /* call out of order (e.g. solve before geometry) */
+    BTE_ERR_BC = -4,       /* unsupported boundary condition for this mesh / quadrature */
+    BTE_ERR_MESH = -5,     /* mesh inconsistent (e.g. more incoming faces than the ELL holds) */
+    BTE_ERR_NOCONV = -6,   /* an inner solve hit its iteration limit (reported, not fatal) */
+    BTE_ERR_NODEVICE = -7  /* no HIP device */
+};
+
+enum bte_precision {
+    BTE_PREC_FP64 = 0,  /* everything in FP64 (the plain form) */
+    BTE_PREC_MIXED = 1  /* FP32 inner BiCGSTAB + FP64 residual refinement, quadrature and T update */
+};
+
+enum bte_precond {
+    BTE_PC_SWEEP = 0,  /* downwind (level-scheduled) sweep; exact when the upwind graph is acyclic */
+    BTE_PC_JACOBI = 1  /* the reference GPU solver's right-Jacobi BiCGSTAB semantics (bicgstab.hpp) */
+};
+
+typedef struct bte_opts {
+    int precision;        /* enum bte_precision, default BTE_PREC_FP64 */
+    int precond;          /* enum bte_precond, default BTE_PC_SWEEP */
+    int max_iter;         /* outer iterations, StaticBTESolver::solve(max_iter); default 10000 */
+    double stop_margin;   /* stop when margin <= stop_margin, StaticBTESolver.cpp:798; default 1e-5 */
+    double inner_tol;     /* relative residual of each inner solve, bicgstab.hpp:699 tol; 1e-9 */
+    int inner_maxit;      /* inner iteration limit, StaticBTESolver.cpp:699; default 1000 */
+    int refine_steps;     /* BTE_PREC_MIXED: max FP64 refinement steps per system; default 4 */
+    int block;            /* work-group size per system (64..1024), 0 = autotuned; default 0 */
+    int use_graph;        /* capture each outer iteration as a hipGraph; default 1 */
+    int verbose;          /* print the reference's per-iteration stdout lines; default 0 */
+    int timing;           /* per-kernel hipEvent timing into bte_stats; default 0 */
+    int learned;          /* apply the loaded learned correction every learned_every iterations */
+    int learned_every;    /* default 8 */
+} bte_opts;
+
+typedef struct bte_result {
+    /* caller-allocated outputs, any may be NULL */
+    double* T;             /* [n_cell] cell temperature (cell_temperature) */
+    double* T_band;        /* [n_cell * n_band] cell-major (cell_band_temperature[c][b]) */
+    double* bc_flux;       /* [n_bc] heat flux per BC (bc_heat_flux) */
+    double* bc_band_flux;  /* [n_bc * n_band] (bc_band_heat_flux[k][b]) */
+    double* margins;       /* [margins_capacity] margin of each outer iteration */
+    int margins_capacity;
+    /* filled by bte_solve */
+    int iterations;           /* outer iterations run */
+    int converged;            /* 1 if the last margin <= stop_margin */
+    long long inner_iters;    /* total inner BiCGSTAB iterations over all systems */
+    int inner_failures;       /* systems that hit inner_maxit (summed over outer iterations) */
+    double solve_seconds;     /* wall time of the outer loop (like StaticBTESolver.cpp:865-876) */
+    double setup_seconds;     /* assembly + sweep schedule + uploads */
+    int learned_accepted;     /* learned corrections accepted / proposed */
+    int learned_proposed;
+} bte_result;
+
+typedef struct bte_stats {
+    int n_kernels;
+    const char* names[16];
+    double ms[16];        /* summed hipEvent time per kernel kind (opts.timing = 1) */
+    long long calls[16];
+    double spmv_bytes;    /* algorithmic bytes of all SpMV row sweeps in the last solve */
+    long long spmv_rows;  /* rows processed by SpMVs in the last solve */
+    int ell_width;        /* off-diagonal slots per row */
+    int max_levels;       /* deepest sweep schedule over directions */
+    int lagged_edges;     /* upwind edges dropped from the sweep to break cycles */
+    int block;            /* work-group size used */
+} bte_stats;
+
+void bte_default_opts(bte_opts* opts);
+
+/* Contexts.  device < 0 uses the current HIP device. */
+int bte_ctx_create(int device, bte_ctx** out);
+void bte_ctx_destroy(bte_ctx* ctx);
+const char* bte_last_error(const bte_ctx* ctx);
+
+/* Geometry from _preprocess (StaticBTESolver.cpp:494-687): per cell volume; per cell face f
+ * (f opposite vertex f) area, outward unit normal (x,y,z) and neighbour (>= 0 a cell, < 0 the
+ * boundary entity k encoded as -k-1).  dim in {1,2,3}, n_face = dim+1. */
+int bte_set_geometry(bte_ctx* ctx, int dim, int n_cell, int n_face, const double* volume,
+                     const double* face_area, const double* face_normal, const int* neighbor);
+
+/* Angular quadrature from _preprocess (StaticBTESolver.cpp:374-492): control angles omega[n_dir],
+ * unit directions dir[n_dir*3], weighted directions S[n_dir*3]; num_phi is needed for the
+ * specular mirror maps (:205-252); solid_angle and T_ref as set by setParam (:48-94). */
+int bte_set_quadrature(bte_ctx* ctx, int DM, int n_dir, const double* omega, const double* dir,
+                       const double* S, int num_theta, int num_phi, double solid_angle, double T_ref);
+
+/* Bands (BTEBand.cpp:8-26): group velocity, relaxation time, Ctot, Lr. */
+int bte_set_bands(bte_ctx* ctx, int n_band, const double* v, const double* tau, const double* C,
+                  const double* Lr);
+
+/* Boundary conditions (BTEBoundaryCondition.cpp:7-23): index (= -k-1), type in {1,2,31,32,33},
+ * temperature. */
+int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const double* T);
+
+/* Runs the source iteration (StaticBTESolver::_iteration + _get_heat_flux).  Returns BTE_OK, or
+ * BTE_ERR_NOCONV when some inner solve hit inner_maxit (results are still filled in). */
+int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res);
+
+/* Final intensities e[b][d][c] ([n_band*n_dir*n_cell], ee_curr of the last iteration). */
+int bte_get_intensity(bte_ctx* ctx, double* ee);
+
+int bte_get_stats(bte_ctx* ctx, bte_stats* stats);
+
+/* ---- piecewise entry points (used by the parity tests; same kernels as bte_solve) ---- */
+
+/* Assembly of system (band, dir), StaticBTESolver.cpp:122-160, in dense face-slot form:
+ * diag[n_cell] and off[n_cell*n_face] (coefficient of neighbour[c*F+f], 0 where the face is
+ * outgoing or on the boundary). */
+int bte_debug_assemble(bte_ctx* ctx, int band, int dir, double* diag, double* off);
+
+/* Right-hand sides of all systems, StaticBTESolver.cpp:162-269, for given ee_prev [B*D*N] and
+ * density [N*B] (cell-major); out rhs [B*D*N]. */
+int bte_debug_rhs(bte_ctx* ctx, const double* ee_prev, const double* density, double* rhs);
+
+/* One batched inner solve of all systems for given ee_prev and density (as bte_debug_rhs);
+ * out x [B*D*N] and per-system iteration counts iters [B*D] (may be NULL). */
+int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, const double* density,
+                    double* x, int* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

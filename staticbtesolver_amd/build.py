@@ -1,0 +1,9 @@
+"""Builds the native libraries in-tree (gfx950 only)."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def build(jobs: int = 8) -> None:
+    subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "all"], check=True)

@@ -1,0 +1,245 @@
+"""ctypes binding of the library's C entry points (include/bte.h, include/bte_case.h).
+
+The shared libraries are built in-tree (staticbtesolver_amd/lib/); loading fails loudly when they
+are missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "libbte_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libStaticBTESolver.so")
+BTECMD = os.path.join(HERE, "bin", "BTEcmd")
+
+BTE_OK = 0
+BTE_ERR_ARG = -1
+BTE_ERR_HIP = -2
+BTE_ERR_STATE = -3
+BTE_ERR_BC = -4
+BTE_ERR_MESH = -5
+BTE_ERR_NOCONV = -6
+BTE_ERR_NODEVICE = -7
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+PC_SWEEP = 0
+PC_JACOBI = 1
+
+
+class BTEError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{msg} (status {code})")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("max_iter", C.c_int),
+                ("stop_margin", C.c_double), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
+                ("refine_steps", C.c_int), ("block", C.c_int), ("use_graph", C.c_int), ("verbose", C.c_int),
+                ("timing", C.c_int), ("learned", C.c_int), ("learned_every", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("T", C.POINTER(C.c_double)), ("T_band", C.POINTER(C.c_double)),
+                ("bc_flux", C.POINTER(C.c_double)), ("bc_band_flux", C.POINTER(C.c_double)),
+                ("margins", C.POINTER(C.c_double)), ("margins_capacity", C.c_int),
+                ("iterations", C.c_int), ("converged", C.c_int), ("inner_iters", C.c_longlong),
+                ("inner_failures", C.c_int), ("solve_seconds", C.c_double), ("setup_seconds", C.c_double),
+                ("learned_accepted", C.c_int), ("learned_proposed", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_kernels", C.c_int), ("names", C.c_char_p * 16), ("ms", C.c_double * 16),
+                ("calls", C.c_longlong * 16), ("spmv_bytes", C.c_double), ("spmv_rows", C.c_longlong),
+                ("ell_width", C.c_int), ("max_levels", C.c_int), ("lagged_edges", C.c_int), ("block", C.c_int)]
+
+
+# symbols declared in include/bte.h and include/bte_case.h
+HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_last_error", "bte_set_geometry",
+               "bte_set_quadrature", "bte_set_bands", "bte_set_bcs", "bte_solve", "bte_get_intensity",
+               "bte_get_stats", "bte_debug_assemble", "bte_debug_rhs", "bte_debug_inner"]
+CASE_SYMBOLS = ["bte_case_open", "bte_case_close", "bte_case_sizes", "bte_case_params", "bte_case_solve",
+                "bte_case_geometry", "bte_case_quadrature", "bte_case_ctx", "bte_case_error"]
+
+_lib = None
+
+
+def lib():
+    """Loads libStaticBTESolver.so (which links libbte_hip.so).  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not (os.path.exists(HOST_LIB) and os.path.exists(HIP_LIB)):
+        raise RuntimeError(f"staticbtesolver_amd native libraries are not built ({LIB_DIR}); run "
+                           "`python -c 'import __graft_entry__ as g; g.build()'` or `make -C staticbtesolver_amd`")
+    C.CDLL(HIP_LIB, mode=C.RTLD_GLOBAL)
+    L = C.CDLL(HOST_LIB)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    L.bte_default_opts.argtypes = [C.POINTER(Opts)]
+    L.bte_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.bte_ctx_destroy.argtypes = [vp]
+    L.bte_last_error.restype = C.c_char_p
+    L.bte_last_error.argtypes = [vp]
+    L.bte_set_geometry.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, ip]
+    L.bte_set_quadrature.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_double, C.c_double]
+    L.bte_set_bands.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+    L.bte_set_bcs.argtypes = [vp, C.c_int, ip, ip, dp]
+    L.bte_solve.argtypes = [vp, C.POINTER(Opts), C.POINTER(Result)]
+    L.bte_get_intensity.argtypes = [vp, dp]
+    L.bte_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.bte_debug_assemble.argtypes = [vp, C.c_int, C.c_int, dp, dp]
+    L.bte_debug_rhs.argtypes = [vp, dp, dp, dp]
+    L.bte_debug_inner.argtypes = [vp, C.POINTER(Opts), dp, dp, dp, ip]
+    L.bte_case_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_int,
+                                C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
+    L.bte_case_close.argtypes = [vp]
+    L.bte_case_sizes.argtypes = [vp, ip]
+    L.bte_case_params.argtypes = [vp, dp]
+    L.bte_case_solve.argtypes = [vp, C.POINTER(Opts), C.c_int, C.POINTER(Result)]
+    L.bte_case_geometry.argtypes = [vp, dp, dp, dp, dp, ip]
+    L.bte_case_quadrature.argtypes = [vp, dp, dp, dp]
+    L.bte_case_ctx.restype = vp
+    L.bte_case_ctx.argtypes = [vp]
+    L.bte_case_error.restype = C.c_char_p
+    L.bte_case_error.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().bte_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Case:
+    """A problem loaded from the reference's input files through the C++ facade
+    (StaticBTESolver + BTEMesh + BTEBand + BTEBoundaryCondition)."""
+
+    def __init__(self, geometry, band_file, bc_file, DM, ntheta, nphi, T_ref=300.0, Lx=0.0, Ly=0.0, Lz=0.0,
+                 device=-1):
+        L = lib()
+        err = C.create_string_buffer(512)
+        h = C.c_void_p()
+        rc = L.bte_case_open(str(geometry).encode(), Lx, Ly, Lz, str(band_file).encode(), str(bc_file).encode(),
+                             DM, ntheta, nphi, T_ref, device, C.byref(h), err, 512)
+        if rc != BTE_OK:
+            raise BTEError(rc, err.value.decode())
+        self.h = h
+        s = np.zeros(6, dtype=np.int32)
+        L.bte_case_sizes(h, _ip(s))
+        self.N_cell, self.N_dir, self.N_band, self.N_face, self.n_bc, self.dim = [int(x) for x in s]
+        p = np.zeros(2)
+        L.bte_case_params(h, _dp(p))
+        self.T_ref, self.solid_angle = float(p[0]), float(p[1])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bte_case_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def geometry(self):
+        N, F = self.N_cell, self.N_face
+        vol, cen = np.zeros(N), np.zeros(3 * N)
+        area, nrm = np.zeros(N * F), np.zeros(3 * N * F)
+        nbr = np.zeros(N * F, dtype=np.int32)
+        lib().bte_case_geometry(self.h, _dp(vol), _dp(cen), _dp(area), _dp(nrm), _ip(nbr))
+        return dict(vol=vol, centre=cen.reshape(N, 3), area=area.reshape(N, F), normal=nrm.reshape(N, F, 3),
+                    nbr=nbr.reshape(N, F))
+
+    def quadrature(self):
+        D = self.N_dir
+        om, d, S = np.zeros(D), np.zeros(3 * D), np.zeros(3 * D)
+        lib().bte_case_quadrature(self.h, _dp(om), _dp(d), _dp(S))
+        return om, d.reshape(D, 3), S.reshape(D, 3)
+
+    def solve(self, opts: Opts | None = None, verbose=False, margins_capacity=None, **kw):
+        if opts is None:
+            opts = default_opts(**kw)
+        N, B, nbc = self.N_cell, self.N_band, self.n_bc
+        cap = margins_capacity or max(1, opts.max_iter)
+        T, Tb, fl, bfl, m = np.zeros(N), np.zeros(N * B), np.zeros(max(nbc, 1)), np.zeros(max(nbc, 1) * B), np.zeros(cap)
+        r = Result()
+        r.T, r.T_band, r.bc_flux, r.bc_band_flux, r.margins = _dp(T), _dp(Tb), _dp(fl), _dp(bfl), _dp(m)
+        r.margins_capacity = cap
+        rc = lib().bte_case_solve(self.h, C.byref(opts), 1 if verbose else 0, C.byref(r))
+        if rc not in (BTE_OK, BTE_ERR_NOCONV):
+            raise BTEError(rc, lib().bte_case_error(self.h).decode())
+        it = r.iterations
+        return dict(T=T, T_band=Tb.reshape(N, B), flux=fl[:nbc], band_flux=bfl[:nbc * B].reshape(nbc, B),
+                    margins=m[:min(it, cap)].copy(), iters=it, converged=bool(r.converged),
+                    inner_iters=int(r.inner_iters), inner_failures=int(r.inner_failures),
+                    solve_seconds=r.solve_seconds, setup_seconds=r.setup_seconds, status=rc,
+                    learned_accepted=r.learned_accepted, learned_proposed=r.learned_proposed)
+
+    def ctx(self):
+        c = lib().bte_case_ctx(self.h)
+        if not c:
+            raise BTEError(BTE_ERR_NODEVICE, lib().bte_case_error(self.h).decode())
+        return c
+
+    # ---- piecewise entry points on the loaded context ----
+    def _check(self, rc, ctx):
+        if rc not in (BTE_OK, BTE_ERR_NOCONV):
+            raise BTEError(rc, lib().bte_last_error(ctx).decode())
+        return rc
+
+    def debug_assemble(self, band, d):
+        ctx = self.ctx()
+        diag, off = np.zeros(self.N_cell), np.zeros(self.N_cell * self.N_face)
+        self._check(lib().bte_debug_assemble(ctx, band, d, _dp(diag), _dp(off)), ctx)
+        return diag, off.reshape(self.N_cell, self.N_face)
+
+    def debug_rhs(self, ee_prev, dens):
+        ctx = self.ctx()
+        out = np.zeros(self.N_band * self.N_dir * self.N_cell)
+        ee_prev = np.ascontiguousarray(ee_prev, dtype=np.float64)
+        dens = np.ascontiguousarray(dens, dtype=np.float64)
+        self._check(lib().bte_debug_rhs(ctx, _dp(ee_prev), _dp(dens), _dp(out)), ctx)
+        return out.reshape(self.N_band, self.N_dir, self.N_cell)
+
+    def debug_inner(self, ee_prev, dens, opts: Opts | None = None, **kw):
+        ctx = self.ctx()
+        if opts is None:
+            opts = default_opts(**kw)
+        out = np.zeros(self.N_band * self.N_dir * self.N_cell)
+        its = np.zeros(self.N_band * self.N_dir, dtype=np.int32)
+        ee_prev = np.ascontiguousarray(ee_prev, dtype=np.float64)
+        dens = np.ascontiguousarray(dens, dtype=np.float64)
+        self._check(lib().bte_debug_inner(ctx, C.byref(opts), _dp(ee_prev), _dp(dens), _dp(out), _ip(its)), ctx)
+        return out.reshape(self.N_band, self.N_dir, self.N_cell), its
+
+    def intensity(self):
+        ctx = self.ctx()
+        out = np.zeros(self.N_band * self.N_dir * self.N_cell)
+        self._check(lib().bte_get_intensity(ctx, _dp(out)), ctx)
+        return out.reshape(self.N_band, self.N_dir, self.N_cell)
+
+    def stats(self):
+        ctx = self.ctx()
+        s = Stats()
+        self._check(lib().bte_get_stats(ctx, C.byref(s)), ctx)
+        names = [s.names[i].decode() for i in range(s.n_kernels)]
+        return dict(ms={n: s.ms[i] for i, n in enumerate(names)}, calls={n: s.calls[i] for i, n in enumerate(names)},
+                    spmv_rows=s.spmv_rows, ell_width=s.ell_width, max_levels=s.max_levels,
+                    lagged_edges=s.lagged_edges, block=s.block)

@@ -1,0 +1,722 @@
+// bte_kernels.hpp -- gfx950 device code of the source iteration.
+//
+// Layout in HBM (one context, N cells, F faces per cell, D directions, B bands, S = B*D systems):
+//   geometry     vol[N], area[N*F], normal[N*F*3], nbr[N*F]           (read by every system)
+//   assembly     per direction, shared by all bands of that direction:
+//                dg[D*N]            sum of A_f (S_d.n_f) over outgoing faces
+//                ell_col/ell_val[(d*W+k)*N + c]  incoming interior faces, k < W = F-1 slots
+//   sweep        per direction, cells in downwind level order:
+//                lvl_ptr[lvl_off[d] .. + L_d], perm[d*N+i], swp_col/swp_val[(d*W+k)*N+i], swp_dg[d*N+i]
+//   intensities  ee[b][d][c] = ee[(b*D+d)*N + c]  (the reference's ee_curr[b]->data layout)
+//   Krylov work  one slice of N per system and vector, same system order as ee
+//
+// Matrix of system (b, d), StaticBTESolver.cpp:122-160:
+//   diag(c) = v_b tau_b dg[d][c] + w_d V_c        (w_d = omega_d in 2D/3D, 1 in 1D)
+//   off(c, k) = v_b tau_b ell_val[d][k][c]       (column ell_col[d][k][c])
+// The band enters only through the scalar v_b tau_b, so the stored operator is shared by bands.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bte {
+
+constexpr int kMaxBC = 32;
+constexpr double kPi = 3.14159265358979323846;
+
+// Dot product without FMA contraction, summed x+y+z left to right: the upwind test d.n >= 0 must
+// classify every face identically in every kernel (and on the host).
+__device__ __forceinline__ double dot_nc(const double* a, const double* b) {
+#pragma clang fp contract(off)
+    double s = a[0] * b[0];
+    s = s + a[1] * b[1];
+    s = s + a[2] * b[2];
+    return s;
+}
+
+struct Geo {
+    int dim, N, F;
+    const double* vol;
+    const double* area;
+    const double* normal;
+    const int* nbr;
+};
+
+struct Quad {
+    int D;
+    const double* omega;
+    const double* dir;
+    const double* S;
+};
+
+// ----------------------------------------------------------------------------------------------
+// Assembly (StaticBTESolver.cpp:122-160), one thread per (direction, cell).  Writes the band-free
+// parts of the operator.  *overflow counts rows with more than W incoming interior faces.
+// ----------------------------------------------------------------------------------------------
+__global__ void k_assemble(Geo g, Quad q, int W, double* __restrict__ dg, int* __restrict__ ell_col,
+                           double* __restrict__ ell_val, int* __restrict__ overflow) {
+    const long long total = (long long)q.D * g.N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int d = (int)(t / g.N), c = (int)(t % g.N);
+        const double* dv = q.dir + 3 * d;
+        const double* sv = (g.dim > 1) ? q.S + 3 * d : dv;
+        double out = 0;
+        int k = 0;
+        for (int f = 0; f < g.F; f++) {
+            const double* n = g.normal + 3 * ((size_t)c * g.F + f);
+            const double coef = g.area[(size_t)c * g.F + f] * dot_nc(sv, n);
+            if (dot_nc(dv, n) >= 0) {
+                out += coef;
+            } else {
+                const int nb = g.nbr[(size_t)c * g.F + f];
+                if (nb >= 0) {
+                    if (k < W) {
+                        ell_col[((size_t)d * W + k) * g.N + c] = nb;
+                        ell_val[((size_t)d * W + k) * g.N + c] = coef;
+                    } else {
+                        atomicAdd(overflow, 1);
+                    }
+                    k++;
+                }
+            }
+        }
+        for (; k < W; k++) {
+            ell_col[((size_t)d * W + k) * g.N + c] = -1;
+            ell_val[((size_t)d * W + k) * g.N + c] = 0;
+        }
+        dg[(size_t)d * g.N + c] = out;
+    }
+}
+
+// Gathers the operator into downwind level order (sweep schedule built on the host from ell_col).
+template <typename R>
+__global__ void k_gather_sweep(int N, int D, int W, const int* __restrict__ perm, const double* __restrict__ dg,
+                               const double* __restrict__ ell_val, R* __restrict__ swp_dg,
+                               R* __restrict__ swp_val) {
+    const long long total = (long long)D * N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int d = (int)(t / N), i = (int)(t % N);
+        const int c = perm[(size_t)d * N + i];
+        swp_dg[(size_t)d * N + i] = (R)dg[(size_t)d * N + c];
+        for (int k = 0; k < W; k++)
+            swp_val[((size_t)d * W + k) * N + i] = (R)ell_val[((size_t)d * W + k) * N + c];
+    }
+}
+
+template <typename R>
+__global__ void k_convert(long long n, const double* __restrict__ in, R* __restrict__ out) {
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        out[t] = (R)in[t];
+}
+
+// ----------------------------------------------------------------------------------------------
+// Boundary data
+// ----------------------------------------------------------------------------------------------
+struct Bnd {
+    int nbc;
+    int bci[kMaxBC];  // index = -k-1
+    int bct[kMaxBC];  // type
+    double bcT[kMaxBC];
+    int n_bf;                // boundary faces (nbr < 0)
+    const int* bf_cell;      // [n_bf]
+    const int* bf_face;      // [n_bf]
+    const int* bfid;         // [N*F] -> boundary face id or -1
+    const int* mirror;       // [3*D] mirror direction for types 31, 32, 33 (-1 unsupported)
+    const double* ediff;     // [B*n_bf] diffuse re-emission of the previous iterate
+    int has_diffuse;
+};
+
+struct Bands {
+    int B;
+    const double* v;
+    const double* tau;
+    const double* C;
+    const double* Lr;
+    double solid_angle, T_ref;
+};
+
+// Diffuse (type 2) re-emitted intensity of each boundary face, StaticBTESolver.cpp:180-204.  It
+// does not depend on the incoming direction, so it is formed once per (band, face) and iteration.
+__global__ void k_diffuse(Geo g, Quad q, Bnd bd, int B, const double* __restrict__ ee_prev,
+                          double* __restrict__ ediff) {
+    const int total = B * bd.n_bf;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int b = t / bd.n_bf, bf = t % bd.n_bf;
+        const int c = bd.bf_cell[bf], f = bd.bf_face[bf];
+        const int nb = g.nbr[(size_t)c * g.F + f];
+        bool diffuse = false;
+        for (int k = 0; k < bd.nbc; k++) diffuse |= (bd.bci[k] == nb && bd.bct[k] == 2);
+        if (!diffuse) {
+            ediff[t] = 0;
+            continue;
+        }
+        const double* n = g.normal + 3 * ((size_t)c * g.F + f);
+        double es = 0, tw = 0;
+        for (int d2 = 0; d2 < q.D; d2++) {
+            const double dn = dot_nc(q.dir + 3 * d2, n);
+            if (dn > 0) {
+                const double e = ee_prev[((size_t)b * q.D + d2) * g.N + c];
+                if (g.dim == 3) {
+                    const double sn = dot_nc(q.S + 3 * d2, n);
+                    es += e * sn * q.omega[d2];
+                    tw += sn * q.omega[d2];
+                } else if (g.dim == 2) {
+                    const double sn = dot_nc(q.S + 3 * d2, n);
+                    es += e * sn;
+                    tw += sn;
+                } else {
+                    es += e * dn * q.omega[d2];
+                }
+            }
+        }
+        ediff[t] = (g.dim != 1) ? es / tw : es / kPi;
+    }
+}
+
+// Right-hand side of system (b, d) at cell c, StaticBTESolver.cpp:162-269.
+__device__ __forceinline__ double rhs_at(const Geo& g, const Quad& q, const Bnd& bd, const Bands& bands, int b, int d,
+                                         int c, const double* __restrict__ ee_prev, const double* __restrict__ dens) {
+    double re = 0;
+    const double* dv = q.dir + 3 * d;
+    const double* sv = (g.dim > 1) ? q.S + 3 * d : dv;
+    const double vt = bands.v[b] * bands.tau[b];
+    for (int f = 0; f < g.F; f++) {
+        const int nb = g.nbr[(size_t)c * g.F + f];
+        if (nb >= 0) continue;  // boundary indices are negative; interior faces carry no BC
+        const double* n = g.normal + 3 * ((size_t)c * g.F + f);
+        if (dot_nc(dv, n) >= 0) continue;
+        const double a = vt * g.area[(size_t)c * g.F + f] * dot_nc(sv, n);
+        for (int k = 0; k < bd.nbc; k++) {
+            if (bd.bci[k] != nb) continue;
+            const int type = bd.bct[k];
+            if (type == 1) {
+                re -= bands.C[b] / bands.solid_angle * (bd.bcT[k] - bands.T_ref) * a;
+            } else if (type == 2) {
+                re -= bd.ediff[(size_t)b * bd.n_bf + bd.bfid[(size_t)c * g.F + f]] * a;
+            } else {
+                const int slot = type - 31;  // 31, 32, 33 validated on the host
+                const int m = bd.mirror[slot * q.D + d];
+                re -= ee_prev[((size_t)b * q.D + m) * g.N + c] * a;
+            }
+        }
+    }
+    const double w = (g.dim != 1) ? q.omega[d] : 1.0;
+    re += dens[(size_t)b * g.N + c] * g.vol[c] * w;
+    return re;
+}
+
+// Standalone RHS of every system (parity tests).
+__global__ void k_rhs(Geo g, Quad q, Bnd bd, Bands bands, const double* __restrict__ ee_prev,
+                      const double* __restrict__ dens, double* __restrict__ out) {
+    const long long total = (long long)bands.B * q.D * g.N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(t % g.N);
+        const int s = (int)(t / g.N);
+        out[t] = rhs_at(g, q, bd, bands, s / q.D, s % q.D, c, ee_prev, dens);
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Work-group reductions: wave64 butterfly, then the per-wave sums are read back from LDS by every
+// thread in the same order, so all threads hold the bit-identical total (uniform control flow).
+// ----------------------------------------------------------------------------------------------
+template <int BLOCK, int K>
+__device__ __forceinline__ void block_sum(double* v, double* lds) {
+    constexpr int NW = BLOCK / 64;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        v[k] = x;
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) lds[k * NW + wid] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) s += lds[k * NW + w];
+        v[k] = s;
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Batched inner solve: one work-group per system, the whole BiCGSTAB loop inside the kernel
+// (per-system convergence, no host round trips).  Systems are s = b*D + d.
+// ----------------------------------------------------------------------------------------------
+struct Ops {
+    int N, D, W;
+    const double* vol;
+    const double* omega;  // w_d (already 1 in 1D)
+    const double* vt;     // [B] v_b * tau_b
+    const double* dg;     // [D*N]
+    const int* ell_col;   // [(d*W+k)*N + c]
+    const double* ell_val;
+    const float* ell_val32;   // FP32 copy for the mixed-precision inner solve
+    const double* dg32src;    // unused placeholder for symmetry
+    const float* dg32;        // [D*N] FP32 copy
+    // sweep schedule
+    const int* lvl_off;   // [D] offset of direction d's level pointer list
+    const int* lvl_cnt;   // [D] number of levels of direction d
+    const int* lvl_ptr;   // concatenated [sum(L_d + 1)]
+    const int* perm;      // [D*N]
+    const int* swp_col;   // [(d*W+k)*N + i]
+    const double* swp_val;
+    const double* swp_dg;
+    const float* swp_val32;
+    const float* swp_dg32;
+};
+
+struct Work {
+    double* x;     // ee_curr [S*N]
+    double* b;     // rhs [S*N]
+    double* r64;   // FP64 residual (mixed precision)
+    void* r;       // Krylov vectors of type R [S*N] each
+    void* rh;
+    void* p;
+    void* v;
+    void* ph;
+    void* sh;
+    void* t;
+    void* y;       // mixed precision: correction of the current refinement step
+    void* f;       // mixed precision: scaled FP64 residual rounded to FP32 (inner right-hand side)
+    int* iters;    // [S] accumulated inner iterations
+    int* fails;    // [S] accumulated iteration-limit hits
+};
+
+struct SolveCfg {
+    int S;
+    int maxit;
+    double tol;
+    int refine_steps;
+    int reference_semantics;  // Jacobi path: reproduce bicgstab.hpp's first step and restarts
+};
+
+template <typename R>
+struct SysView {
+    int N, W, d;
+    R vt;           // v_b tau_b
+    R wv;           // w_d
+    const double* vol;
+    const R* dg;    // dg[d*N ..]
+    const int* col; // ell_col + d*W*N
+    const R* val;
+    // sweep
+    int L;
+    const int* lp;
+    const int* perm;
+    const int* scol;
+    const R* sval;
+    const R* sdg;
+};
+
+template <typename R>
+__device__ __forceinline__ R diag_of(const SysView<R>& sv, R dgc, int c) {
+    return sv.vt * dgc + sv.wv * (R)sv.vol[c];
+}
+
+// y = A x (+ returns nothing; caller syncs)
+template <typename R, int BLOCK>
+__device__ __forceinline__ void spmv(const SysView<R>& sv, const R* x, R* __restrict__ y) {
+    for (int c = threadIdx.x; c < sv.N; c += BLOCK) {
+        R acc = diag_of(sv, sv.dg[c], c) * x[c];
+        R off = 0;
+#pragma unroll 3
+        for (int k = 0; k < sv.W; k++) {
+            const int j = sv.col[(size_t)k * sv.N + c];
+            if (j >= 0) off += sv.val[(size_t)k * sv.N + c] * x[j];
+        }
+        y[c] = acc + sv.vt * off;
+    }
+}
+
+// y = M^-1 v: forward substitution in downwind level order (exact inverse when no edge is lagged).
+template <typename R, int BLOCK>
+__device__ __forceinline__ void sweep(const SysView<R>& sv, const R* v, R* y) {
+    for (int l = 0; l < sv.L; l++) {
+        const int beg = sv.lp[l], end = sv.lp[l + 1];
+        for (int i = beg + (int)threadIdx.x; i < end; i += BLOCK) {
+            const int c = sv.perm[i];
+            R s = 0;
+#pragma unroll 3
+            for (int k = 0; k < sv.W; k++) {
+                const int j = sv.scol[(size_t)k * sv.N + i];
+                if (j >= 0) s += sv.sval[(size_t)k * sv.N + i] * y[j];
+            }
+            y[c] = (v[c] - sv.vt * s) / diag_of(sv, sv.sdg[i], c);
+        }
+        __syncthreads();
+    }
+}
+
+template <typename R, int BLOCK>
+__device__ __forceinline__ void jacobi(const SysView<R>& sv, const R* v, R* __restrict__ y) {
+    for (int c = threadIdx.x; c < sv.N; c += BLOCK) y[c] = v[c] / diag_of(sv, sv.dg[c], c);
+}
+
+template <typename R, int BLOCK, bool SWEEP>
+__device__ __forceinline__ void precond(const SysView<R>& sv, const R* v, R* y) {
+    if constexpr (SWEEP)
+        sweep<R, BLOCK>(sv, v, y);
+    else
+        jacobi<R, BLOCK>(sv, v, y);
+}
+
+// BiCGSTAB on A x = rhs for one system, right-preconditioned.  On entry r holds rhs; x is
+// overwritten.  Two variants:
+//  - reference semantics (Jacobi path): bicgstab.hpp:146-298 -- p starts as r, q = 0, the first
+//    step forms p = (1 + rho) r; breakdowns (|rho| or |<r_hat, q>| < 1e-200) restart from x = 0
+//    with r_hat = 6.8 rhs + 0.6; stop when ||r|| / ||r0|| < tol on the recursive residual.
+//  - standard form (sweep path): p = 0, v = 0 so the first step is p = r; an early exit when
+//    ||s|| / ||r0|| < tol (with the exact sweep this ends after one sweep and one product).
+// Returns the iterations used (maxit + 1 if the limit was hit without convergence).
+template <typename R, int BLOCK, bool SWEEP>
+__device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R* p, R* v, R* ph, R* sh, R* t,
+                        int maxit, double tol, bool refsem, double* lds) {
+    const int N = sv.N;
+    double red[2];
+    // setup: x = 0, r = rhs, r_hat = r, p = r (reference) or 0, v = 0
+    red[0] = 0;
+    for (int c = threadIdx.x; c < N; c += BLOCK) {
+        const R bc = rhs[c];
+        x[c] = 0;
+        r[c] = bc;
+        rh[c] = bc;
+        p[c] = refsem ? bc : (R)0;
+        v[c] = 0;
+        red[0] += (double)bc * (double)bc;
+    }
+    block_sum<BLOCK, 1>(red, lds);
+    const double nrm0 = sqrt(red[0]);
+    double nrm = nrm0;
+    double rho = 1, rhop = 1, alpha = 1, omega = 1;
+    double rho_next = red[0];  // <r_hat, r> with r_hat = r
+    int it = 0;
+    bool converged = false;
+    for (int i = 1; i <= maxit; i++) {
+        it = i;
+        if (nrm < 1e-50) {
+            converged = true;
+            break;
+        }
+        rhop = rho;
+        rho = rho_next;
+        if (rho < 1e-200 && rho > -1e-200) {
+            // restart (bicgstab.hpp:170-180)
+            rho = alpha = omega = 1;
+            for (int c = threadIdx.x; c < N; c += BLOCK) {
+                const R bc = rhs[c];
+                x[c] = 0;
+                v[c] = 0;
+                p[c] = 0;
+                r[c] = bc;
+                rh[c] = refsem ? (R)(bc * (R)6.8 + (R)0.6) : bc;
+            }
+            __syncthreads();
+            red[0] = 0;
+            for (int c = threadIdx.x; c < N; c += BLOCK) red[0] += (double)rh[c] * (double)r[c];
+            block_sum<BLOCK, 1>(red, lds);
+            rho_next = red[0];
+            continue;
+        }
+        const double beta = (rho / rhop) * (alpha / omega);
+        // p = (p - omega v) * beta + r
+        for (int c = threadIdx.x; c < N; c += BLOCK) p[c] = (p[c] - (R)omega * v[c]) * (R)beta + r[c];
+        __syncthreads();
+        precond<R, BLOCK, SWEEP>(sv, p, ph);
+        __syncthreads();
+        spmv<R, BLOCK>(sv, ph, v);
+        red[0] = 0;
+        for (int c = threadIdx.x; c < N; c += BLOCK) red[0] += (double)rh[c] * (double)v[c];
+        block_sum<BLOCK, 1>(red, lds);
+        const double rv = red[0];
+        if (rv < 1e-200 && rv > -1e-200) {
+            // restart (bicgstab.hpp:224-235)
+            rho = alpha = omega = 1;
+            for (int c = threadIdx.x; c < N; c += BLOCK) {
+                const R bc = rhs[c];
+                x[c] = 0;
+                v[c] = 0;
+                p[c] = 0;
+                r[c] = bc;
+                rh[c] = refsem ? (R)(bc * (R)6.8 + (R)0.6) : bc;
+            }
+            __syncthreads();
+            red[0] = 0;
+            for (int c = threadIdx.x; c < N; c += BLOCK) red[0] += (double)rh[c] * (double)r[c];
+            block_sum<BLOCK, 1>(red, lds);
+            rho_next = red[0];
+            continue;
+        }
+        alpha = rho / rv;
+        // s = r - alpha v (stored in r)
+        red[0] = 0;
+        for (int c = threadIdx.x; c < N; c += BLOCK) {
+            const R s = r[c] - (R)alpha * v[c];
+            r[c] = s;
+            red[0] += (double)s * (double)s;
+        }
+        block_sum<BLOCK, 1>(red, lds);
+        if (!refsem && sqrt(red[0]) < tol * nrm0) {
+            for (int c = threadIdx.x; c < N; c += BLOCK) x[c] += (R)alpha * ph[c];
+            __syncthreads();
+            converged = true;
+            break;
+        }
+        precond<R, BLOCK, SWEEP>(sv, r, sh);
+        __syncthreads();
+        spmv<R, BLOCK>(sv, sh, t);
+        red[0] = 0;
+        red[1] = 0;
+        for (int c = threadIdx.x; c < N; c += BLOCK) {
+            red[0] += (double)t[c] * (double)r[c];
+            red[1] += (double)t[c] * (double)t[c];
+        }
+        block_sum<BLOCK, 2>(red, lds);
+        omega = (red[1] != 0) ? red[0] / red[1] : 0.0;
+        red[0] = 0;
+        red[1] = 0;
+        for (int c = threadIdx.x; c < N; c += BLOCK) {
+            x[c] += (R)alpha * ph[c] + (R)omega * sh[c];
+            const R rn = r[c] - (R)omega * t[c];
+            r[c] = rn;
+            red[0] += (double)rn * (double)rn;
+            red[1] += (double)rh[c] * (double)rn;
+        }
+        block_sum<BLOCK, 2>(red, lds);
+        nrm = sqrt(red[0]);
+        rho_next = red[1];
+        if (nrm / nrm0 < tol) {
+            converged = true;
+            break;
+        }
+    }
+    __syncthreads();
+    return converged ? it : maxit + 1;
+}
+
+template <typename R>
+__device__ __forceinline__ SysView<R> make_view(const Ops& o, int b, int d);
+
+template <>
+__device__ __forceinline__ SysView<double> make_view<double>(const Ops& o, int b, int d) {
+    SysView<double> sv;
+    sv.N = o.N;
+    sv.W = o.W;
+    sv.d = d;
+    sv.vt = o.vt[b];
+    sv.wv = o.omega[d];
+    sv.vol = o.vol;
+    sv.dg = o.dg + (size_t)d * o.N;
+    sv.col = o.ell_col + (size_t)d * o.W * o.N;
+    sv.val = o.ell_val + (size_t)d * o.W * o.N;
+    sv.L = o.lvl_cnt ? o.lvl_cnt[d] : 0;
+    sv.lp = o.lvl_ptr ? o.lvl_ptr + o.lvl_off[d] : nullptr;
+    sv.perm = o.perm ? o.perm + (size_t)d * o.N : nullptr;
+    sv.scol = o.swp_col ? o.swp_col + (size_t)d * o.W * o.N : nullptr;
+    sv.sval = o.swp_val ? o.swp_val + (size_t)d * o.W * o.N : nullptr;
+    sv.sdg = o.swp_dg ? o.swp_dg + (size_t)d * o.N : nullptr;
+    return sv;
+}
+
+template <>
+__device__ __forceinline__ SysView<float> make_view<float>(const Ops& o, int b, int d) {
+    SysView<float> sv;
+    sv.N = o.N;
+    sv.W = o.W;
+    sv.d = d;
+    sv.vt = (float)o.vt[b];
+    sv.wv = (float)o.omega[d];
+    sv.vol = o.vol;
+    sv.dg = o.dg32 + (size_t)d * o.N;
+    sv.col = o.ell_col + (size_t)d * o.W * o.N;
+    sv.val = o.ell_val32 + (size_t)d * o.W * o.N;
+    sv.L = o.lvl_cnt ? o.lvl_cnt[d] : 0;
+    sv.lp = o.lvl_ptr ? o.lvl_ptr + o.lvl_off[d] : nullptr;
+    sv.perm = o.perm ? o.perm + (size_t)d * o.N : nullptr;
+    sv.scol = o.swp_col ? o.swp_col + (size_t)d * o.W * o.N : nullptr;
+    sv.sval = o.swp_val32 ? o.swp_val32 + (size_t)d * o.W * o.N : nullptr;
+    sv.sdg = o.swp_dg32 ? o.swp_dg32 + (size_t)d * o.N : nullptr;
+    return sv;
+}
+
+// FP64 path: RHS formed in-kernel, then BiCGSTAB in FP64 writes ee_curr directly.
+template <int BLOCK, bool SWEEP>
+__global__ __launch_bounds__(BLOCK) void k_solve64(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, SolveCfg cfg,
+                                                  const double* __restrict__ ee_prev, const double* __restrict__ dens) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    for (int s = blockIdx.x; s < cfg.S; s += gridDim.x) {
+        const int b = s / o.D, d = s % o.D;
+        const size_t base = (size_t)s * o.N;
+        double* rhs = w.b + base;
+        for (int c = threadIdx.x; c < o.N; c += BLOCK) rhs[c] = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
+        __syncthreads();
+        const SysView<double> sv = make_view<double>(o, b, d);
+        const int it = bicgstab<double, BLOCK, SWEEP>(
+            sv, rhs, w.x + base, (double*)w.r + base, (double*)w.rh + base, (double*)w.p + base,
+            (double*)w.v + base, (double*)w.ph + base, (double*)w.sh + base, (double*)w.t + base, cfg.maxit, cfg.tol,
+            cfg.reference_semantics != 0, lds);
+        if (threadIdx.x == 0) {
+            w.iters[s] += (it > cfg.maxit) ? cfg.maxit : it;
+            w.fails[s] += (it > cfg.maxit) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Mixed precision: FP64 RHS and residual, FP32 BiCGSTAB on the scaled residual equation,
+// x += ||r|| * y per refinement step, until ||b - A x|| / ||b|| < tol (FP64, on the GPU).
+template <int BLOCK, bool SWEEP>
+__global__ __launch_bounds__(BLOCK) void k_solve_mixed(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w,
+                                                      SolveCfg cfg, const double* __restrict__ ee_prev,
+                                                      const double* __restrict__ dens) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    for (int s = blockIdx.x; s < cfg.S; s += gridDim.x) {
+        const int b = s / o.D, d = s % o.D;
+        const size_t base = (size_t)s * o.N;
+        double* rhs = w.b + base;
+        double* x = w.x + base;
+        double* r64 = w.r64 + base;
+        double red[1] = {0};
+        for (int c = threadIdx.x; c < o.N; c += BLOCK) {
+            const double bc = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
+            rhs[c] = bc;
+            r64[c] = bc;
+            x[c] = 0;
+            red[0] += bc * bc;
+        }
+        block_sum<BLOCK, 1>(red, lds);
+        const double nb = sqrt(red[0]);
+        double nr = nb;
+        const SysView<float> sv32 = make_view<float>(o, b, d);
+        const SysView<double> sv64 = make_view<double>(o, b, d);
+        float* y = (float*)w.y + base;
+        float* r32 = (float*)w.r + base;
+        int total = 0, fail = 0;
+        for (int step = 0; step < cfg.refine_steps && nr > cfg.tol * nb && nr > 1e-300; step++) {
+            // y solves A y = r64 / ||r64|| in FP32 (scaled into FP32 range)
+            const double inv = 1.0 / nr;
+            float* fr = (float*)w.f + base;  // scaled residual as the FP32 right-hand side
+            for (int c = threadIdx.x; c < o.N; c += BLOCK) fr[c] = (float)(r64[c] * inv);
+            __syncthreads();
+            // FP32 inner tolerance: as deep as FP32 allows; refinement supplies the rest
+            const int it = bicgstab<float, BLOCK, SWEEP>(sv32, fr, y, r32, (float*)w.rh + base, (float*)w.p + base,
+                                                         (float*)w.v + base, (float*)w.ph + base,
+                                                         (float*)w.sh + base, (float*)w.t + base, cfg.maxit,
+                                                         1e-6, false, lds);
+            total += (it > cfg.maxit) ? cfg.maxit : it;
+            fail += (it > cfg.maxit) ? 1 : 0;
+            for (int c = threadIdx.x; c < o.N; c += BLOCK) x[c] += nr * (double)y[c];
+            __syncthreads();
+            // r64 = b - A x in FP64
+            red[0] = 0;
+            for (int c = threadIdx.x; c < o.N; c += BLOCK) {
+                double acc = diag_of(sv64, sv64.dg[c], c) * x[c];
+                double off = 0;
+                for (int k = 0; k < sv64.W; k++) {
+                    const int j = sv64.col[(size_t)k * sv64.N + c];
+                    if (j >= 0) off += sv64.val[(size_t)k * sv64.N + c] * x[j];
+                }
+                const double rc = rhs[c] - (acc + sv64.vt * off);
+                r64[c] = rc;
+                red[0] += rc * rc;
+            }
+            block_sum<BLOCK, 1>(red, lds);
+            nr = sqrt(red[0]);
+        }
+        if (threadIdx.x == 0) {
+            w.iters[s] += total;
+            w.fails[s] += fail + ((nr > cfg.tol * nb && nr > 1e-300) ? 1 : 0);
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Temperature update (StaticBTESolver.cpp:96-120) fused with the margin partial sums (:311-321).
+// One thread per cell; per-block partial of sum (e_curr - e_prev)^2.
+// ----------------------------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_temperature(int N, int D, Bands bands, const double* __restrict__ omega,
+                                                      const double* __restrict__ ee_curr,
+                                                      const double* __restrict__ ee_prev, double* __restrict__ Tband,
+                                                      double* __restrict__ T, double* __restrict__ dens,
+                                                      double* __restrict__ partial) {
+    __shared__ double lds[BLOCK / 64];
+    double mp[1] = {0};
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c < N) {
+        double rrn = 0, rnn = 0;
+        for (int b = 0; b < bands.B; b++) {
+            double e0 = 0;
+            const size_t base = (size_t)b * D * N + c;
+            for (int d = 0; d < D; d++) {
+                const double ec = ee_curr[base + (size_t)d * N];
+                const double ep = ee_prev[base + (size_t)d * N];
+                e0 += ec * omega[d];
+                const double df = ec - ep;
+                mp[0] += df * df;
+            }
+            const double tb = e0 / bands.C[b] + bands.T_ref;
+            Tband[(size_t)b * N + c] = tb;
+            rrn += bands.Lr[b] * tb;
+            rnn += bands.Lr[b];
+        }
+        const double tc = rrn / rnn;
+        T[c] = tc;
+        for (int b = 0; b < bands.B; b++) dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
+    }
+    block_sum<BLOCK, 1>(mp, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = mp[0];
+}
+
+// Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).
+__global__ void k_margin(int nparts, const double* __restrict__ partial, double scale, double* __restrict__ out) {
+    __shared__ double lds[4];
+    double s[1] = {0};
+    for (int i = threadIdx.x; i < nparts; i += 256) s[0] += partial[i];
+    block_sum<256, 1>(s, lds);
+    if (threadIdx.x == 0) out[0] = s[0] * scale;
+}
+
+// Heat flux per (BC, band), StaticBTESolver.cpp:323-370.  One work-group per pair.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Quad q, Bnd bd, Bands bands, int DM,
+                                               const double* __restrict__ ee, double* __restrict__ band_flux) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    const int k = blockIdx.x / bands.B, b = blockIdx.x % bands.B;
+    double red[2] = {0, 0};
+    for (int bf = threadIdx.x; bf < bd.n_bf; bf += BLOCK) {
+        const int c = bd.bf_cell[bf], f = bd.bf_face[bf];
+        if (g.nbr[(size_t)c * g.F + f] != bd.bci[k]) continue;
+        const double* n = g.normal + 3 * ((size_t)c * g.F + f);
+        const double A = g.area[(size_t)c * g.F + f];
+        double heat = 0;
+        for (int d = 0; d < q.D; d++) {
+            const double e = ee[((size_t)b * q.D + d) * g.N + c];
+            if (DM == 3 && g.dim == 3)
+                heat += e * dot_nc(q.dir + 3 * d, n) * A * q.omega[d];
+            else if (g.dim == 1)
+                heat += e * dot_nc(q.dir + 3 * d, n) * q.omega[d];
+            else
+                heat += e * dot_nc(q.S + 3 * d, n) * A;
+        }
+        red[0] += heat;
+        red[1] += A;
+    }
+    block_sum<BLOCK, 2>(red, lds);
+    if (threadIdx.x == 0) {
+        const double qv = (g.dim != 1) ? red[0] / red[1] * bands.v[b] : red[0] * bands.v[b];
+        band_flux[(size_t)k * bands.B + b] = qv;
+    }
+}
+
+}  // namespace bte

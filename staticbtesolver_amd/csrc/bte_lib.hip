@@ -1,0 +1,948 @@
+// bte_lib.hip -- host side of the C ABI (include/bte.h): device-resident state, sweep schedule,
+// outer source iteration with one hipGraph per iteration parity.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/bte.h"
+#include "bte_kernels.hpp"
+
+using namespace bte;
+
+namespace {
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 16));
+        if (e == hipSuccess) bytes = n;
+        return e;
+    }
+    template <class T>
+    T* as() const { return (T*)p; }
+};
+
+const char* kKernelNames[] = {"assemble", "sweep_schedule", "diffuse", "solve", "temperature", "margin", "flux"};
+enum { KA_ASM = 0, KA_SCHED, KA_DIFF, KA_SOLVE, KA_TEMP, KA_MARGIN, KA_FLUX, KA_COUNT };
+
+}  // namespace
+
+struct bte_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // problem (host copies)
+    int dim = 0, N = 0, F = 0, D = 0, B = 0, DM = 0, num_theta = 0, num_phi = 0;
+    double solid_angle = 4 * kPi, T_ref = 0;
+    std::vector<double> h_vol, h_area, h_normal, h_omega, h_dir, h_S, h_v, h_tau, h_C, h_Lr, h_bcT;
+    std::vector<int> h_nbr, h_bci, h_bct;
+    bool have_geo = false, have_quad = false, have_bands = false, have_bcs = false;
+    bool prepared = false;
+    // device: geometry / quadrature / bands
+    DBuf vol, area, normal, nbr, omega, dir, S, w_eff, bv, btau, bC, bLr, vt;
+    // boundary
+    DBuf bf_cell, bf_face, bfid, mirror, ediff;
+    int n_bf = 0;
+    bool has_diffuse = false;
+    // assembly + sweep schedule
+    int W = 0;
+    DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
+    DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32;
+    int max_levels = 0, lagged = 0;
+    bool have_sweep = false, have_fp32 = false;
+    // state + work
+    DBuf eeA, eeB, dens, T, Tband, partial, margin, bflux;
+    DBuf wb, wr64, wr, wrh, wp, wv, wph, wsh, wt, wy, wf, iters, fails;
+    size_t work_elems = 0;
+    int work_prec = -1;
+    double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
+    double* h_margin_pinned = nullptr;
+    // graphs
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_key = -1;
+    // stats
+    double ms[KA_COUNT] = {0};
+    long long calls[KA_COUNT] = {0};
+    double spmv_bytes = 0;
+    long long spmv_rows = 0;
+    int block_used = 0;
+    int tuned_block[2][2] = {{0, 0}, {0, 0}};  // [precision][precond]
+};
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                         \
+            return BTE_ERR_HIP;                                                                   \
+        }                                                                                         \
+    } while (0)
+
+#define SETERR(code, msg)   \
+    do {                    \
+        ctx->err = (msg);   \
+        return (code);      \
+    } while (0)
+
+namespace {
+
+template <class T>
+hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t s) {
+    hipError_t e = d.ensure(h.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (h.empty()) return hipSuccess;
+    return hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    return (int)std::max<long long>(1, std::min<long long>(g, 65536));
+}
+
+Geo geo_of(bte_ctx* c) {
+    return Geo{c->dim, c->N, c->F, c->vol.as<double>(), c->area.as<double>(), c->normal.as<double>(),
+               c->nbr.as<int>()};
+}
+Quad quad_of(bte_ctx* c) { return Quad{c->D, c->omega.as<double>(), c->dir.as<double>(), c->S.as<double>()}; }
+Bands bands_of(bte_ctx* c) {
+    return Bands{c->B, c->bv.as<double>(), c->btau.as<double>(), c->bC.as<double>(), c->bLr.as<double>(),
+                 c->solid_angle, c->T_ref};
+}
+Bnd bnd_of(bte_ctx* c) {
+    Bnd b{};
+    b.nbc = (int)c->h_bci.size();
+    for (int k = 0; k < b.nbc; k++) {
+        b.bci[k] = c->h_bci[k];
+        b.bct[k] = c->h_bct[k];
+        b.bcT[k] = c->h_bcT[k];
+    }
+    b.n_bf = c->n_bf;
+    b.bf_cell = c->bf_cell.as<int>();
+    b.bf_face = c->bf_face.as<int>();
+    b.bfid = c->bfid.as<int>();
+    b.mirror = c->mirror.as<int>();
+    b.ediff = c->ediff.as<double>();
+    b.has_diffuse = c->has_diffuse ? 1 : 0;
+    return b;
+}
+
+// Mirror direction of the specular types, StaticBTESolver.cpp:205-252 (-1 where the reference exits).
+int mirror_of(int dim, int num_phi, int type, int d) {
+    const int p = num_phi;
+    const int it = d / (4 * p), ip = d % (4 * p);
+    if (type == 31) {
+        if (dim == 2) return it * 4 * p + ((ip >= 2 * p) ? 6 * p - ip - 1 : 2 * p - ip - 1);
+        if (dim == 3) return it * 4 * p + (4 * p - ip - 1);
+        return -1;
+    }
+    if (type == 32) {
+        if (dim == 2) return it * 4 * p + (4 * p - ip - 1);
+        if (dim == 3) return it * 4 * p + (2 * p - (ip % (2 * p)) - 1) + (ip / (2 * p)) * 2 * p;
+        return -1;
+    }
+    if (type == 33) {
+        if (dim < 3) return -1;
+        return ip + (2 * p - it - 1) * 4 * p;  // theta index mirrored with num_phi (quirk of :248-251)
+    }
+    return -1;
+}
+
+// Downwind level schedule of one direction from the ELL columns (Kahn's algorithm, level by level).
+// Edges whose source is still unplaced when no cell is ready are lagged (dropped from the sweep).
+void level_schedule(int N, int W, const int* col /* [W*N] of this direction */, std::vector<int>& order,
+                    std::vector<int>& lptr, std::vector<int>& scol, int& lagged) {
+    std::vector<int> indeg(N, 0), outcnt(N + 1, 0);
+    for (int k = 0; k < W; k++)
+        for (int c = 0; c < N; c++) {
+            const int j = col[(size_t)k * N + c];
+            if (j >= 0) {
+                indeg[c]++;
+                outcnt[j + 1]++;
+            }
+        }
+    for (int i = 0; i < N; i++) outcnt[i + 1] += outcnt[i];
+    std::vector<int> out(outcnt[N]), fill(outcnt.begin(), outcnt.end() - 1);
+    for (int k = 0; k < W; k++)
+        for (int c = 0; c < N; c++) {
+            const int j = col[(size_t)k * N + c];
+            if (j >= 0) out[fill[j]++] = c;
+        }
+    std::vector<char> placed(N, 0), lag((size_t)W * N, 0);
+    order.clear();
+    order.reserve(N);
+    lptr.assign(1, 0);
+    std::vector<int> frontier, next;
+    for (int c = 0; c < N; c++)
+        if (indeg[c] == 0) frontier.push_back(c);
+    int scan = 0;
+    lagged = 0;
+    while ((int)order.size() < N) {
+        if (frontier.empty()) {
+            while (placed[scan]) scan++;
+            const int pick = scan;
+            for (int k = 0; k < W; k++) {
+                const int j = col[(size_t)k * N + pick];
+                if (j >= 0 && !placed[j]) {
+                    lag[(size_t)k * N + pick] = 1;
+                    lagged++;
+                }
+            }
+            frontier.push_back(pick);
+        }
+        for (int c : frontier) {
+            placed[c] = 1;
+            order.push_back(c);
+        }
+        lptr.push_back((int)order.size());
+        next.clear();
+        for (int c : frontier)
+            for (int e = outcnt[c]; e < outcnt[c + 1]; e++) {
+                const int o = out[e];
+                if (!placed[o] && --indeg[o] == 0) next.push_back(o);
+            }
+        frontier.swap(next);
+    }
+    scol.assign((size_t)W * N, -1);
+    for (int i = 0; i < N; i++) {
+        const int c = order[i];
+        for (int k = 0; k < W; k++)
+            scol[(size_t)k * N + i] = lag[(size_t)k * N + c] ? -1 : col[(size_t)k * N + c];
+    }
+}
+
+struct Timer {
+    bte_ctx* ctx;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on;
+    Timer(bte_ctx* c, int k, bool enabled) : ctx(c), kind(k), on(enabled) {
+        if (on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~Timer() {
+        if (on) {
+            (void)hipEventRecord(b, ctx->stream);
+            (void)hipEventSynchronize(b);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, a, b);
+            ctx->ms[kind] += ms;
+            ctx->calls[kind]++;
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    }
+};
+
+void drop_graphs(bte_ctx* ctx) {
+    for (auto& g : ctx->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    ctx->graph_key = -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+void bte_default_opts(bte_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->precision = BTE_PREC_FP64;
+    o->precond = BTE_PC_SWEEP;
+    o->max_iter = 10000;
+    o->stop_margin = 1e-5;
+    o->inner_tol = 1e-9;
+    o->inner_maxit = 1000;
+    o->refine_steps = 4;
+    o->block = 0;
+    o->use_graph = 1;
+    o->verbose = 0;
+    o->timing = 0;
+    o->learned = 0;
+    o->learned_every = 8;
+}
+
+int bte_ctx_create(int device, bte_ctx** out) {
+    if (!out) return BTE_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return BTE_ERR_NODEVICE;
+    bte_ctx* ctx = new bte_ctx();
+    if (device < 0) (void)hipGetDevice(&device);
+    if (device >= n) {
+        delete ctx;
+        return BTE_ERR_ARG;
+    }
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_margin_pinned, sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        delete ctx;
+        return BTE_ERR_HIP;
+    }
+    *out = ctx;
+    return BTE_OK;
+}
+
+void bte_ctx_destroy(bte_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_graphs(ctx);
+    if (ctx->h_margin_pinned) (void)hipHostFree(ctx->h_margin_pinned);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* bte_last_error(const bte_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int bte_set_geometry(bte_ctx* ctx, int dim, int n_cell, int n_face, const double* volume, const double* face_area,
+                     const double* face_normal, const int* neighbor) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (dim < 1 || dim > 3 || n_face != dim + 1 || n_cell <= 0 || !volume || !face_area || !face_normal || !neighbor)
+        SETERR(BTE_ERR_ARG, "bte_set_geometry: need dim in {1,2,3}, n_face = dim+1, n_cell > 0 and all arrays");
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->dim = dim;
+    ctx->N = n_cell;
+    ctx->F = n_face;
+    const size_t NF = (size_t)n_cell * n_face;
+    ctx->h_vol.assign(volume, volume + n_cell);
+    ctx->h_area.assign(face_area, face_area + NF);
+    ctx->h_normal.assign(face_normal, face_normal + 3 * NF);
+    ctx->h_nbr.assign(neighbor, neighbor + NF);
+    for (size_t i = 0; i < NF; i++)
+        if (ctx->h_nbr[i] >= n_cell) SETERR(BTE_ERR_MESH, "bte_set_geometry: neighbour index out of range");
+    HIPCHK(upload(ctx->vol, ctx->h_vol, ctx->stream));
+    HIPCHK(upload(ctx->area, ctx->h_area, ctx->stream));
+    HIPCHK(upload(ctx->normal, ctx->h_normal, ctx->stream));
+    HIPCHK(upload(ctx->nbr, ctx->h_nbr, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_geo = true;
+    ctx->prepared = false;
+    drop_graphs(ctx);
+    return BTE_OK;
+}
+
+int bte_set_quadrature(bte_ctx* ctx, int DM, int n_dir, const double* omega, const double* dir, const double* S,
+                       int num_theta, int num_phi, double solid_angle, double T_ref) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (n_dir <= 0 || !omega || !dir || !S || (DM != 2 && DM != 3) || num_phi <= 0)
+        SETERR(BTE_ERR_ARG, "bte_set_quadrature: need n_dir > 0, DM in {2,3}, num_phi > 0 and all arrays");
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->DM = DM;
+    ctx->D = n_dir;
+    ctx->num_theta = num_theta;
+    ctx->num_phi = num_phi;
+    ctx->solid_angle = solid_angle;
+    ctx->T_ref = T_ref;
+    ctx->h_omega.assign(omega, omega + n_dir);
+    ctx->h_dir.assign(dir, dir + 3 * n_dir);
+    ctx->h_S.assign(S, S + 3 * n_dir);
+    HIPCHK(upload(ctx->omega, ctx->h_omega, ctx->stream));
+    HIPCHK(upload(ctx->dir, ctx->h_dir, ctx->stream));
+    HIPCHK(upload(ctx->S, ctx->h_S, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_quad = true;
+    ctx->prepared = false;
+    drop_graphs(ctx);
+    return BTE_OK;
+}
+
+int bte_set_bands(bte_ctx* ctx, int n_band, const double* v, const double* tau, const double* C, const double* Lr) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (n_band <= 0 || !v || !tau || !C || !Lr) SETERR(BTE_ERR_ARG, "bte_set_bands: need n_band > 0 and all arrays");
+    HIPCHK(hipSetDevice(ctx->device));
+    ctx->B = n_band;
+    ctx->h_v.assign(v, v + n_band);
+    ctx->h_tau.assign(tau, tau + n_band);
+    ctx->h_C.assign(C, C + n_band);
+    ctx->h_Lr.assign(Lr, Lr + n_band);
+    std::vector<double> vt(n_band);
+    for (int b = 0; b < n_band; b++) vt[b] = v[b] * tau[b];
+    HIPCHK(upload(ctx->bv, ctx->h_v, ctx->stream));
+    HIPCHK(upload(ctx->btau, ctx->h_tau, ctx->stream));
+    HIPCHK(upload(ctx->bC, ctx->h_C, ctx->stream));
+    HIPCHK(upload(ctx->bLr, ctx->h_Lr, ctx->stream));
+    HIPCHK(upload(ctx->vt, vt, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_bands = true;
+    drop_graphs(ctx);
+    return BTE_OK;
+}
+
+int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const double* T) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (n_bc < 0 || n_bc > kMaxBC || (n_bc > 0 && (!index || !type || !T)))
+        SETERR(BTE_ERR_ARG, "bte_set_bcs: 0 <= n_bc <= 32 and all arrays");
+    for (int k = 0; k < n_bc; k++)
+        if (index[k] >= 0) SETERR(BTE_ERR_BC, "bte_set_bcs: boundary index must be -k-1 with k >= 0");
+    ctx->h_bci.assign(index, index + n_bc);
+    ctx->h_bct.assign(type, type + n_bc);
+    ctx->h_bcT.assign(T, T + n_bc);
+    ctx->have_bcs = true;
+    ctx->prepared = false;
+    drop_graphs(ctx);
+    return BTE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One-time preparation: boundary tables, assembly, sweep schedule, FP32 copies.
+int prepare(bte_ctx* ctx, bool timing) {
+    if (!ctx->have_geo || !ctx->have_quad || !ctx->have_bands || !ctx->have_bcs)
+        SETERR(BTE_ERR_STATE, "bte_solve: geometry, quadrature, bands and boundary conditions must be set first");
+    if (ctx->prepared) return BTE_OK;
+    const int N = ctx->N, F = ctx->F, D = ctx->D;
+    // boundary faces + validation of the BC types they reference (StaticBTESolver.cpp:173-257)
+    std::vector<int> bfc, bff, bfid((size_t)N * F, -1);
+    ctx->has_diffuse = false;
+    for (int c = 0; c < N; c++)
+        for (int f = 0; f < F; f++) {
+            const int nb = ctx->h_nbr[(size_t)c * F + f];
+            if (nb >= 0) continue;
+            bfid[(size_t)c * F + f] = (int)bfc.size();
+            bfc.push_back(c);
+            bff.push_back(f);
+            for (size_t k = 0; k < ctx->h_bci.size(); k++) {
+                if (ctx->h_bci[k] != nb) continue;
+                const int t = ctx->h_bct[k];
+                if (t == 2) ctx->has_diffuse = true;
+                if (t == 1 || t == 2) continue;
+                if (t == 31 || t == 32 || t == 33) {
+                    for (int d = 0; d < D; d++) {
+                        const int m = mirror_of(ctx->dim, ctx->num_phi, t, d);
+                        if (m < 0 || m >= D)
+                            SETERR(BTE_ERR_BC, "boundary condition type " + std::to_string(t) +
+                                                   " not supported for this mesh dimension / quadrature");
+                    }
+                    continue;
+                }
+                SETERR(BTE_ERR_BC, "Boundary Condition Type " + std::to_string(t) + " not supported.");
+            }
+        }
+    ctx->n_bf = (int)bfc.size();
+    std::vector<int> mir(3 * (size_t)D, -1);
+    for (int t = 0; t < 3; t++)
+        for (int d = 0; d < D; d++) mir[(size_t)t * D + d] = mirror_of(ctx->dim, ctx->num_phi, 31 + t, d);
+    HIPCHK(upload(ctx->bf_cell, bfc, ctx->stream));
+    HIPCHK(upload(ctx->bf_face, bff, ctx->stream));
+    HIPCHK(upload(ctx->bfid, bfid, ctx->stream));
+    HIPCHK(upload(ctx->mirror, mir, ctx->stream));
+    HIPCHK(ctx->ediff.ensure(sizeof(double) * std::max<size_t>(1, (size_t)ctx->B * ctx->n_bf)));
+    // effective angular weight of the volume term: omega_d in 2D/3D, 1 in 1D (StaticBTESolver.cpp:143-147)
+    std::vector<double> weff(ctx->h_omega);
+    if (ctx->dim == 1) std::fill(weff.begin(), weff.end(), 1.0);
+    HIPCHK(upload(ctx->w_eff, weff, ctx->stream));
+
+    // assembly
+    const int W = std::max(1, F - 1);
+    ctx->W = W;
+    {
+        Timer tm(ctx, KA_ASM, timing);
+        HIPCHK(ctx->dg.ensure(sizeof(double) * (size_t)D * N));
+        HIPCHK(ctx->ell_col.ensure(sizeof(int) * (size_t)D * W * N));
+        HIPCHK(ctx->ell_val.ensure(sizeof(double) * (size_t)D * W * N));
+        HIPCHK(ctx->overflow.ensure(sizeof(int)));
+        HIPCHK(hipMemsetAsync(ctx->overflow.p, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(k_assemble, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream, geo_of(ctx),
+                           quad_of(ctx), W, ctx->dg.as<double>(), ctx->ell_col.as<int>(), ctx->ell_val.as<double>(),
+                           ctx->overflow.as<int>());
+        HIPCHK(hipGetLastError());
+    }
+    int ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, ctx->overflow.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ovf) SETERR(BTE_ERR_MESH, "mesh has cells with more incoming interior faces than F-1 (degenerate cells)");
+
+    // sweep schedule: levels per direction on the host (threads over directions)
+    {
+        Timer tm(ctx, KA_SCHED, timing);
+        std::vector<int> hcol((size_t)D * W * N);
+        HIPCHK(hipMemcpyAsync(hcol.data(), ctx->ell_col.p, hcol.size() * sizeof(int), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::vector<std::vector<int>> order(D), lptr(D), scol(D);
+        std::vector<int> lag(D, 0);
+        unsigned nth = std::max(1u, std::min<unsigned>({std::thread::hardware_concurrency(), 16u, (unsigned)D}));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nth; t++)
+            pool.emplace_back([&, t]() {
+                for (int d = (int)t; d < D; d += (int)nth)
+                    level_schedule(N, W, hcol.data() + (size_t)d * W * N, order[d], lptr[d], scol[d], lag[d]);
+            });
+        for (auto& th : pool) th.join();
+        std::vector<int> off(D), cnt(D), lp_all, perm_all((size_t)D * N), scol_all((size_t)D * W * N);
+        ctx->max_levels = 0;
+        ctx->lagged = 0;
+        for (int d = 0; d < D; d++) {
+            off[d] = (int)lp_all.size();
+            cnt[d] = (int)lptr[d].size() - 1;
+            ctx->max_levels = std::max(ctx->max_levels, cnt[d]);
+            ctx->lagged += lag[d];
+            lp_all.insert(lp_all.end(), lptr[d].begin(), lptr[d].end());
+            std::copy(order[d].begin(), order[d].end(), perm_all.begin() + (size_t)d * N);
+            std::copy(scol[d].begin(), scol[d].end(), scol_all.begin() + (size_t)d * W * N);
+        }
+        HIPCHK(upload(ctx->lvl_off, off, ctx->stream));
+        HIPCHK(upload(ctx->lvl_cnt, cnt, ctx->stream));
+        HIPCHK(upload(ctx->lvl_ptr, lp_all, ctx->stream));
+        HIPCHK(upload(ctx->perm, perm_all, ctx->stream));
+        HIPCHK(upload(ctx->swp_col, scol_all, ctx->stream));
+        HIPCHK(ctx->swp_dg.ensure(sizeof(double) * (size_t)D * N));
+        HIPCHK(ctx->swp_val.ensure(sizeof(double) * (size_t)D * W * N));
+        hipLaunchKernelGGL(k_gather_sweep<double>, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream, N,
+                           D, W, ctx->perm.as<int>(), ctx->dg.as<double>(), ctx->ell_val.as<double>(),
+                           ctx->swp_dg.as<double>(), ctx->swp_val.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    ctx->have_sweep = true;
+    ctx->have_fp32 = false;
+    ctx->prepared = true;
+    drop_graphs(ctx);
+    return BTE_OK;
+}
+
+int ensure_fp32(bte_ctx* ctx) {
+    if (ctx->have_fp32) return BTE_OK;
+    const int N = ctx->N, D = ctx->D, W = ctx->W;
+    HIPCHK(ctx->dg32.ensure(sizeof(float) * (size_t)D * N));
+    HIPCHK(ctx->ell_val32.ensure(sizeof(float) * (size_t)D * W * N));
+    HIPCHK(ctx->swp_dg32.ensure(sizeof(float) * (size_t)D * N));
+    HIPCHK(ctx->swp_val32.ensure(sizeof(float) * (size_t)D * W * N));
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream,
+                       (long long)D * N, ctx->dg.as<double>(), ctx->dg32.as<float>());
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for((long long)D * W * N, 256)), dim3(256), 0, ctx->stream,
+                       (long long)D * W * N, ctx->ell_val.as<double>(), ctx->ell_val32.as<float>());
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream,
+                       (long long)D * N, ctx->swp_dg.as<double>(), ctx->swp_dg32.as<float>());
+    hipLaunchKernelGGL(k_convert<float>, dim3(grid_for((long long)D * W * N, 256)), dim3(256), 0, ctx->stream,
+                       (long long)D * W * N, ctx->swp_val.as<double>(), ctx->swp_val32.as<float>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->have_fp32 = true;
+    return BTE_OK;
+}
+
+int ensure_work(bte_ctx* ctx, int precision) {
+    const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
+    const int S = ctx->B * ctx->D;
+    if (ctx->work_elems == SN && ctx->work_prec == precision) return BTE_OK;
+    const size_t ks = (precision == BTE_PREC_MIXED) ? sizeof(float) : sizeof(double);
+    HIPCHK(ctx->eeA.ensure(sizeof(double) * SN));
+    HIPCHK(ctx->eeB.ensure(sizeof(double) * SN));
+    HIPCHK(ctx->wb.ensure(sizeof(double) * SN));
+    if (precision == BTE_PREC_MIXED) {
+        HIPCHK(ctx->wr64.ensure(sizeof(double) * SN));
+        HIPCHK(ctx->wy.ensure(ks * SN));
+        HIPCHK(ctx->wf.ensure(ks * SN));
+    }
+    for (DBuf* b : {&ctx->wr, &ctx->wrh, &ctx->wp, &ctx->wv, &ctx->wph, &ctx->wsh, &ctx->wt}) HIPCHK(b->ensure(ks * SN));
+    HIPCHK(ctx->iters.ensure(sizeof(int) * S));
+    HIPCHK(ctx->fails.ensure(sizeof(int) * S));
+    HIPCHK(ctx->dens.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
+    HIPCHK(ctx->T.ensure(sizeof(double) * ctx->N));
+    HIPCHK(ctx->Tband.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
+    HIPCHK(ctx->partial.ensure(sizeof(double) * grid_for(ctx->N, 256)));
+    HIPCHK(ctx->margin.ensure(sizeof(double) * 2));
+    HIPCHK(ctx->bflux.ensure(sizeof(double) * std::max<size_t>(1, ctx->h_bci.size() * ctx->B)));
+    ctx->work_elems = SN;
+    ctx->work_prec = precision;
+    drop_graphs(ctx);
+    return BTE_OK;
+}
+
+Ops ops_of(bte_ctx* ctx) {
+    Ops o{};
+    o.N = ctx->N;
+    o.D = ctx->D;
+    o.W = ctx->W;
+    o.vol = ctx->vol.as<double>();
+    o.omega = ctx->w_eff.as<double>();
+    o.vt = ctx->vt.as<double>();
+    o.dg = ctx->dg.as<double>();
+    o.ell_col = ctx->ell_col.as<int>();
+    o.ell_val = ctx->ell_val.as<double>();
+    o.ell_val32 = ctx->ell_val32.as<float>();
+    o.dg32 = ctx->dg32.as<float>();
+    o.lvl_off = ctx->lvl_off.as<int>();
+    o.lvl_cnt = ctx->lvl_cnt.as<int>();
+    o.lvl_ptr = ctx->lvl_ptr.as<int>();
+    o.perm = ctx->perm.as<int>();
+    o.swp_col = ctx->swp_col.as<int>();
+    o.swp_val = ctx->swp_val.as<double>();
+    o.swp_dg = ctx->swp_dg.as<double>();
+    o.swp_val32 = ctx->swp_val32.as<float>();
+    o.swp_dg32 = ctx->swp_dg32.as<float>();
+    return o;
+}
+
+Work work_of(bte_ctx* ctx, double* ee_curr) {
+    Work w{};
+    w.x = ee_curr;
+    w.b = ctx->wb.as<double>();
+    w.r64 = ctx->wr64.as<double>();
+    w.r = ctx->wr.p;
+    w.rh = ctx->wrh.p;
+    w.p = ctx->wp.p;
+    w.v = ctx->wv.p;
+    w.ph = ctx->wph.p;
+    w.sh = ctx->wsh.p;
+    w.t = ctx->wt.p;
+    w.y = ctx->wy.p;
+    w.f = ctx->wf.p;
+    w.iters = ctx->iters.as<int>();
+    w.fails = ctx->fails.as<int>();
+    return w;
+}
+
+template <int BLOCK>
+void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int S, const double* ee_prev, double* ee_curr, const double* dens) {
+    SolveCfg cfg{S, o->inner_maxit, o->inner_tol, std::max(1, o->refine_steps), o->precond == BTE_PC_JACOBI ? 1 : 0};
+    const Geo g = geo_of(ctx);
+    const Quad q = quad_of(ctx);
+    const Bnd bd = bnd_of(ctx);
+    const Bands bn = bands_of(ctx);
+    const Ops op = ops_of(ctx);
+    const Work w = work_of(ctx, ee_curr);
+    dim3 grid(S), blk(BLOCK);
+    if (o->precision == BTE_PREC_MIXED) {
+        if (o->precond == BTE_PC_JACOBI)
+            hipLaunchKernelGGL((k_solve_mixed<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
+                               dens);
+        else
+            hipLaunchKernelGGL((k_solve_mixed<BLOCK, true>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
+                               dens);
+    } else {
+        if (o->precond == BTE_PC_JACOBI)
+            hipLaunchKernelGGL((k_solve64<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
+                               dens);
+        else
+            hipLaunchKernelGGL((k_solve64<BLOCK, true>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev, dens);
+    }
+}
+
+void launch_solve(bte_ctx* ctx, const bte_opts* o, int block, const double* ee_prev, double* ee_curr,
+                  const double* dens) {
+    const int S = ctx->B * ctx->D;
+    switch (block) {
+        case 64: launch_solve_b<64>(ctx, o, S, ee_prev, ee_curr, dens); break;
+        case 128: launch_solve_b<128>(ctx, o, S, ee_prev, ee_curr, dens); break;
+        case 512: launch_solve_b<512>(ctx, o, S, ee_prev, ee_curr, dens); break;
+        case 1024: launch_solve_b<1024>(ctx, o, S, ee_prev, ee_curr, dens); break;
+        default: launch_solve_b<256>(ctx, o, S, ee_prev, ee_curr, dens); break;
+    }
+}
+
+// The kernels of one outer iteration (diffuse BC data, inner solves, T update, margin).
+int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, int block, const double* ee_prev, double* ee_curr,
+                      bool timing) {
+    const Geo g = geo_of(ctx);
+    const Quad q = quad_of(ctx);
+    const Bnd bd = bnd_of(ctx);
+    const Bands bn = bands_of(ctx);
+    if (ctx->has_diffuse) {
+        Timer tm(ctx, KA_DIFF, timing);
+        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream, g,
+                           q, bd, ctx->B, ee_prev, ctx->ediff.as<double>());
+    }
+    {
+        Timer tm(ctx, KA_SOLVE, timing);
+        launch_solve(ctx, o, block, ee_prev, ee_curr, ctx->dens.as<double>());
+    }
+    const int nblk = grid_for(ctx->N, 256);
+    {
+        Timer tm(ctx, KA_TEMP, timing);
+        hipLaunchKernelGGL(k_temperature<256>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
+                           ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), ctx->T.as<double>(),
+                           ctx->dens.as<double>(), ctx->partial.as<double>());
+    }
+    {
+        Timer tm(ctx, KA_MARGIN, timing);
+        hipLaunchKernelGGL(k_margin, dim3(1), dim3(256), 0, ctx->stream, nblk, ctx->partial.as<double>(),
+                           1.0 / ctx->B / ctx->N, ctx->margin.as<double>());
+    }
+    HIPCHK(hipGetLastError());
+    return BTE_OK;
+}
+
+int pick_block(bte_ctx* ctx, const bte_opts* o) {
+    if (o->block == 64 || o->block == 128 || o->block == 256 || o->block == 512 || o->block == 1024) return o->block;
+    // heuristic default; the autotuner in bte_autotune() refines it
+    int& tuned = ctx->tuned_block[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI];
+    return tuned ? tuned : 256;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
+    if (!ctx || !res) return BTE_ERR_ARG;
+    bte_opts o;
+    if (opts)
+        o = *opts;
+    else
+        bte_default_opts(&o);
+    if (o.precision != BTE_PREC_FP64 && o.precision != BTE_PREC_MIXED) SETERR(BTE_ERR_ARG, "unknown precision");
+    if (o.precond != BTE_PC_SWEEP && o.precond != BTE_PC_JACOBI) SETERR(BTE_ERR_ARG, "unknown preconditioner");
+    if (o.inner_maxit <= 0 || o.inner_tol <= 0) SETERR(BTE_ERR_ARG, "inner_maxit and inner_tol must be positive");
+    HIPCHK(hipSetDevice(ctx->device));
+    const bool timing = o.timing != 0;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = prepare(ctx, timing);
+    if (rc) return rc;
+    if (o.precision == BTE_PREC_MIXED && (rc = ensure_fp32(ctx))) return rc;
+    if ((rc = ensure_work(ctx, o.precision))) return rc;
+    const int N = ctx->N, B = ctx->B, D = ctx->D, S = B * D;
+    const size_t SN = (size_t)S * N;
+    const int block = pick_block(ctx, &o);
+    ctx->block_used = block;
+    HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->iters.p, 0, sizeof(int) * S, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    res->setup_seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (o.verbose)
+        std::printf("N_cell: %d\nN_dir: %d\nN_band: %d\nnum_theta: %d\nnum_phi: %d\n\n", N, D, B, ctx->num_theta,
+                    ctx->num_phi);
+    double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
+    // graph key: everything that changes the captured launches
+    const int key = (o.precision << 0) | (o.precond << 1) | (block << 2);
+    const bool use_graph = o.use_graph && !timing;
+    if (use_graph && ctx->graph_key != key) {
+        drop_graphs(ctx);
+        for (int par = 0; par < 2; par++) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_iteration(ctx, &o, block, buf[par], buf[1 - par], false);
+            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) return rc;
+            HIPCHK(ec);
+            HIPCHK(hipGraphInstantiate(&ctx->graph[par], g, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(g));
+        }
+        ctx->graph_key = key;
+    }
+    int it = 0;
+    double m = 0;
+    bool conv = false;
+    for (; it < o.max_iter; it++) {
+        // iteration parity p: ee_prev = buf[p], ee_curr = buf[1-p]; iteration 0 reads the zero buffer
+        const int par = it & 1;
+        if (use_graph) {
+            HIPCHK(hipGraphLaunch(ctx->graph[par], ctx->stream));
+        } else {
+            rc = enqueue_iteration(ctx, &o, block, buf[par], buf[1 - par], timing);
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, ctx->margin.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        m = *ctx->h_margin_pinned;
+        if (res->margins && it < res->margins_capacity) res->margins[it] = m;
+        if (o.verbose) {
+            std::printf("----------------------------------------------------------------------------------\n");
+            std::printf("Iteration #%d\t Margin per band per cell: %g\n", it, m);
+        }
+        ctx->ee_final = buf[1 - par];
+        if (m <= o.stop_margin) {
+            conv = true;
+            it++;
+            if (o.verbose) std::printf("\n");
+            break;
+        }
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    res->solve_seconds = std::chrono::duration<double>(t2 - t1).count();
+    res->iterations = it;
+    res->converged = conv ? 1 : 0;
+    // heat flux of the last iterate
+    const int nbc = (int)ctx->h_bci.size();
+    if (nbc > 0 && it > 0) {
+        Timer tm(ctx, KA_FLUX, timing);
+        hipLaunchKernelGGL(k_flux<256>, dim3(nbc * B), dim3(256), 0, ctx->stream, geo_of(ctx), quad_of(ctx), bnd_of(ctx),
+                           bands_of(ctx), ctx->DM, ctx->ee_final, ctx->bflux.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    std::vector<int> hit(S), hfail(S);
+    HIPCHK(hipMemcpyAsync(hit.data(), ctx->iters.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hfail.data(), ctx->fails.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> hT(N), hTb((size_t)B * N), hbf((size_t)nbc * B);
+    HIPCHK(hipMemcpyAsync(hT.data(), ctx->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hTb.data(), ctx->Tband.p, sizeof(double) * B * N, hipMemcpyDeviceToHost, ctx->stream));
+    if (nbc > 0)
+        HIPCHK(hipMemcpyAsync(hbf.data(), ctx->bflux.p, sizeof(double) * nbc * B, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    res->inner_iters = 0;
+    res->inner_failures = 0;
+    for (int s = 0; s < S; s++) {
+        res->inner_iters += hit[s];
+        res->inner_failures += hfail[s];
+    }
+    if (res->T) std::copy(hT.begin(), hT.end(), res->T);
+    if (res->T_band)
+        for (int c = 0; c < N; c++)
+            for (int b = 0; b < B; b++) res->T_band[(size_t)c * B + b] = hTb[(size_t)b * N + c];
+    if (res->bc_band_flux) std::copy(hbf.begin(), hbf.end(), res->bc_band_flux);
+    if (res->bc_flux)
+        for (int k = 0; k < nbc; k++) {
+            double s = 0;
+            for (int b = 0; b < B; b++) s += hbf[(size_t)k * B + b];
+            res->bc_flux[k] = s;
+        }
+    // SpMV accounting: rows processed = inner iterations x 2 products (sweep path: 1 per early exit)
+    ctx->spmv_rows = res->inner_iters * 2LL * N;
+    if (o.verbose)
+        for (int k = 0; k < nbc; k++) {
+            double s = 0;
+            for (int b = 0; b < B; b++) s += hbf[(size_t)k * B + b];
+            std::printf("heat flux of Boundary Condition #%d: %g\n", -1 - ctx->h_bci[k], s);
+        }
+    if (res->inner_failures > 0) {
+        ctx->err = "some inner solves reached inner_maxit";
+        return BTE_ERR_NOCONV;
+    }
+    return BTE_OK;
+}
+
+int bte_get_intensity(bte_ctx* ctx, double* ee) {
+    if (!ctx || !ee) return BTE_ERR_ARG;
+    if (!ctx->ee_final) SETERR(BTE_ERR_STATE, "no solve has run");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpy(ee, ctx->ee_final, sizeof(double) * (size_t)ctx->B * ctx->D * ctx->N, hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_get_stats(bte_ctx* ctx, bte_stats* st) {
+    if (!ctx || !st) return BTE_ERR_ARG;
+    std::memset(st, 0, sizeof(*st));
+    st->n_kernels = KA_COUNT;
+    for (int k = 0; k < KA_COUNT; k++) {
+        st->names[k] = kKernelNames[k];
+        st->ms[k] = ctx->ms[k];
+        st->calls[k] = ctx->calls[k];
+    }
+    st->spmv_rows = ctx->spmv_rows;
+    st->ell_width = ctx->W;
+    st->max_levels = ctx->max_levels;
+    st->lagged_edges = ctx->lagged;
+    st->block = ctx->block_used;
+    return BTE_OK;
+}
+
+int bte_debug_assemble(bte_ctx* ctx, int band, int dir, double* diag, double* off) {
+    if (!ctx || !diag || !off) return BTE_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = prepare(ctx, false);
+    if (rc) return rc;
+    if (band < 0 || band >= ctx->B || dir < 0 || dir >= ctx->D) SETERR(BTE_ERR_ARG, "band/dir out of range");
+    const int N = ctx->N, F = ctx->F, W = ctx->W;
+    std::vector<double> dg(N), val((size_t)W * N);
+    std::vector<int> col((size_t)W * N);
+    HIPCHK(hipMemcpy(dg.data(), ctx->dg.as<double>() + (size_t)dir * N, sizeof(double) * N, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(val.data(), ctx->ell_val.as<double>() + (size_t)dir * W * N, sizeof(double) * W * N,
+                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(col.data(), ctx->ell_col.as<int>() + (size_t)dir * W * N, sizeof(int) * W * N,
+                     hipMemcpyDeviceToHost));
+    const double vt = ctx->h_v[band] * ctx->h_tau[band];
+    const double w = (ctx->dim > 1) ? ctx->h_omega[dir] : 1.0;
+    for (int c = 0; c < N; c++) {
+        diag[c] = vt * dg[c] + w * ctx->h_vol[c];
+        for (int f = 0; f < F; f++) off[(size_t)c * F + f] = 0;
+        for (int k = 0; k < W; k++) {
+            const int j = col[(size_t)k * N + c];
+            if (j < 0) continue;
+            for (int f = 0; f < F; f++)
+                if (ctx->h_nbr[(size_t)c * F + f] == j) {
+                    off[(size_t)c * F + f] = vt * val[(size_t)k * N + c];
+                    break;
+                }
+        }
+    }
+    return BTE_OK;
+}
+
+static int upload_state(bte_ctx* ctx, const double* ee_prev, const double* density) {
+    const int N = ctx->N, B = ctx->B;
+    const size_t SN = (size_t)B * ctx->D * N;
+    HIPCHK(hipMemcpy(ctx->eeB.p, ee_prev, sizeof(double) * SN, hipMemcpyHostToDevice));
+    std::vector<double> dT((size_t)B * N);
+    for (int c = 0; c < N; c++)
+        for (int b = 0; b < B; b++) dT[(size_t)b * N + c] = density[(size_t)c * B + b];
+    HIPCHK(hipMemcpy(ctx->dens.p, dT.data(), sizeof(double) * dT.size(), hipMemcpyHostToDevice));
+    return BTE_OK;
+}
+
+int bte_debug_rhs(bte_ctx* ctx, const double* ee_prev, const double* density, double* rhs) {
+    if (!ctx || !ee_prev || !density || !rhs) return BTE_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = prepare(ctx, false);
+    if (rc) return rc;
+    if ((rc = ensure_work(ctx, BTE_PREC_FP64))) return rc;
+    if ((rc = upload_state(ctx, ee_prev, density))) return rc;
+    const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
+    if (ctx->has_diffuse)
+        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+                           geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeB.as<double>(), ctx->ediff.as<double>());
+    hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)SN, 256)), dim3(256), 0, ctx->stream, geo_of(ctx), quad_of(ctx),
+                       bnd_of(ctx), bands_of(ctx), ctx->eeB.as<double>(), ctx->dens.as<double>(), ctx->wb.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rhs, ctx->wb.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BTE_OK;
+}
+
+int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, const double* density, double* x,
+                    int* iters) {
+    if (!ctx || !ee_prev || !density || !x) return BTE_ERR_ARG;
+    bte_opts o;
+    if (opts)
+        o = *opts;
+    else
+        bte_default_opts(&o);
+    HIPCHK(hipSetDevice(ctx->device));
+    int rc = prepare(ctx, false);
+    if (rc) return rc;
+    if (o.precision == BTE_PREC_MIXED && (rc = ensure_fp32(ctx))) return rc;
+    if ((rc = ensure_work(ctx, o.precision))) return rc;
+    if ((rc = upload_state(ctx, ee_prev, density))) return rc;
+    const int S = ctx->B * ctx->D;
+    const size_t SN = (size_t)S * ctx->N;
+    HIPCHK(hipMemsetAsync(ctx->iters.p, 0, sizeof(int) * S, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
+    if (ctx->has_diffuse)
+        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+                           geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeB.as<double>(), ctx->ediff.as<double>());
+    launch_solve(ctx, &o, pick_block(ctx, &o), ctx->eeB.as<double>(), ctx->eeA.as<double>(), ctx->dens.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(x, ctx->eeA.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
+    if (iters) HIPCHK(hipMemcpyAsync(iters, ctx->iters.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BTE_OK;
+}
+
+}  // extern "C"

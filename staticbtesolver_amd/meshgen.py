@@ -1,0 +1,220 @@
+"""Seeded inputs in the reference's file formats: band files (BTEBand.cpp), boundary-condition files
+(BTEBoundaryCondition.cpp) and COMSOL .mphtxt meshes (BTEMesh.cpp).
+
+Meshes use COMSOL's boundary entity numbering as found in the reference's meshes:
+  2D unit square: 0 = {x=0}, 1 = {y=0}, 2 = {y=1}, 3 = {x=1}
+  3D unit cube:   0 = {x=0}, 1 = {y=0}, 2 = {z=0}, 3 = {z=1}, 4 = {y=1}, 5 = {x=1}
+Coordinates are written on the unit square / cube; the solver scales them by -x/-y/-z.
+"""
+from __future__ import annotations
+
+import itertools
+import math
+import os
+
+import numpy as np
+
+
+# ---------------------------------------------------------------------------------------------
+# bands and boundary conditions
+# ---------------------------------------------------------------------------------------------
+def random_bands(n_band: int, seed: int, v=(1e3, 6e3), log_tau=(-12.0, -10.0), C=(1e5, 1e6)):
+    """v ~ U[v], tau ~ 10^U[log_tau], C ~ U[C], Lr = (C/tau) / sum(C/tau)."""
+    rng = np.random.default_rng(seed)
+    vg = rng.uniform(*v, n_band)
+    tau = 10.0 ** rng.uniform(*log_tau, n_band)
+    Cv = rng.uniform(*C, n_band)
+    w = Cv / tau
+    return np.stack([vg, tau, Cv, w / w.sum()], axis=1)
+
+
+def write_bands(path, bands) -> None:
+    with open(path, "w") as f:
+        for v, t, c, l in bands:
+            f.write(f"{v:.17g} {t:.17g} {c:.17g} {l:.17g}\n")
+
+
+def write_bcs(path, rows) -> None:
+    """rows: (entity k, type, temperature)."""
+    with open(path, "w") as f:
+        f.write(f"{len(rows)} #number of boundaries\n\n")
+        f.write("#boundary index, type (1 isothermal, 2 diffuse, 31/32/33 specular), temperature\n")
+        for k, t, T in rows:
+            f.write(f"{int(k)} {int(t)} {float(T):.17g}\n")
+
+
+# ---------------------------------------------------------------------------------------------
+# 2D: Poisson-disk points + Delaunay triangulation of the unit square
+# ---------------------------------------------------------------------------------------------
+def poisson_disk_square(r: float, seed: int, k: int = 30) -> np.ndarray:
+    """Bridson sampling with minimum distance r in [0,1]^2, seeded with equispaced boundary points."""
+    rng = np.random.default_rng(seed)
+    nb = max(1, int(round(1.0 / r)))
+    t = np.linspace(0.0, 1.0, nb + 1)
+    bpts = [(x, 0.0) for x in t] + [(x, 1.0) for x in t] + [(0.0, y) for y in t[1:-1]] + [(1.0, y) for y in t[1:-1]]
+    cell = r / math.sqrt(2)
+    gn = int(math.ceil(1.0 / cell)) + 1
+    grid = -np.ones((gn, gn), dtype=np.int64)
+    pts = []
+
+    def insert(p):
+        i, j = min(int(p[0] / cell), gn - 1), min(int(p[1] / cell), gn - 1)
+        grid[i, j] = len(pts)
+        pts.append(p)
+
+    for p in bpts:
+        insert(p)
+    P = None
+    active = [len(pts)]
+    insert((rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7)))
+    margin = 0.5 * r
+    while active:
+        a = active[rng.integers(len(active))]
+        base = np.asarray(pts[a])
+        ang = rng.uniform(0, 2 * math.pi, k)
+        rad = r * np.sqrt(rng.uniform(1.0, 4.0, k))
+        cand = base + np.stack([np.cos(ang) * rad, np.sin(ang) * rad], axis=1)
+        ok_any = False
+        for c in cand:
+            if not (margin <= c[0] <= 1 - margin and margin <= c[1] <= 1 - margin):
+                continue
+            i, j = int(c[0] / cell), int(c[1] / cell)
+            i0, i1, j0, j1 = max(i - 2, 0), min(i + 3, gn), max(j - 2, 0), min(j + 3, gn)
+            nbrs = grid[i0:i1, j0:j1]
+            nbrs = nbrs[nbrs >= 0]
+            if nbrs.size:
+                P = np.asarray([pts[n] for n in nbrs])
+                if np.min(np.sum((P - c) ** 2, axis=1)) < r * r:
+                    continue
+            active.append(len(pts))
+            insert((float(c[0]), float(c[1])))
+            ok_any = True
+            break
+        if not ok_any:
+            active.remove(a)
+    return np.asarray(pts)
+
+
+def square_mesh(n_cells: int, seed: int):
+    """Triangle mesh of the unit square with about n_cells triangles.  Returns (points, tris,
+    boundary edges, boundary entity ids)."""
+    from scipy.spatial import Delaunay
+
+    # a Poisson-disk set of density rho gives ~2 rho triangles; rho ~ 0.7 / r^2 for Bridson
+    r = math.sqrt(1.4 / n_cells) * 0.935
+    pts = poisson_disk_square(r, seed)
+    tri = Delaunay(pts).simplices.astype(np.int64)
+    a, b, c = pts[tri[:, 0]], pts[tri[:, 1]], pts[tri[:, 2]]
+    area = 0.5 * np.abs((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    tri = tri[area > 1e-6 * area.mean()]
+    edges, ents = _boundary_faces(tri, pts, 2)
+    return pts, tri, edges, ents
+
+
+# ---------------------------------------------------------------------------------------------
+# 3D: jittered Kuhn (Freudenthal) tetrahedralisation of the unit cube
+# ---------------------------------------------------------------------------------------------
+def cube_mesh(n: int, seed: int, jitter: float = 0.12):
+    """n^3 cubes x 6 tetrahedra; interior vertices jittered by up to jitter*h per coordinate."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    g = np.arange(n + 1) * h
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
+    pts[inner] += rng.uniform(-jitter * h, jitter * h, (inner.sum(), 3))
+    vid = lambda i, j, k: (i * (n + 1) + j) * (n + 1) + k  # noqa: E731
+    I, J, K = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    tets = []
+    for perm in itertools.permutations(range(3)):
+        off = np.zeros((4, 3), dtype=np.int64)
+        for s in range(3):
+            off[s + 1] = off[s]
+            off[s + 1, perm[s]] += 1
+        tets.append(np.stack([vid(I + o[0], J + o[1], K + o[2]) for o in off], axis=1))
+    tet = np.concatenate(tets, axis=0)
+    faces, ents = _boundary_faces(tet, pts, 3)
+    return pts, tet, faces, ents
+
+
+def _boundary_faces(cells, pts, dim):
+    """Faces used by exactly one cell, with the entity id of the box side they lie on."""
+    nv = dim + 1
+    fl = []
+    for f in range(nv):
+        fl.append(np.delete(cells, f, axis=1))
+    allf = np.concatenate(fl, axis=0)
+    key = np.sort(allf, axis=1)
+    uniq, idx, cnt = np.unique(key, axis=0, return_index=True, return_counts=True)
+    bf = allf[idx[cnt == 1]]
+    cen = pts[bf].mean(axis=1)
+    tol = 1e-9
+    if dim == 2:
+        side = [(0, 0, 0.0), (1, 1, 0.0), (2, 1, 1.0), (3, 0, 1.0)]  # (entity, axis, value)
+    else:
+        side = [(0, 0, 0.0), (1, 1, 0.0), (2, 2, 0.0), (3, 2, 1.0), (4, 1, 1.0), (5, 0, 1.0)]
+    ent = -np.ones(len(bf), dtype=np.int64)
+    for e, ax, val in side:
+        on = np.all(np.abs(pts[bf][:, :, ax] - val) < tol, axis=1)
+        ent[on & (ent < 0)] = e
+    if np.any(ent < 0):
+        raise RuntimeError("boundary face not on a box side")
+    order = np.lexsort((bf[:, 0], ent))
+    return bf[order], ent[order]
+
+
+# ---------------------------------------------------------------------------------------------
+# .mphtxt writer (the layout the reference reader BTEMesh.cpp:7-130 expects)
+# ---------------------------------------------------------------------------------------------
+def write_mphtxt(path, pts, cells, bfaces, bents) -> None:
+    dim = pts.shape[1] if pts.ndim == 2 else 1
+    lines = ["# Created by staticbtesolver_amd.meshgen", "", "", "# Major & minor version", "0 1",
+             "1 # number of tags", "# Tags", "5 mesh1", "1 # number of types", "# Types", "3 obj", "",
+             "# --------- Object 0 ----------", "", "0 0 1", "4 Mesh # class", "4 # version", f"{dim} # sdim",
+             f"{len(pts)} # number of mesh points", "0 # lowest mesh point index", "", "# Mesh point coordinates"]
+    fmt = " ".join(["{:.17g}"] * dim)
+    lines += [fmt.format(*p) for p in pts]
+    types = []
+    if dim == 2:
+        types = [("edg", bfaces, bents), ("tri", cells, np.ones(len(cells), dtype=np.int64))]
+    else:
+        types = [("edg", np.zeros((0, 2), dtype=np.int64), np.zeros(0, dtype=np.int64)), ("tri", bfaces, bents),
+                 ("tet", cells, np.ones(len(cells), dtype=np.int64))]
+    lines += ["", f"{len(types)} # number of element types", ""]
+    for t, (name, el, ent) in enumerate(types):
+        lines += [f"# Type #{t}", "", f"3 {name} # type name", "", "", f"{el.shape[1]} # number of nodes per element",
+                  f"{len(el)} # number of elements", "# Elements"]
+        lines += [" ".join(str(int(v)) for v in row) for row in el]
+        lines += ["", f"{len(ent)} # number of geometric entity indices", "# Geometric entity indices"]
+        lines += [str(int(e)) for e in ent]
+        lines += [""]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def make_square_case(directory, n_cells, seed, n_band, band_seed, L=1e-7, bc_rows=None, name="square"):
+    """Writes <name>.mphtxt, <name>_band.dat, <name>_bc.dat into directory; returns their paths and
+    the number of triangles."""
+    os.makedirs(directory, exist_ok=True)
+    pts, tri, edges, ents = square_mesh(n_cells, seed)
+    mesh = os.path.join(directory, f"{name}.mphtxt")
+    write_mphtxt(mesh, pts, tri, edges, ents)
+    band = os.path.join(directory, f"{name}_band.dat")
+    write_bands(band, random_bands(n_band, band_seed))
+    bc = os.path.join(directory, f"{name}_bc.dat")
+    write_bcs(bc, bc_rows or [(0, 1, 301.0), (1, 2, 300.0), (2, 2, 300.0), (3, 1, 299.0)])
+    return dict(mesh=mesh, band=band, bc=bc, n_cells=len(tri), L=L)
+
+
+def make_cube_case(directory, n, seed, n_band, band_seed, L=1e-7, bc_rows=None, name="cube"):
+    os.makedirs(directory, exist_ok=True)
+    pts, tet, faces, ents = cube_mesh(n, seed)
+    mesh = os.path.join(directory, f"{name}.mphtxt")
+    write_mphtxt(mesh, pts, tet, faces, ents)
+    band = os.path.join(directory, f"{name}_band.dat")
+    write_bands(band, random_bands(n_band, band_seed))
+    bc = os.path.join(directory, f"{name}_bc.dat")
+    write_bcs(bc, bc_rows or [(0, 1, 301.0), (1, 2, 300.0), (2, 2, 300.0), (3, 2, 300.0), (4, 2, 300.0),
+                              (5, 1, 299.0)])
+    return dict(mesh=mesh, band=band, bc=bc, n_cells=len(tet), L=L)

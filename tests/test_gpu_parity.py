@@ -1,0 +1,173 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle and the reference goldens.
+
+Tolerances (FP64 plain form): assembly and RHS to 1e-13 relative (only the multiplication order
+differs); inner solves to 1e-11 relative to the exact sweep; converged T to 1e-10 and main heat
+fluxes to 1e-8 relative to the exact oracle; adiabatic-wall fluxes to 1e-8 of the largest flux.
+Mixed precision: T within 1e-4 relative of the FP64 oracle (target of the mixed form).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from staticbtesolver_amd import capi
+from tests.cases import GOLDEN_CASES, case_kwargs, check_against_golden, golden_sol
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def csr_to_faces(rp, col, val, nbr):
+    N, F = nbr.shape
+    diag, off = np.zeros(N), np.zeros((N, F))
+    for c in range(N):
+        for k in range(rp[c], rp[c + 1]):
+            j = col[k]
+            if j == c:
+                diag[c] = val[k]
+            else:
+                f = int(np.nonzero(nbr[c] == j)[0][0])
+                off[c, f] = val[k]
+    return diag, off
+
+
+@pytest.fixture(scope="module", params=sorted(GOLDEN_CASES))
+def pair(request):
+    name = request.param
+    kw = case_kwargs(name)
+    return name, O.OracleCase(**kw), capi.Case(**kw)
+
+
+def test_assembly_matches_oracle(pair):
+    name, o, c = pair
+    say("assembly", name)
+    nbr = c.geometry()["nbr"]
+    for b in range(c.N_band):
+        for d in range(c.N_dir):
+            rp, col, val = o.assemble(b, d)
+            od, oo = csr_to_faces(rp, col, val, nbr)
+            gd, go = c.debug_assemble(b, d)
+            np.testing.assert_allclose(gd, od, rtol=1e-13, err_msg=f"diag b={b} d={d}")
+            np.testing.assert_allclose(go, oo, rtol=1e-13, atol=1e-13 * np.abs(oo).max(), err_msg=f"off b={b} d={d}")
+            assert np.array_equal(go != 0, oo != 0)
+
+
+def _random_state(c, seed):
+    rng = np.random.default_rng(seed)
+    scale = 1e5 if c.dim > 1 else 1e7
+    ee = rng.uniform(0.5, 1.5, (c.N_band, c.N_dir, c.N_cell)) * scale
+    dens = rng.uniform(-1, 1, (c.N_cell, c.N_band)) * scale
+    return ee, dens
+
+
+def test_rhs_matches_oracle(pair):
+    name, o, c = pair
+    say("rhs", name)
+    ee, dens = _random_state(c, 1)
+    g = c.debug_rhs(ee, dens)
+    for b in range(c.N_band):
+        for d in range(c.N_dir):
+            r = o.rhs(b, d, ee, dens)
+            np.testing.assert_allclose(g[b, d], r, rtol=1e-12, atol=1e-13 * np.abs(r).max(), err_msg=f"b={b} d={d}")
+
+
+def test_inner_solve_matches_exact_sweep(pair):
+    name, o, c = pair
+    say("inner", name)
+    ee, dens = _random_state(c, 2)
+    x, its = c.debug_inner(ee, dens)
+    for b in range(c.N_band):
+        for d in range(c.N_dir):
+            rp, col, val = o.assemble(b, d)
+            rhs = o.rhs(b, d, ee, dens)
+            xe, _ = O.inner_solve(O.INNER_EXACT, rp, col, val, rhs)
+            np.testing.assert_allclose(x[b, d], xe, rtol=1e-11, atol=1e-12 * np.abs(xe).max(), err_msg=f"b={b} d={d}")
+    assert its.max() <= 2  # the sweep is exact on these acyclic upwind graphs
+
+
+def test_full_solve_matches_oracle_and_golden(pair):
+    name, o, c = pair
+    say("solve", name)
+    o.set_inner(O.INNER_EXACT)
+    o.set_stop(1e-4)
+    ref = o.solve(10000)
+    r = c.solve(stop_margin=1e-4, max_iter=10000)
+    assert r["status"] == capi.BTE_OK
+    assert r["iters"] == ref["iters"]
+    np.testing.assert_allclose(r["margins"], ref["margins"], rtol=1e-6)
+    np.testing.assert_allclose(r["margins"][:-3], ref["margins"][:-3], rtol=1e-9)
+    np.testing.assert_allclose(r["T"], ref["T"], rtol=1e-10)
+    np.testing.assert_allclose(r["T_band"], ref["T_band"], rtol=1e-10)
+    qmax = np.abs(ref["flux"]).max()
+    main = np.abs(ref["flux"]) > 1e-3 * qmax
+    assert np.all(np.abs(r["flux"] - ref["flux"])[main] <= 1e-8 * np.abs(ref["flux"])[main])
+    assert np.all(np.abs(r["flux"] - ref["flux"]) <= 1e-8 * qmax)
+    gm, gf = golden_sol(name)
+    check_against_golden(r["margins"], r["flux"], gm, gf)
+
+
+def test_default_stop_and_repeat_determinism():
+    kw = case_kwargs("2DM2DG")
+    c = capi.Case(**kw)
+    say("determinism 2DM2DG")
+    r1 = c.solve()
+    r2 = c.solve()
+    assert r1["margins"][-1] <= 1e-5 and r1["iters"] == r2["iters"]
+    assert np.array_equal(r1["T"], r2["T"]) and np.array_equal(r1["flux"], r2["flux"])
+    r3 = c.solve(use_graph=0)
+    assert np.array_equal(r1["T"], r3["T"])
+
+
+def test_mixed_precision_within_1e4_of_fp64_oracle():
+    kw = case_kwargs("2DM2DG")
+    say("mixed 2DM2DG")
+    o = O.OracleCase(**kw)
+    o.set_inner(O.INNER_EXACT)
+    ref = o.solve(10000)
+    c = capi.Case(**kw)
+    r = c.solve(precision=capi.PREC_MIXED)
+    assert r["status"] == capi.BTE_OK
+    assert np.max(np.abs(r["T"] - ref["T"]) / np.abs(ref["T"])) <= 1e-4
+    dT, dTr = r["T"] - o.T_ref, ref["T"] - o.T_ref
+    assert np.max(np.abs(dT - dTr)) <= 1e-4 * np.max(np.abs(dTr))
+
+
+def test_jacobi_path_matches_reference_bicgstab_restatement():
+    kw = case_kwargs("2DM2DG")
+    say("jacobi 2DM2DG")
+    o = O.OracleCase(**kw)
+    o.set_inner(O.INNER_BICGSTAB)
+    o.set_stop(1e-4)
+    ref = o.solve(10000)
+    c = capi.Case(**kw)
+    r = c.solve(precond=capi.PC_JACOBI, stop_margin=1e-4)
+    assert r["iters"] == ref["iters"]
+    np.testing.assert_allclose(r["T"], ref["T"], rtol=1e-9)
+    np.testing.assert_allclose(r["margins"][:-3], ref["margins"][:-3], rtol=1e-5)
+
+
+def test_equal_wall_temperature_is_exact_fixed_point(tmp_path):
+    say("fixed point")
+    (tmp_path / "bc.dat").write_text("4\n\n#\n0 1 305\n1 2 0\n2 31 0\n3 1 305\n")
+    kw = case_kwargs("2DM2DG")
+    kw["bc_file"] = str(tmp_path / "bc.dat")
+    c = capi.Case(**kw)
+    r = c.solve()
+    # converged to margin <= 1e-5 from e = 0: residual error ~1e-3 of e ~ 6e5
+    np.testing.assert_allclose(r["T"], 305.0, rtol=1e-9)
+    qmax = 1.627e6 / (4 * np.pi) * 5 * 2677  # scale of a 5 K flux
+    assert np.all(np.abs(r["flux"]) < 1e-7 * qmax)
+
+
+@pytest.mark.parametrize("bad", ["33", "7"])
+def test_unsupported_boundary_type_is_an_error(tmp_path, bad):
+    say("bad bc", bad)
+    (tmp_path / "bc.dat").write_text(f"4\n\n#\n0 1 301\n1 {bad} 300\n2 31 300\n3 1 300\n")
+    kw = case_kwargs("2DM2DG")
+    kw["bc_file"] = str(tmp_path / "bc.dat")
+    c = capi.Case(**kw)
+    with pytest.raises(capi.BTEError) as e:
+        c.solve(max_iter=2)
+    assert e.value.code == capi.BTE_ERR_BC
