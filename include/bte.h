@@ -58,6 +58,9 @@ typedef struct bte_opts {
     int timing;           /* per-kernel hipEvent timing into bte_stats; default 0 */
     int learned;          /* apply the loaded learned correction every learned_every iterations */
     int learned_every;    /* default 8 */
+    int direct_sweep;     /* BTE_PC_SWEEP + FP64: when no upwind edge had to be lagged (acyclic
+                             upwind graph), solve each system by one downwind sweep with the RHS
+                             formed on the fly instead of BiCGSTAB (same solution); default 1 */
 } bte_opts;
 
 typedef struct bte_result {

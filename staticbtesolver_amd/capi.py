@@ -41,7 +41,8 @@ class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("precond", C.c_int), ("max_iter", C.c_int),
                 ("stop_margin", C.c_double), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
                 ("refine_steps", C.c_int), ("block", C.c_int), ("use_graph", C.c_int), ("verbose", C.c_int),
-                ("timing", C.c_int), ("learned", C.c_int), ("learned_every", C.c_int)]
+                ("timing", C.c_int), ("learned", C.c_int), ("learned_every", C.c_int),
+                ("direct_sweep", C.c_int)]
 
 
 class Result(C.Structure):

@@ -548,6 +548,37 @@ __device__ __forceinline__ SysView<float> make_view<float>(const Ops& o, int b, 
     return sv;
 }
 
+// Exact inner solve by one downwind sweep with the right-hand side formed on the fly.  Valid when
+// the level schedule lagged no upwind edge: the operator is then triangular in level order and the
+// sweep is its exact inverse (what BiCGSTAB with the sweep preconditioner returns after its first
+// step).  One work-group per system; ee_curr is written once per cell, no Krylov vectors.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sweep_direct(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, int S,
+                                                       const double* __restrict__ ee_prev,
+                                                       const double* __restrict__ dens) {
+    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+        const int b = s / o.D, d = s % o.D;
+        const SysView<double> sv = make_view<double>(o, b, d);
+        double* x = w.x + (size_t)s * o.N;
+        for (int l = 0; l < sv.L; l++) {
+            const int beg = sv.lp[l], end = sv.lp[l + 1];
+            for (int i = beg + (int)threadIdx.x; i < end; i += BLOCK) {
+                const int c = sv.perm[i];
+                const double rhs = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
+                double acc = 0;
+#pragma unroll 3
+                for (int k = 0; k < sv.W; k++) {
+                    const int j = sv.scol[(size_t)k * sv.N + i];
+                    if (j >= 0) acc += sv.sval[(size_t)k * sv.N + i] * x[j];
+                }
+                x[c] = (rhs - sv.vt * acc) / diag_of(sv, sv.sdg[i], c);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) w.iters[s] += 1;
+    }
+}
+
 // FP64 path: RHS formed in-kernel, then BiCGSTAB in FP64 writes ee_curr directly.
 template <int BLOCK, bool SWEEP>
 __global__ __launch_bounds__(BLOCK) void k_solve64(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, SolveCfg cfg,

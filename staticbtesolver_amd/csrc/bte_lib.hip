@@ -280,6 +280,7 @@ void bte_default_opts(bte_opts* o) {
     o->timing = 0;
     o->learned = 0;
     o->learned_every = 8;
+    o->direct_sweep = 1;
 }
 
 int bte_ctx_create(int device, bte_ctx** out) {
@@ -627,6 +628,10 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int S, const double* ee_pre
     const Ops op = ops_of(ctx);
     const Work w = work_of(ctx, ee_curr);
     dim3 grid(S), blk(BLOCK);
+    if (o->precision == BTE_PREC_FP64 && o->precond == BTE_PC_SWEEP && o->direct_sweep && ctx->lagged == 0) {
+        hipLaunchKernelGGL((k_sweep_direct<BLOCK>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev, dens);
+        return;
+    }
     if (o->precision == BTE_PREC_MIXED) {
         if (o->precond == BTE_PC_JACOBI)
             hipLaunchKernelGGL((k_solve_mixed<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
@@ -732,7 +737,7 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
                     ctx->num_phi);
     double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
     // graph key: everything that changes the captured launches
-    const int key = (o.precision << 0) | (o.precond << 1) | (block << 2);
+    const int key = (o.precision << 0) | (o.precond << 1) | ((o.direct_sweep ? 1 : 0) << 2) | (block << 3);
     const bool use_graph = o.use_graph && !timing;
     if (use_graph && ctx->graph_key != key) {
         drop_graphs(ctx);

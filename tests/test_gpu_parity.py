@@ -73,11 +73,12 @@ def test_rhs_matches_oracle(pair):
             np.testing.assert_allclose(g[b, d], r, rtol=1e-12, atol=1e-13 * np.abs(r).max(), err_msg=f"b={b} d={d}")
 
 
-def test_inner_solve_matches_exact_sweep(pair):
+@pytest.mark.parametrize("direct", [1, 0])
+def test_inner_solve_matches_exact_sweep(pair, direct):
     name, o, c = pair
-    say("inner", name)
+    say("inner", name, "direct sweep" if direct else "bicgstab+sweep")
     ee, dens = _random_state(c, 2)
-    x, its = c.debug_inner(ee, dens)
+    x, its = c.debug_inner(ee, dens, direct_sweep=direct)
     for b in range(c.N_band):
         for d in range(c.N_dir):
             rp, col, val = o.assemble(b, d)
