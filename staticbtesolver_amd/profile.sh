@@ -1,0 +1,20 @@
+#!/usr/bin/env bash
+# Profiles the bench workload with rocprofv3: kernel trace + stats, then HBM byte counters
+# (FETCH_SIZE, WRITE_SIZE) in separate passes, each also for the calibration copy kernel.
+# Usage (from the repository root, on a GPU box): bash staticbtesolver_amd/profile.sh [bench args]
+set -euo pipefail
+ROOT=$(cd "$(dirname "$0")/.." && pwd)
+OUT=$ROOT/gpurun_out/prof
+mkdir -p "$OUT"
+export TMPDIR=/tmp
+ARGS=${*:-"--profile-run"}
+cd /tmp
+echo "== kernel trace: bench.py $ARGS"
+timeout -k 10 300 rocprofv3 --kernel-trace --stats -d "$OUT/trace" -o bench -- python3 "$ROOT/bench.py" $ARGS
+for C in FETCH_SIZE WRITE_SIZE; do
+  echo "== pmc $C: bench.py $ARGS"
+  timeout -s KILL 200 rocprofv3 --pmc $C -d "$OUT/pmc_$C" -o bench -- python3 "$ROOT/bench.py" $ARGS
+  echo "== pmc $C: hbm_copy 1024 MiB"
+  timeout -s KILL 60 rocprofv3 --pmc $C -d "$OUT/pmc_copy_$C" -o copy -- "$ROOT/staticbtesolver_amd/bin/hbm_copy" 1024 3
+done
+echo "== done"
