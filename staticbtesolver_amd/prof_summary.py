@@ -49,10 +49,10 @@ def summary(prof_dir, title, command, copy_mib=1024):
              "| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
     tk = top_kernels(prof_dir)
     for n, calls, tot, avg, pct in tk[:10]:
-        lines.append(f"| `{short(n)}` | {calls} | {tot / 1e3:.1f} | {avg / 1e3:.2f} | {pct:.1f} |")
+        lines.append(f"| `{short(n)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |")
     dom = short(tk[0][0]).split("::")[-1].split("<")[0]
-    lines += ["", f"Dominant kernel: `{dom}`, average {tk[0][3] / 1e3:.2f} us per dispatch.", ""]
-    res = {"dominant": dom, "avg_us": tk[0][3] / 1e3}
+    lines += ["", f"Dominant kernel: `{dom}`, average {tk[0][3]:.2f} us per dispatch.", ""]
+    res = {"dominant": dom, "avg_us": tk[0][3]}
     try:
         f_k, nf = counter_per_dispatch(os.path.join(prof_dir, "pmc_FETCH_SIZE"), "FETCH_SIZE", dom)
         w_k, nw = counter_per_dispatch(os.path.join(prof_dir, "pmc_WRITE_SIZE"), "WRITE_SIZE", dom)
