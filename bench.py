@@ -165,6 +165,8 @@ def main():
                    "systems_per_launch": case.N_band * case.N_dir, "form": args.precision + "+sweep",
                    "outer_iterations": int(statistics.median(iters)),
                    "ms_per_outer_iteration": round(1e3 * t_med / statistics.median(iters), 4),
+                   "sweep_levels_max": st["max_levels"], "lagged_edges": st["lagged_edges"],
+                   "block": st["block"], "sweep_variant": st["variant"],
                    "max_rel_err_T_vs_fp64": rel},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_solve",

@@ -61,6 +61,9 @@ typedef struct bte_opts {
     int direct_sweep;     /* BTE_PC_SWEEP + FP64: when no upwind edge had to be lagged (acyclic
                              upwind graph), solve each system by one downwind sweep with the RHS
                              formed on the fly instead of BiCGSTAB (same solution); default 1 */
+    int sweep_variant;    /* direct sweep kernel: 0 autotuned, 1 basic, 2 level-ordered operands,
+                             3 level-ordered operands with the intensity vector held in LDS
+                             (needs n_cell * 8 bytes of LDS); default 0 */
 } bte_opts;
 
 typedef struct bte_result {
@@ -93,6 +96,7 @@ typedef struct bte_stats {
     int max_levels;       /* deepest sweep schedule over directions */
     int lagged_edges;     /* upwind edges dropped from the sweep to break cycles */
     int block;            /* work-group size used */
+    int variant;          /* direct-sweep variant used (1..3), 0 for the Krylov paths */
 } bte_stats;
 
 void bte_default_opts(bte_opts* opts);

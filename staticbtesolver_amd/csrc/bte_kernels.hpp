@@ -104,6 +104,24 @@ __global__ void k_gather_sweep(int N, int D, int W, const int* __restrict__ perm
     }
 }
 
+// Per-direction, level-ordered volume weights w_d V_c and a flag for cells with an incoming
+// boundary face in direction d (the only cells whose right-hand side needs the face loop).
+__global__ void k_sweep_aux(Geo g, Quad q, const int* __restrict__ perm, const double* __restrict__ w_eff,
+                            double* __restrict__ swp_vw, unsigned char* __restrict__ swp_bnd) {
+    const long long total = (long long)q.D * g.N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int d = (int)(t / g.N), i = (int)(t % g.N);
+        const int c = perm[(size_t)d * g.N + i];
+        swp_vw[t] = w_eff[d] * g.vol[c];
+        unsigned char bnd = 0;
+        for (int f = 0; f < g.F; f++)
+            if (g.nbr[(size_t)c * g.F + f] < 0 && dot_nc(q.dir + 3 * d, g.normal + 3 * ((size_t)c * g.F + f)) < 0)
+                bnd = 1;
+        swp_bnd[t] = bnd;
+    }
+}
+
 template <typename R>
 __global__ void k_convert(long long n, const double* __restrict__ in, R* __restrict__ out) {
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
@@ -273,6 +291,8 @@ struct Ops {
     const double* swp_dg;
     const float* swp_val32;
     const float* swp_dg32;
+    const double* swp_vw;          // [D*N] w_d V_c in level order
+    const unsigned char* swp_bnd;  // [D*N] 1 if the cell has an incoming boundary face
 };
 
 struct Work {
@@ -579,6 +599,61 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_direct(Geo g, Quad q, Bnd bd, B
     }
 }
 
+// Direct sweep, tuned form.  Per level every thread loads its cell's level-ordered data (operator
+// slots, w_d V_c, boundary flag) contiguously in i; only the density and the upwind values are
+// gathered by cell index.  Interior cells skip the face loop of the right-hand side.  With
+// XLDS the system's intensity vector lives in LDS during the sweep (gathers and stores never
+// leave the CU) and is written to HBM once, coalesced, at the end; this needs N*8 bytes of LDS.
+template <int BLOCK, bool XLDS>
+__global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, int S,
+                                                     const double* __restrict__ ee_prev,
+                                                     const double* __restrict__ dens) {
+    extern __shared__ double xs[];
+    const int N = o.N, W = o.W;
+    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+        const int b = s / o.D, d = s % o.D;
+        const double vt = o.vt[b];
+        double* xg = w.x + (size_t)s * N;
+        double* x = XLDS ? xs : xg;
+        const int* __restrict__ perm = o.perm + (size_t)d * N;
+        const int* __restrict__ scol = o.swp_col + (size_t)d * W * N;
+        const double* __restrict__ sval = o.swp_val + (size_t)d * W * N;
+        const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
+        const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
+        const unsigned char* __restrict__ sbnd = o.swp_bnd + (size_t)d * N;
+        const double* __restrict__ densb = dens + (size_t)b * N;
+        const int* lp = o.lvl_ptr + o.lvl_off[d];
+        const int L = o.lvl_cnt[d];
+        for (int l = 0; l < L; l++) {
+            const int beg = lp[l], end = lp[l + 1];
+            for (int i = beg + (int)threadIdx.x; i < end; i += BLOCK) {
+                const int c = perm[i];
+                const double vw = svw[i];
+                const double dgc = sdg[i];
+                int j[3];
+                double a[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    j[k] = (k < W) ? scol[(size_t)k * N + i] : -1;
+                    a[k] = (k < W) ? sval[(size_t)k * N + i] : 0.0;
+                }
+                const double rhs = sbnd[i] ? rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens) : densb[c] * vw;
+                double acc = 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+                    if (j[k] >= 0) acc += a[k] * x[j[k]];
+                x[c] = (rhs - vt * acc) / (vt * dgc + vw);
+            }
+            __syncthreads();
+        }
+        if constexpr (XLDS) {
+            for (int c = threadIdx.x; c < N; c += BLOCK) xg[c] = xs[c];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) w.iters[s] += 1;
+    }
+}
+
 // FP64 path: RHS formed in-kernel, then BiCGSTAB in FP64 writes ee_curr directly.
 template <int BLOCK, bool SWEEP>
 __global__ __launch_bounds__(BLOCK) void k_solve64(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, SolveCfg cfg,
@@ -707,6 +782,48 @@ __global__ __launch_bounds__(BLOCK) void k_temperature(int N, int D, Bands bands
     }
     block_sum<BLOCK, 1>(mp, lds);
     if (threadIdx.x == 0) partial[blockIdx.x] = mp[0];
+}
+
+// Band temperatures (StaticBTESolver.cpp:96-104) with one thread per (band, cell) -- B times more
+// threads than a cell loop, so enough loads are in flight to stream both intensity arrays at HBM
+// rate -- fused with the margin partial sums (:311-321).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, const double* __restrict__ omega,
+                                                const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
+                                                double* __restrict__ Tband, double* __restrict__ partial) {
+    __shared__ double lds[BLOCK / 64];
+    double mp[1] = {0};
+    const long long t = blockIdx.x * (long long)BLOCK + threadIdx.x;
+    if (t < (long long)bands.B * N) {
+        const int b = (int)(t / N), c = (int)(t % N);
+        const size_t base = (size_t)b * D * N + c;
+        double e0 = 0;
+#pragma unroll 8
+        for (int d = 0; d < D; d++) {
+            const double ec = ee_curr[base + (size_t)d * N];
+            const double df = ec - ee_prev[base + (size_t)d * N];
+            e0 += ec * omega[d];
+            mp[0] += df * df;
+        }
+        Tband[t] = e0 / bands.C[b] + bands.T_ref;
+    }
+    block_sum<BLOCK, 1>(mp, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = mp[0];
+}
+
+// Equilibrium temperature and emission density per cell (StaticBTESolver.cpp:106-120).
+__global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, double* __restrict__ T,
+                        double* __restrict__ dens) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        double rrn = 0, rnn = 0;
+        for (int b = 0; b < bands.B; b++) {
+            rrn += bands.Lr[b] * Tband[(size_t)b * N + c];
+            rnn += bands.Lr[b];
+        }
+        const double tc = rrn / rnn;
+        T[c] = tc;
+        for (int b = 0; b < bands.B; b++) dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
+    }
 }
 
 // Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).

@@ -63,8 +63,9 @@ struct bte_ctx {
     // assembly + sweep schedule
     int W = 0;
     DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
-    DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32;
+    DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd;
     int max_levels = 0, lagged = 0;
+    int max_lds = 65536;  // dynamic LDS bytes a work-group may use on this device
     bool have_sweep = false, have_fp32 = false;
     // state + work
     DBuf eeA, eeB, dens, T, Tband, partial, margin, bflux;
@@ -81,8 +82,10 @@ struct bte_ctx {
     long long calls[KA_COUNT] = {0};
     double spmv_bytes = 0;
     long long spmv_rows = 0;
-    int block_used = 0;
-    int tuned_block[2][2] = {{0, 0}, {0, 0}};  // [precision][precond]
+    int block_used = 0, variant_used = 0;
+    // autotuned launch shape per solver form: work-group size and sweep variant (0 = not tuned)
+    int tuned_block[2][2][2] = {};    // [precision][precond][direct]
+    int tuned_variant[2][2][2] = {};  // sweep variant for the direct form
 };
 
 #define HIPCHK(expr)                                                                              \
@@ -281,6 +284,7 @@ void bte_default_opts(bte_opts* o) {
     o->learned = 0;
     o->learned_every = 8;
     o->direct_sweep = 1;
+    o->sweep_variant = 0;
 }
 
 int bte_ctx_create(int device, bte_ctx** out) {
@@ -515,7 +519,15 @@ int prepare(bte_ctx* ctx, bool timing) {
         hipLaunchKernelGGL(k_gather_sweep<double>, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream, N,
                            D, W, ctx->perm.as<int>(), ctx->dg.as<double>(), ctx->ell_val.as<double>(),
                            ctx->swp_dg.as<double>(), ctx->swp_val.as<double>());
+        HIPCHK(ctx->swp_vw.ensure(sizeof(double) * (size_t)D * N));
+        HIPCHK(ctx->swp_bnd.ensure((size_t)D * N));
+        hipLaunchKernelGGL(k_sweep_aux, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream, geo_of(ctx),
+                           quad_of(ctx), ctx->perm.as<int>(), ctx->w_eff.as<double>(), ctx->swp_vw.as<double>(),
+                           ctx->swp_bnd.as<unsigned char>());
         HIPCHK(hipGetLastError());
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
+            ctx->max_lds = lds;
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     ctx->have_sweep = true;
@@ -565,7 +577,7 @@ int ensure_work(bte_ctx* ctx, int precision) {
     HIPCHK(ctx->dens.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
     HIPCHK(ctx->T.ensure(sizeof(double) * ctx->N));
     HIPCHK(ctx->Tband.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
-    HIPCHK(ctx->partial.ensure(sizeof(double) * grid_for(ctx->N, 256)));
+    HIPCHK(ctx->partial.ensure(sizeof(double) * grid_for((long long)ctx->B * ctx->N, 256)));
     HIPCHK(ctx->margin.ensure(sizeof(double) * 2));
     HIPCHK(ctx->bflux.ensure(sizeof(double) * std::max<size_t>(1, ctx->h_bci.size() * ctx->B)));
     ctx->work_elems = SN;
@@ -596,6 +608,8 @@ Ops ops_of(bte_ctx* ctx) {
     o.swp_dg = ctx->swp_dg.as<double>();
     o.swp_val32 = ctx->swp_val32.as<float>();
     o.swp_dg32 = ctx->swp_dg32.as<float>();
+    o.swp_vw = ctx->swp_vw.as<double>();
+    o.swp_bnd = ctx->swp_bnd.as<unsigned char>();
     return o;
 }
 
@@ -618,8 +632,22 @@ Work work_of(bte_ctx* ctx, double* ee_curr) {
     return w;
 }
 
+// Launch shape of the inner-solve kernel: work-group size and, for the direct sweep, its variant
+// (1 basic, 2 level-ordered operands, 3 level-ordered operands + intensity vector in LDS).
+struct Shape {
+    int block = 256;
+    int variant = 2;
+};
+
+bool direct_path(const bte_ctx* ctx, const bte_opts* o) {
+    return o->precision == BTE_PREC_FP64 && o->precond == BTE_PC_SWEEP && o->direct_sweep && ctx->lagged == 0;
+}
+
+bool lds_fits(const bte_ctx* ctx) { return (size_t)ctx->N * sizeof(double) + 1024 <= (size_t)ctx->max_lds; }
+
 template <int BLOCK>
-void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int S, const double* ee_prev, double* ee_curr, const double* dens) {
+void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const double* ee_prev, double* ee_curr,
+                    const double* dens) {
     SolveCfg cfg{S, o->inner_maxit, o->inner_tol, std::max(1, o->refine_steps), o->precond == BTE_PC_JACOBI ? 1 : 0};
     const Geo g = geo_of(ctx);
     const Quad q = quad_of(ctx);
@@ -628,8 +656,19 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int S, const double* ee_pre
     const Ops op = ops_of(ctx);
     const Work w = work_of(ctx, ee_curr);
     dim3 grid(S), blk(BLOCK);
-    if (o->precision == BTE_PREC_FP64 && o->precond == BTE_PC_SWEEP && o->direct_sweep && ctx->lagged == 0) {
-        hipLaunchKernelGGL((k_sweep_direct<BLOCK>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev, dens);
+    if (direct_path(ctx, o)) {
+        if (variant == 3 && lds_fits(ctx)) {
+            const size_t lds = (size_t)ctx->N * sizeof(double);
+            (void)hipFuncSetAttribute((const void*)k_sweep_fast<BLOCK, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
+            hipLaunchKernelGGL((k_sweep_fast<BLOCK, true>), grid, blk, lds, ctx->stream, g, q, bd, bn, op, w, S, ee_prev,
+                               dens);
+        } else if (variant == 1) {
+            hipLaunchKernelGGL((k_sweep_direct<BLOCK>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev, dens);
+        } else {
+            hipLaunchKernelGGL((k_sweep_fast<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev,
+                               dens);
+        }
         return;
     }
     if (o->precision == BTE_PREC_MIXED) {
@@ -648,20 +687,20 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int S, const double* ee_pre
     }
 }
 
-void launch_solve(bte_ctx* ctx, const bte_opts* o, int block, const double* ee_prev, double* ee_curr,
+void launch_solve(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_prev, double* ee_curr,
                   const double* dens) {
     const int S = ctx->B * ctx->D;
-    switch (block) {
-        case 64: launch_solve_b<64>(ctx, o, S, ee_prev, ee_curr, dens); break;
-        case 128: launch_solve_b<128>(ctx, o, S, ee_prev, ee_curr, dens); break;
-        case 512: launch_solve_b<512>(ctx, o, S, ee_prev, ee_curr, dens); break;
-        case 1024: launch_solve_b<1024>(ctx, o, S, ee_prev, ee_curr, dens); break;
-        default: launch_solve_b<256>(ctx, o, S, ee_prev, ee_curr, dens); break;
+    switch (sh.block) {
+        case 64: launch_solve_b<64>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
+        case 128: launch_solve_b<128>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
+        case 512: launch_solve_b<512>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
+        case 1024: launch_solve_b<1024>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
+        default: launch_solve_b<256>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
     }
 }
 
 // The kernels of one outer iteration (diffuse BC data, inner solves, T update, margin).
-int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, int block, const double* ee_prev, double* ee_curr,
+int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_prev, double* ee_curr,
                       bool timing) {
     const Geo g = geo_of(ctx);
     const Quad q = quad_of(ctx);
@@ -674,14 +713,15 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, int block, const double* 
     }
     {
         Timer tm(ctx, KA_SOLVE, timing);
-        launch_solve(ctx, o, block, ee_prev, ee_curr, ctx->dens.as<double>());
+        launch_solve(ctx, o, sh, ee_prev, ee_curr, ctx->dens.as<double>());
     }
-    const int nblk = grid_for(ctx->N, 256);
+    const int nblk = grid_for((long long)ctx->B * ctx->N, 256);
     {
         Timer tm(ctx, KA_TEMP, timing);
-        hipLaunchKernelGGL(k_temperature<256>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
-                           ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), ctx->T.as<double>(),
-                           ctx->dens.as<double>(), ctx->partial.as<double>());
+        hipLaunchKernelGGL(k_tband<256>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
+                           ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), ctx->partial.as<double>());
+        hipLaunchKernelGGL(k_tcell, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, bn,
+                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>());
     }
     {
         Timer tm(ctx, KA_MARGIN, timing);
@@ -692,11 +732,71 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, int block, const double* 
     return BTE_OK;
 }
 
-int pick_block(bte_ctx* ctx, const bte_opts* o) {
-    if (o->block == 64 || o->block == 128 || o->block == 256 || o->block == 512 || o->block == 1024) return o->block;
-    // heuristic default; the autotuner in bte_autotune() refines it
-    int& tuned = ctx->tuned_block[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI];
-    return tuned ? tuned : 256;
+// Autotuner: times the inner-solve kernel of one outer iteration (from a zero state) for each
+// candidate work-group size and sweep variant, keeps the fastest for this solver form.
+int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
+    const bool direct = direct_path(ctx, o);
+    std::vector<Shape> cand;
+    for (int blk : {64, 128, 256, 512}) {
+        if (direct) {
+            cand.push_back({blk, 2});
+            if (lds_fits(ctx)) cand.push_back({blk, 3});
+        } else {
+            cand.push_back({blk, 0});
+        }
+    }
+    const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
+    HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)ctx->B * ctx->N, ctx->stream));
+    if (ctx->has_diffuse)
+        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+                           geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeA.as<double>(), ctx->ediff.as<double>());
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    for (const Shape& sh : cand) {
+        float tot = 0;
+        for (int rep = 0; rep < 3; rep++) {
+            HIPCHK(hipEventRecord(e0, ctx->stream));
+            launch_solve(ctx, o, sh, ctx->eeA.as<double>(), ctx->eeB.as<double>(), ctx->dens.as<double>());
+            HIPCHK(hipEventRecord(e1, ctx->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) tot += ms;  // first launch warms caches and code
+        }
+        if (tot < best_ms) {
+            best_ms = tot;
+            best = sh;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    return BTE_OK;
+}
+
+int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh) {
+    const bool direct = direct_path(ctx, o);
+    int& tb = ctx->tuned_block[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct];
+    int& tv = ctx->tuned_variant[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct];
+    const bool fixed_block = o->block == 64 || o->block == 128 || o->block == 256 || o->block == 512 || o->block == 1024;
+    if (fixed_block && (!direct || o->sweep_variant != 0)) {
+        sh.block = o->block;
+        sh.variant = direct ? o->sweep_variant : 0;
+        return BTE_OK;
+    }
+    if (!tb) {
+        Shape best;
+        int rc = autotune(ctx, o, best);
+        if (rc) return rc;
+        tb = best.block;
+        tv = best.variant;
+    }
+    sh.block = fixed_block ? o->block : tb;
+    sh.variant = (direct && o->sweep_variant != 0) ? o->sweep_variant : tv;
+    return BTE_OK;
 }
 
 }  // namespace
@@ -722,8 +822,10 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
     if ((rc = ensure_work(ctx, o.precision))) return rc;
     const int N = ctx->N, B = ctx->B, D = ctx->D, S = B * D;
     const size_t SN = (size_t)S * N;
-    const int block = pick_block(ctx, &o);
-    ctx->block_used = block;
+    Shape shape;
+    if ((rc = pick_shape(ctx, &o, shape))) return rc;
+    ctx->block_used = shape.block;
+    ctx->variant_used = shape.variant;
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
@@ -737,14 +839,15 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
                     ctx->num_phi);
     double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
     // graph key: everything that changes the captured launches
-    const int key = (o.precision << 0) | (o.precond << 1) | ((o.direct_sweep ? 1 : 0) << 2) | (block << 3);
+    const int key = (o.precision << 0) | (o.precond << 1) | ((o.direct_sweep ? 1 : 0) << 2) | (shape.variant << 3) |
+                    (shape.block << 5);
     const bool use_graph = o.use_graph && !timing;
     if (use_graph && ctx->graph_key != key) {
         drop_graphs(ctx);
         for (int par = 0; par < 2; par++) {
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue_iteration(ctx, &o, block, buf[par], buf[1 - par], false);
+            rc = enqueue_iteration(ctx, &o, shape, buf[par], buf[1 - par], false);
             hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
             if (rc) return rc;
             HIPCHK(ec);
@@ -762,7 +865,7 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
         if (use_graph) {
             HIPCHK(hipGraphLaunch(ctx->graph[par], ctx->stream));
         } else {
-            rc = enqueue_iteration(ctx, &o, block, buf[par], buf[1 - par], timing);
+            rc = enqueue_iteration(ctx, &o, shape, buf[par], buf[1 - par], timing);
             if (rc) return rc;
         }
         HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, ctx->margin.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -856,6 +959,7 @@ int bte_get_stats(bte_ctx* ctx, bte_stats* st) {
     st->max_levels = ctx->max_levels;
     st->lagged_edges = ctx->lagged;
     st->block = ctx->block_used;
+    st->variant = ctx->variant_used;
     return BTE_OK;
 }
 
@@ -942,7 +1046,9 @@ int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, c
     if (ctx->has_diffuse)
         hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
                            geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeB.as<double>(), ctx->ediff.as<double>());
-    launch_solve(ctx, &o, pick_block(ctx, &o), ctx->eeB.as<double>(), ctx->eeA.as<double>(), ctx->dens.as<double>());
+    Shape shape;
+    if ((rc = pick_shape(ctx, &o, shape))) return rc;
+    launch_solve(ctx, &o, shape, ctx->eeB.as<double>(), ctx->eeA.as<double>(), ctx->dens.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(x, ctx->eeA.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
     if (iters) HIPCHK(hipMemcpyAsync(iters, ctx->iters.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));

@@ -116,6 +116,27 @@ def test_full_solve_matches_oracle_and_golden(pair):
     check_against_golden(r["margins"], r["flux"], gm, gf)
 
 
+@pytest.mark.parametrize("name", ["2DM2DG", "3DM3DG", "2DM1DG"])
+def test_sweep_variants_and_block_sizes_agree(name):
+    """Every launch shape the autotuner can pick solves the same systems (FP64, same operations up
+    to the order of a 3-term sum)."""
+    say("variants", name)
+    c = capi.Case(**case_kwargs(name))
+    base = c.solve(sweep_variant=1, block=256, stop_margin=1e-4)
+    for variant in (2, 3):
+        for block in (64, 128, 256, 512):
+            r = c.solve(sweep_variant=variant, block=block, stop_margin=1e-4)
+            st = c.stats()
+            assert st["block"] == block and st["variant"] in (variant, 2)
+            assert r["iters"] == base["iters"]
+            np.testing.assert_allclose(r["T"], base["T"], rtol=1e-13)
+            np.testing.assert_allclose(r["flux"], base["flux"], rtol=1e-9, atol=1e-12 * np.abs(base["flux"]).max())
+    r = c.solve(stop_margin=1e-4)  # autotuned
+    st = c.stats()
+    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3)
+    np.testing.assert_allclose(r["T"], base["T"], rtol=1e-13)
+
+
 def test_default_stop_and_repeat_determinism():
     kw = case_kwargs("2DM2DG")
     c = capi.Case(**kw)
