@@ -777,7 +777,9 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
     return BTE_OK;
 }
 
-int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh) {
+// allow_tune: the autotuner overwrites the intensity buffers and the density, so it only runs where
+// that state is initialised afterwards (bte_solve); elsewhere an untuned form uses {256, 2}.
+int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh, bool allow_tune) {
     const bool direct = direct_path(ctx, o);
     int& tb = ctx->tuned_block[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct];
     int& tv = ctx->tuned_variant[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct];
@@ -785,6 +787,11 @@ int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh) {
     if (fixed_block && (!direct || o->sweep_variant != 0)) {
         sh.block = o->block;
         sh.variant = direct ? o->sweep_variant : 0;
+        return BTE_OK;
+    }
+    if (!tb && !allow_tune) {
+        sh.block = fixed_block ? o->block : 256;
+        sh.variant = direct ? (o->sweep_variant ? o->sweep_variant : 2) : 0;
         return BTE_OK;
     }
     if (!tb) {
