@@ -830,7 +830,7 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
     const int N = ctx->N, B = ctx->B, D = ctx->D, S = B * D;
     const size_t SN = (size_t)S * N;
     Shape shape;
-    if ((rc = pick_shape(ctx, &o, shape))) return rc;
+    if ((rc = pick_shape(ctx, &o, shape, true))) return rc;
     ctx->block_used = shape.block;
     ctx->variant_used = shape.variant;
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
@@ -1054,7 +1054,7 @@ int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, c
         hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
                            geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeB.as<double>(), ctx->ediff.as<double>());
     Shape shape;
-    if ((rc = pick_shape(ctx, &o, shape))) return rc;
+    if ((rc = pick_shape(ctx, &o, shape, false))) return rc;
     launch_solve(ctx, &o, shape, ctx->eeB.as<double>(), ctx->eeA.as<double>(), ctx->dens.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(x, ctx->eeA.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
