@@ -130,6 +130,20 @@ int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const
  * BTE_ERR_NOCONV when some inner solve hit inner_maxit (results are still filled in). */
 int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res);
 
+/* The same iteration in steps (bte_solve = bte_begin + bte_step(max_iter) + bte_finish), for
+ * callers that inspect or correct the temperature field between outer iterations.
+ * bte_begin: prepares and resets the state (e = 0, density = 0 as StaticBTESolver.cpp:702-705).
+ * bte_step: runs up to n_iter outer iterations, stopping early once margin <= stop_margin.
+ * bte_get_temperature: T[n_cell] and T_band[n_cell*n_band] (cell-major) of the last iteration.
+ * bte_set_temperature: replaces T and the emission density C_b/Omega (T - T_ref) used by the next
+ *   iteration (the boundary intensities of the last iteration are kept).
+ * bte_finish: heat flux and outputs of the last iteration, as bte_solve. */
+int bte_begin(bte_ctx* ctx, const bte_opts* opts);
+int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged);
+int bte_get_temperature(bte_ctx* ctx, double* T, double* T_band);
+int bte_set_temperature(bte_ctx* ctx, const double* T);
+int bte_finish(bte_ctx* ctx, bte_result* res);
+
 /* Final intensities e[b][d][c] ([n_band*n_dir*n_cell], ee_curr of the last iteration). */
 int bte_get_intensity(bte_ctx* ctx, double* ee);
 

@@ -64,7 +64,8 @@ class Stats(C.Structure):
 # symbols declared in include/bte.h and include/bte_case.h
 HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_last_error", "bte_set_geometry",
                "bte_set_quadrature", "bte_set_bands", "bte_set_bcs", "bte_solve", "bte_get_intensity",
-               "bte_get_stats", "bte_debug_assemble", "bte_debug_rhs", "bte_debug_inner"]
+               "bte_get_stats", "bte_debug_assemble", "bte_debug_rhs", "bte_debug_inner", "bte_begin", "bte_step",
+               "bte_get_temperature", "bte_set_temperature", "bte_finish"]
 CASE_SYMBOLS = ["bte_case_open", "bte_case_close", "bte_case_sizes", "bte_case_params", "bte_case_solve",
                 "bte_case_geometry", "bte_case_quadrature", "bte_case_ctx", "bte_case_error"]
 
@@ -97,6 +98,11 @@ def lib():
     L.bte_debug_assemble.argtypes = [vp, C.c_int, C.c_int, dp, dp]
     L.bte_debug_rhs.argtypes = [vp, dp, dp, dp]
     L.bte_debug_inner.argtypes = [vp, C.POINTER(Opts), dp, dp, dp, ip]
+    L.bte_begin.argtypes = [vp, C.POINTER(Opts)]
+    L.bte_step.argtypes = [vp, C.c_int, ip, ip]
+    L.bte_get_temperature.argtypes = [vp, dp, dp]
+    L.bte_set_temperature.argtypes = [vp, dp]
+    L.bte_finish.argtypes = [vp, C.POINTER(Result)]
     L.bte_case_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_int,
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
     L.bte_case_close.argtypes = [vp]
@@ -230,6 +236,44 @@ class Case:
         dens = np.ascontiguousarray(dens, dtype=np.float64)
         self._check(lib().bte_debug_inner(ctx, C.byref(opts), _dp(ee_prev), _dp(dens), _dp(out), _ip(its)), ctx)
         return out.reshape(self.N_band, self.N_dir, self.N_cell), its
+
+    # ---- stepping (bte_begin / bte_step / bte_get_temperature / bte_set_temperature / bte_finish) ----
+    def begin(self, opts: Opts | None = None, **kw):
+        ctx = self.ctx()
+        self._run_opts = opts if opts is not None else default_opts(**kw)
+        self._check(lib().bte_begin(ctx, C.byref(self._run_opts)), ctx)
+
+    def step(self, n):
+        ctx = self.ctx()
+        done, conv = C.c_int(), C.c_int()
+        self._check(lib().bte_step(ctx, n, C.byref(done), C.byref(conv)), ctx)
+        return done.value, bool(conv.value)
+
+    def temperature(self, with_bands=False):
+        ctx = self.ctx()
+        T = np.zeros(self.N_cell)
+        Tb = np.zeros(self.N_cell * self.N_band) if with_bands else None
+        self._check(lib().bte_get_temperature(ctx, _dp(T), _dp(Tb) if with_bands else None), ctx)
+        return (T, Tb.reshape(self.N_cell, self.N_band)) if with_bands else T
+
+    def set_temperature(self, T):
+        ctx = self.ctx()
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        self._check(lib().bte_set_temperature(ctx, _dp(T)), ctx)
+
+    def finish(self):
+        ctx = self.ctx()
+        N, B, nbc = self.N_cell, self.N_band, self.n_bc
+        cap = max(1, self._run_opts.max_iter)
+        T, Tb, fl, bfl, m = np.zeros(N), np.zeros(N * B), np.zeros(max(nbc, 1)), np.zeros(max(nbc, 1) * B), np.zeros(cap)
+        r = Result()
+        r.T, r.T_band, r.bc_flux, r.bc_band_flux, r.margins = _dp(T), _dp(Tb), _dp(fl), _dp(bfl), _dp(m)
+        r.margins_capacity = cap
+        rc = self._check(lib().bte_finish(ctx, C.byref(r)), ctx)
+        it = r.iterations
+        return dict(T=T, T_band=Tb.reshape(N, B), flux=fl[:nbc], band_flux=bfl[:nbc * B].reshape(nbc, B),
+                    margins=m[:min(it, cap)].copy(), iters=it, converged=bool(r.converged),
+                    inner_iters=int(r.inner_iters), solve_seconds=r.solve_seconds, status=rc)
 
     def intensity(self):
         ctx = self.ctx()

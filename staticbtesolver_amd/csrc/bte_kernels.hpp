@@ -826,6 +826,13 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
     }
 }
 
+// Emission density of a given temperature field (StaticBTESolver.cpp:115-119).
+__global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x)
+        for (int b = 0; b < bands.B; b++)
+            dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (T[c] - bands.T_ref);
+}
+
 // Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).
 __global__ void k_margin(int nparts, const double* __restrict__ partial, double scale, double* __restrict__ out) {
     __shared__ double lds[4];

@@ -85,6 +85,13 @@ struct bte_ctx {
     int block_used = 0, variant_used = 0;
     // autotuned launch shape per solver form: work-group size and sweep variant (0 = not tuned)
     int tuned_block[2][2][2] = {};    // [precision][precond][direct]
+    // iteration state between bte_begin and bte_finish
+    bte_opts run_opts{};
+    int run_shape_block = 256, run_shape_variant = 2;
+    bool run_graph = false, run_active = false, run_converged = false;
+    int run_iter = 0, run_accepted = 0, run_proposed = 0;
+    std::vector<double> run_margins;
+    double run_solve_seconds = 0, run_setup_seconds = 0;
     int tuned_variant[2][2][2] = {};  // sweep variant for the direct form
 };
 
@@ -810,21 +817,26 @@ int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh, bool allow_tune) {
 
 extern "C" {
 
-int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
-    if (!ctx || !res) return BTE_ERR_ARG;
+static int check_opts(bte_ctx* ctx, const bte_opts& o) {
+    if (o.precision != BTE_PREC_FP64 && o.precision != BTE_PREC_MIXED) SETERR(BTE_ERR_ARG, "unknown precision");
+    if (o.precond != BTE_PC_SWEEP && o.precond != BTE_PC_JACOBI) SETERR(BTE_ERR_ARG, "unknown preconditioner");
+    if (o.inner_maxit <= 0 || o.inner_tol <= 0) SETERR(BTE_ERR_ARG, "inner_maxit and inner_tol must be positive");
+    return BTE_OK;
+}
+
+int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
+    if (!ctx) return BTE_ERR_ARG;
     bte_opts o;
     if (opts)
         o = *opts;
     else
         bte_default_opts(&o);
-    if (o.precision != BTE_PREC_FP64 && o.precision != BTE_PREC_MIXED) SETERR(BTE_ERR_ARG, "unknown precision");
-    if (o.precond != BTE_PC_SWEEP && o.precond != BTE_PC_JACOBI) SETERR(BTE_ERR_ARG, "unknown preconditioner");
-    if (o.inner_maxit <= 0 || o.inner_tol <= 0) SETERR(BTE_ERR_ARG, "inner_maxit and inner_tol must be positive");
+    int rc = check_opts(ctx, o);
+    if (rc) return rc;
     HIPCHK(hipSetDevice(ctx->device));
     const bool timing = o.timing != 0;
     auto t0 = std::chrono::steady_clock::now();
-    int rc = prepare(ctx, timing);
-    if (rc) return rc;
+    if ((rc = prepare(ctx, timing))) return rc;
     if (o.precision == BTE_PREC_MIXED && (rc = ensure_fp32(ctx))) return rc;
     if ((rc = ensure_work(ctx, o.precision))) return rc;
     const int N = ctx->N, B = ctx->B, D = ctx->D, S = B * D;
@@ -836,14 +848,10 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->T.p, 0, sizeof(double) * N, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->iters.p, 0, sizeof(int) * S, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    auto t1 = std::chrono::steady_clock::now();
-    res->setup_seconds = std::chrono::duration<double>(t1 - t0).count();
-    if (o.verbose)
-        std::printf("N_cell: %d\nN_dir: %d\nN_band: %d\nnum_theta: %d\nnum_phi: %d\n\n", N, D, B, ctx->num_theta,
-                    ctx->num_phi);
     double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
     // graph key: everything that changes the captured launches
     const int key = (o.precision << 0) | (o.precond << 1) | ((o.direct_sweep ? 1 : 0) << 2) | (shape.variant << 3) |
@@ -863,42 +871,122 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
         }
         ctx->graph_key = key;
     }
-    int it = 0;
-    double m = 0;
-    bool conv = false;
-    for (; it < o.max_iter; it++) {
-        // iteration parity p: ee_prev = buf[p], ee_curr = buf[1-p]; iteration 0 reads the zero buffer
-        const int par = it & 1;
-        if (use_graph) {
-            HIPCHK(hipGraphLaunch(ctx->graph[par], ctx->stream));
-        } else {
-            rc = enqueue_iteration(ctx, &o, shape, buf[par], buf[1 - par], timing);
-            if (rc) return rc;
-        }
-        HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, ctx->margin.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        m = *ctx->h_margin_pinned;
-        if (res->margins && it < res->margins_capacity) res->margins[it] = m;
+    ctx->run_opts = o;
+    ctx->run_shape_block = shape.block;
+    ctx->run_shape_variant = shape.variant;
+    ctx->run_graph = use_graph;
+    ctx->run_iter = 0;
+    ctx->run_converged = false;
+    ctx->run_active = true;
+    ctx->run_margins.clear();
+    ctx->run_solve_seconds = 0;
+    ctx->ee_final = nullptr;
+    ctx->run_setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (o.verbose)
+        std::printf("N_cell: %d\nN_dir: %d\nN_band: %d\nnum_theta: %d\nnum_phi: %d\n\n", N, D, B, ctx->num_theta,
+                    ctx->num_phi);
+    return BTE_OK;
+}
+
+// Runs one plain outer iteration (diffuse data, inner solves, T update, margin) and returns the margin.
+static int run_iteration(bte_ctx* ctx, double& m) {
+    const bte_opts& o = ctx->run_opts;
+    double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
+    const int par = ctx->run_iter & 1;  // iteration parity: ee_prev = buf[par], ee_curr = buf[1-par]
+    if (ctx->run_graph) {
+        HIPCHK(hipGraphLaunch(ctx->graph[par], ctx->stream));
+    } else {
+        Shape sh;
+        sh.block = ctx->run_shape_block;
+        sh.variant = ctx->run_shape_variant;
+        int rc = enqueue_iteration(ctx, &o, sh, buf[par], buf[1 - par], o.timing != 0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, ctx->margin.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    m = *ctx->h_margin_pinned;
+    ctx->ee_final = buf[1 - par];
+    ctx->run_iter++;
+    return BTE_OK;
+}
+
+int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (!ctx->run_active) SETERR(BTE_ERR_STATE, "bte_step: call bte_begin first");
+    HIPCHK(hipSetDevice(ctx->device));
+    const bte_opts& o = ctx->run_opts;
+    auto t0 = std::chrono::steady_clock::now();
+    int done = 0;
+    while (done < n_iter && !ctx->run_converged) {
+        double m = 0;
+        int rc = run_iteration(ctx, m);
+        if (rc) return rc;
+        done++;
+        ctx->run_margins.push_back(m);
         if (o.verbose) {
             std::printf("----------------------------------------------------------------------------------\n");
-            std::printf("Iteration #%d\t Margin per band per cell: %g\n", it, m);
+            std::printf("Iteration #%d\t Margin per band per cell: %g\n", ctx->run_iter - 1, m);
         }
-        ctx->ee_final = buf[1 - par];
         if (m <= o.stop_margin) {
-            conv = true;
-            it++;
+            ctx->run_converged = true;
             if (o.verbose) std::printf("\n");
-            break;
         }
     }
-    auto t2 = std::chrono::steady_clock::now();
-    res->solve_seconds = std::chrono::duration<double>(t2 - t1).count();
+    ctx->run_solve_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (n_done) *n_done = done;
+    if (converged) *converged = ctx->run_converged ? 1 : 0;
+    return BTE_OK;
+}
+
+int bte_get_temperature(bte_ctx* ctx, double* T, double* T_band) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (!ctx->run_active) SETERR(BTE_ERR_STATE, "no iteration state");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int N = ctx->N, B = ctx->B;
+    if (T) HIPCHK(hipMemcpyAsync(T, ctx->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> hTb;
+    if (T_band) {
+        hTb.resize((size_t)B * N);
+        HIPCHK(hipMemcpyAsync(hTb.data(), ctx->Tband.p, sizeof(double) * B * N, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (T_band)
+        for (int c = 0; c < N; c++)
+            for (int b = 0; b < B; b++) T_band[(size_t)c * B + b] = hTb[(size_t)b * N + c];
+    return BTE_OK;
+}
+
+int bte_set_temperature(bte_ctx* ctx, const double* T) {
+    if (!ctx || !T) return BTE_ERR_ARG;
+    if (!ctx->run_active) SETERR(BTE_ERR_STATE, "no iteration state");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(ctx->T.p, T, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_set_density, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, bands_of(ctx),
+                       ctx->T.as<double>(), ctx->dens.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return BTE_OK;
+}
+
+int bte_finish(bte_ctx* ctx, bte_result* res) {
+    if (!ctx || !res) return BTE_ERR_ARG;
+    if (!ctx->run_active) SETERR(BTE_ERR_STATE, "bte_finish: call bte_begin first");
+    HIPCHK(hipSetDevice(ctx->device));
+    const bte_opts& o = ctx->run_opts;
+    const int N = ctx->N, B = ctx->B, S = B * ctx->D;
+    const int it = ctx->run_iter;
+    res->setup_seconds = ctx->run_setup_seconds;
+    res->solve_seconds = ctx->run_solve_seconds;
     res->iterations = it;
-    res->converged = conv ? 1 : 0;
+    res->converged = ctx->run_converged ? 1 : 0;
+    res->learned_accepted = ctx->run_accepted;
+    res->learned_proposed = ctx->run_proposed;
+    if (res->margins)
+        for (int i = 0; i < it && i < res->margins_capacity; i++) res->margins[i] = ctx->run_margins[i];
     // heat flux of the last iterate
     const int nbc = (int)ctx->h_bci.size();
     if (nbc > 0 && it > 0) {
-        Timer tm(ctx, KA_FLUX, timing);
+        Timer tm(ctx, KA_FLUX, o.timing != 0);
         hipLaunchKernelGGL(k_flux<256>, dim3(nbc * B), dim3(256), 0, ctx->stream, geo_of(ctx), quad_of(ctx), bnd_of(ctx),
                            bands_of(ctx), ctx->DM, ctx->ee_final, ctx->bflux.as<double>());
         HIPCHK(hipGetLastError());
@@ -909,7 +997,7 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
     std::vector<double> hT(N), hTb((size_t)B * N), hbf((size_t)nbc * B);
     HIPCHK(hipMemcpyAsync(hT.data(), ctx->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(hTb.data(), ctx->Tband.p, sizeof(double) * B * N, hipMemcpyDeviceToHost, ctx->stream));
-    if (nbc > 0)
+    if (nbc > 0 && it > 0)
         HIPCHK(hipMemcpyAsync(hbf.data(), ctx->bflux.p, sizeof(double) * nbc * B, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     res->inner_iters = 0;
@@ -929,7 +1017,6 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
             for (int b = 0; b < B; b++) s += hbf[(size_t)k * B + b];
             res->bc_flux[k] = s;
         }
-    // SpMV accounting: rows processed = inner iterations x 2 products (sweep path: 1 per early exit)
     ctx->spmv_rows = res->inner_iters * 2LL * N;
     if (o.verbose)
         for (int k = 0; k < nbc; k++) {
@@ -937,11 +1024,26 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
             for (int b = 0; b < B; b++) s += hbf[(size_t)k * B + b];
             std::printf("heat flux of Boundary Condition #%d: %g\n", -1 - ctx->h_bci[k], s);
         }
+    ctx->run_active = false;
     if (res->inner_failures > 0) {
         ctx->err = "some inner solves reached inner_maxit";
         return BTE_ERR_NOCONV;
     }
     return BTE_OK;
+}
+
+int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
+    if (!ctx || !res) return BTE_ERR_ARG;
+    bte_opts o;
+    if (opts)
+        o = *opts;
+    else
+        bte_default_opts(&o);
+    int rc = bte_begin(ctx, &o);
+    if (rc) return rc;
+    ctx->run_accepted = ctx->run_proposed = 0;
+    if ((rc = bte_step(ctx, o.max_iter, nullptr, nullptr))) return rc;
+    return bte_finish(ctx, res);
 }
 
 int bte_get_intensity(bte_ctx* ctx, double* ee) {
