@@ -96,6 +96,7 @@ private:
     bte_ctx* ctx_ = nullptr;
     int device_ = -1;
     bool verbose_ = true;
+    bool uploaded_ = false;  // the context holds this problem (cleared by setParam)
     int status_ = 0;
     std::string error_;
 };

@@ -269,6 +269,7 @@ void drop_graphs(bte_ctx* ctx) {
             g = nullptr;
         }
     ctx->graph_key = -1;
+    ctx->run_active = false;  // captured launches referenced buffers that may change
 }
 
 }  // namespace
@@ -894,6 +895,7 @@ static int run_iteration(bte_ctx* ctx, double& m) {
     double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
     const int par = ctx->run_iter & 1;  // iteration parity: ee_prev = buf[par], ee_curr = buf[1-par]
     if (ctx->run_graph) {
+        if (!ctx->graph[par]) SETERR(BTE_ERR_STATE, "iteration state was invalidated (inputs changed); call bte_begin");
         HIPCHK(hipGraphLaunch(ctx->graph[par], ctx->stream));
     } else {
         Shape sh;

@@ -74,6 +74,7 @@ void StaticBTESolver::setParam(int DM, int num_theta, int num_phi, double T_ref)
     solid_angle_ = 4 * kPi;
     params_set_ = false;
     preprocessed_ = false;
+    uploaded_ = false;
     if (!mesh_ || !bands_ || !bcs_) {
         fail(BTE_ERR_ARG, "StaticBTESolver: mesh, boundary conditions and bands must not be null");
         return;
@@ -322,6 +323,7 @@ int StaticBTESolver::upload() {
 void StaticBTESolver::solve(int max_iter) {
     if (preprocess() != BTE_OK) return;
     if (upload() != BTE_OK) return;
+    uploaded_ = true;
     const int nbc = (int)bcs_->size();
     cell_temperature_.assign(N_cell_, 0);
     std::vector<double> Tb((size_t)N_cell_ * N_band_), bflux(nbc), bbf((size_t)nbc * N_band_);
@@ -357,7 +359,8 @@ void StaticBTESolver::solve(int max_iter) {
 
 bte_ctx* StaticBTESolver::context() {
     if (preprocess() != BTE_OK) return nullptr;
-    if (upload() != BTE_OK) return nullptr;
+    if (!uploaded_ && upload() != BTE_OK) return nullptr;
+    uploaded_ = true;
     return ctx_;
 }
 
