@@ -64,7 +64,13 @@ typedef struct bte_opts {
     int sweep_variant;    /* direct sweep kernel: 0 autotuned, 1 basic, 2 level-ordered operands,
                              3 level-ordered operands with the intensity vector held in LDS
                              (needs n_cell * 8 bytes of LDS); default 0 */
+    int accel;            /* outer iteration: BTE_ACCEL_NONE (the reference's source iteration) or
+                             BTE_ACCEL_ANDERSON (Anderson mixing of the temperature field over the
+                             last accel_memory iterates); default BTE_ACCEL_NONE */
+    int accel_memory;     /* Anderson history length; default 20 */
 } bte_opts;
+
+enum bte_accel { BTE_ACCEL_NONE = 0, BTE_ACCEL_ANDERSON = 1 };
 
 typedef struct bte_result {
     /* caller-allocated outputs, any may be NULL */

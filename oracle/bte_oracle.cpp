@@ -1315,6 +1315,51 @@ void orc_results(orc_case* cs, double* T, double* Tband, double* flux, double* b
 // final intensities [B*D*N]
 void orc_intensity(orc_case* cs, double* ee) { std::copy(cs->ee_curr.begin(), cs->ee_curr.end(), ee); }
 
+// Stepping form of solve() for studies of outer-iteration corrections: orc_begin resets the state,
+// orc_step runs n outer iterations (returns the last margin), orc_get_T / orc_set_T read / replace
+// the cell temperature and the emission density derived from it (StaticBTESolver.cpp:115-119).
+int orc_begin(orc_case* cs) {
+    if (!preprocess(*cs)) return -1;
+    cs->ee_curr.assign((size_t)cs->N_band * cs->N_dir * cs->N_cell, 0.0);
+    std::fill(cs->dens.begin(), cs->dens.end(), 0.0);
+    std::fill(cs->T.begin(), cs->T.end(), 0.0);
+    cs->margins.clear();
+    cs->iters = 0;
+    return 0;
+}
+
+double orc_step(orc_case* cs, int n) {
+    const int N = cs->N_cell, B = cs->N_band, D = cs->N_dir;
+    Csr A;
+    std::vector<double> Re, sol;
+    double m = 0;
+    for (int it = 0; it < n; it++) {
+        cs->ee_prev = cs->ee_curr;
+        for (int b = 0; b < B; b++)
+            for (int d = 0; d < D; d++) {
+                assemble(*cs, b, d, A);
+                if (!rhs(*cs, b, d, Re)) return -1;
+                inner_solve(*cs, A, Re, sol);
+                std::copy(sol.begin(), sol.end(), cs->ee_curr.begin() + ((size_t)b * D + d) * N);
+            }
+        update_temperature(*cs);
+        m = margin(*cs);
+        cs->margins.push_back(m);
+        cs->iters++;
+    }
+    return m;
+}
+
+void orc_get_T(orc_case* cs, double* T) { std::copy(cs->T.begin(), cs->T.end(), T); }
+
+void orc_set_T(orc_case* cs, const double* T) {
+    const int N = cs->N_cell, B = cs->N_band;
+    for (int c = 0; c < N; c++) {
+        cs->T[c] = T[c];
+        for (int b = 0; b < B; b++) cs->dens[(size_t)c * B + b] = cs->Ctot[b] / cs->solid_angle * (T[c] - cs->T_ref);
+    }
+}
+
 int orc_iterations(orc_case* cs) { return cs->iters; }
 double orc_worst_resid(orc_case* cs) { return cs->worst_true_resid; }
 long long orc_inner_iters(orc_case* cs) { return cs->inner_iter_total; }

@@ -70,6 +70,12 @@ def lib():
     L.orc_gauss.argtypes = [C.c_double, C.c_double, C.c_int, dp]
     L.orc_error.restype = C.c_char_p
     L.orc_error.argtypes = [C.c_void_p]
+    L.orc_begin.restype = C.c_int
+    L.orc_begin.argtypes = [C.c_void_p]
+    L.orc_step.restype = C.c_double
+    L.orc_step.argtypes = [C.c_void_p, C.c_int]
+    L.orc_get_T.argtypes = [C.c_void_p, dp]
+    L.orc_set_T.argtypes = [C.c_void_p, dp]
     _lib = L
     return L
 
@@ -191,6 +197,23 @@ class OracleCase:
                     margins=m, iters=it, ee=ee.reshape(B, self.N_dir, N),
                     worst_resid=lib().orc_worst_resid(self.h),
                     inner_iters=int(lib().orc_inner_iters(self.h)))
+
+    # stepping form (outer-iteration correction studies)
+    def begin(self):
+        if lib().orc_begin(self.h) != 0:
+            raise RuntimeError("oracle begin: " + lib().orc_error(self.h).decode())
+
+    def step(self, n=1):
+        return lib().orc_step(self.h, n)
+
+    def get_T(self):
+        T = np.zeros(self.N_cell)
+        lib().orc_get_T(self.h, _dp(T))
+        return T
+
+    def set_T(self, T):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        lib().orc_set_T(self.h, _dp(T))
 
     def time_systems(self, begin, end):
         return lib().orc_time_systems(self.h, begin, end)

@@ -29,6 +29,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PC_SWEEP = 0
 PC_JACOBI = 1
+ACCEL_NONE = 0
+ACCEL_ANDERSON = 1
 
 
 class BTEError(RuntimeError):
@@ -42,7 +44,8 @@ class Opts(C.Structure):
                 ("stop_margin", C.c_double), ("inner_tol", C.c_double), ("inner_maxit", C.c_int),
                 ("refine_steps", C.c_int), ("block", C.c_int), ("use_graph", C.c_int), ("verbose", C.c_int),
                 ("timing", C.c_int), ("learned", C.c_int), ("learned_every", C.c_int),
-                ("direct_sweep", C.c_int), ("sweep_variant", C.c_int)]
+                ("direct_sweep", C.c_int), ("sweep_variant", C.c_int), ("accel", C.c_int),
+                ("accel_memory", C.c_int)]
 
 
 class Result(C.Structure):

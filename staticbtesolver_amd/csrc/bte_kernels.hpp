@@ -826,6 +826,69 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
     }
 }
 
+// ----------------------------------------------------------------------------------------------
+// Anderson mixing of the outer iteration (fixed-point map T -> G(T) of one source iteration).
+// History columns dX_j = x_{j+1} - x_j and dF_j = f_{j+1} - f_j (f = G(x) - x) live in ring
+// buffers [m][N]; the m x m least-squares system is solved on the host.
+// ----------------------------------------------------------------------------------------------
+__global__ void k_aa_diff(int N, int first, const double* __restrict__ g, const double* __restrict__ x,
+                          double* __restrict__ x_prev, double* __restrict__ f_prev, double* __restrict__ f_cur,
+                          double* __restrict__ dX, double* __restrict__ dF) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        const double f = g[c] - x[c];
+        if (!first) {
+            dX[c] = x[c] - x_prev[c];
+            dF[c] = f - f_prev[c];
+        }
+        x_prev[c] = x[c];
+        f_prev[c] = f;
+        f_cur[c] = f;
+    }
+}
+
+// partial[q * nblk + blk] for q < ncols: dF_q . dF_new, q in [ncols, 2 ncols): dF_{q-ncols} . f,
+// q = 2 ncols: f . f
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const double* __restrict__ dF,
+                                                  const double* __restrict__ dF_new, const double* __restrict__ f,
+                                                  double* __restrict__ partial) {
+    __shared__ double lds[BLOCK / 64];
+    const int nq = 2 * ncols + 1;
+    for (int q = 0; q < nq; q++) {
+        const double* a = (q < 2 * ncols) ? dF + (size_t)(q % ncols) * N : f;
+        const double* b = (q < ncols) ? dF_new : f;
+        double s[1] = {0};
+        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += gridDim.x * BLOCK) s[0] += a[c] * b[c];
+        block_sum<BLOCK, 1>(s, lds);
+        if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = s[0];
+    }
+}
+
+// out[q] = sum_b partial[q * nblk + b] in a fixed order
+__global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ partial, double* __restrict__ out) {
+    __shared__ double lds[4];
+    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+        double s[1] = {0};
+        for (int i = threadIdx.x; i < nblk; i += 256) s[0] += partial[(size_t)q * nblk + i];
+        block_sum<256, 1>(s, lds);
+        if (threadIdx.x == 0) out[q] = s[0];
+    }
+}
+
+__global__ void k_fill(int n, double v, double* __restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
+}
+
+// x = g - sum_j gamma_j (dX_j + dF_j)
+__global__ void k_aa_update(int N, int ncols, const double* __restrict__ gamma, const double* __restrict__ g,
+                            const double* __restrict__ dX, const double* __restrict__ dF, double* __restrict__ x) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        double v = g[c];
+        for (int j = 0; j < ncols; j++) v -= gamma[j] * (dX[(size_t)j * N + c] + dF[(size_t)j * N + c]);
+        x[c] = v;
+    }
+}
+
 // Emission density of a given temperature field (StaticBTESolver.cpp:115-119).
 __global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x)

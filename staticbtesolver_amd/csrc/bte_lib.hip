@@ -92,6 +92,13 @@ struct bte_ctx {
     int run_iter = 0, run_accepted = 0, run_proposed = 0;
     std::vector<double> run_margins;
     double run_solve_seconds = 0, run_setup_seconds = 0;
+    // Anderson mixing state (ring of m difference columns)
+    DBuf aa_x, aa_xprev, aa_fprev, aa_f, aa_dX, aa_dF, aa_part, aa_dots, aa_gamma;
+    int aa_m = 0;
+    bool aa_first = true;
+    std::vector<int> aa_order;  // active slots, oldest first
+    std::vector<double> aa_H;   // Gram matrix of the dF columns by slot [m*m]
+    double aa_best = 0;
     int tuned_variant[2][2][2] = {};  // sweep variant for the direct form
 };
 
@@ -293,6 +300,8 @@ void bte_default_opts(bte_opts* o) {
     o->learned_every = 8;
     o->direct_sweep = 1;
     o->sweep_variant = 0;
+    o->accel = BTE_ACCEL_NONE;
+    o->accel_memory = 20;
 }
 
 int bte_ctx_create(int device, bte_ctx** out) {
@@ -818,6 +827,114 @@ int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh, bool allow_tune) {
 
 extern "C" {
 
+// Solves (H + lambda I) y = b for a small symmetric positive semi-definite H by Gaussian
+// elimination with partial pivoting; lambda = 1e-12 * max diag keeps it regular.
+static void small_solve(std::vector<double> H, std::vector<double> b, int n, std::vector<double>& y) {
+    double dmax = 0;
+    for (int i = 0; i < n; i++) dmax = std::max(dmax, H[(size_t)i * n + i]);
+    for (int i = 0; i < n; i++) H[(size_t)i * n + i] += 1e-12 * dmax + 1e-300;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(H[(size_t)i * n + k]) > std::fabs(H[(size_t)p * n + k])) p = i;
+        if (p != k) {
+            for (int j = 0; j < n; j++) std::swap(H[(size_t)k * n + j], H[(size_t)p * n + j]);
+            std::swap(b[k], b[p]);
+        }
+        const double piv = H[(size_t)k * n + k];
+        for (int i = k + 1; i < n; i++) {
+            const double l = H[(size_t)i * n + k] / piv;
+            for (int j = k; j < n; j++) H[(size_t)i * n + j] -= l * H[(size_t)k * n + j];
+            b[i] -= l * b[k];
+        }
+    }
+    y.assign(n, 0.0);
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++) s -= H[(size_t)i * n + j] * y[j];
+        y[i] = s / H[(size_t)i * n + i];
+    }
+}
+
+static int aa_reset(bte_ctx* ctx) {
+    const int N = ctx->N, m = std::max(1, std::min(ctx->run_opts.accel_memory, 64));
+    ctx->aa_m = m;
+    for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f}) HIPCHK(b->ensure(sizeof(double) * N));
+    HIPCHK(ctx->aa_dX.ensure(sizeof(double) * (size_t)m * N));
+    HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * N));
+    HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
+    HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
+    HIPCHK(ctx->aa_gamma.ensure(sizeof(double) * m));
+    HIPCHK(hipMemsetAsync(ctx->aa_dX.p, 0, sizeof(double) * (size_t)m * N, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->aa_dF.p, 0, sizeof(double) * (size_t)m * N, ctx->stream));
+    ctx->aa_first = true;
+    ctx->aa_order.clear();
+    ctx->aa_H.assign((size_t)m * m, 0.0);
+    ctx->aa_best = 0;
+    return BTE_OK;
+}
+
+// One Anderson step after an outer iteration produced g = G(x) in ctx->T: forms the next iterate
+// x in aa_x and the emission density of x for the following iteration.
+static int aa_step(bte_ctx* ctx) {
+    const int N = ctx->N, m = ctx->aa_m;
+    const int gb = grid_for(N, 256);
+    int slot = 0;
+    if (!ctx->aa_first) {
+        if ((int)ctx->aa_order.size() < m) {
+            slot = (int)ctx->aa_order.size();
+        } else {
+            slot = ctx->aa_order.front();
+            ctx->aa_order.erase(ctx->aa_order.begin());
+        }
+    }
+    const double* g = ctx->T.as<double>();
+    hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->aa_first ? 1 : 0, g,
+                       ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(),
+                       ctx->aa_f.as<double>(), ctx->aa_dX.as<double>() + (size_t)slot * N,
+                       ctx->aa_dF.as<double>() + (size_t)slot * N);
+    if (ctx->aa_first) {
+        // plain step: x_1 = G(x_0)
+        HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->aa_first = false;
+    } else {
+        ctx->aa_order.push_back(slot);
+        const int nq = 2 * m + 1, nblk = 64;
+        hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk), dim3(256), 0, ctx->stream, N, m, ctx->aa_dF.as<double>(),
+                           ctx->aa_dF.as<double>() + (size_t)slot * N, ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
+        hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
+                           ctx->aa_part.as<double>(), ctx->aa_dots.as<double>());
+        std::vector<double> dots(nq);
+        HIPCHK(hipMemcpyAsync(dots.data(), ctx->aa_dots.p, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
+        const double ff = dots[2 * m];
+        // divergence guard: restart the history when the residual grew far beyond the best seen
+        if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
+        std::vector<double> gamma(m, 0.0);
+        if (ff > 1e6 * ctx->aa_best) {
+            ctx->aa_order.clear();
+        } else {
+            const int n = (int)ctx->aa_order.size();
+            std::vector<double> H((size_t)n * n), b(n), y;
+            for (int i = 0; i < n; i++) {
+                b[i] = dots[m + ctx->aa_order[i]];
+                for (int j = 0; j < n; j++) H[(size_t)i * n + j] = ctx->aa_H[(size_t)ctx->aa_order[i] * m + ctx->aa_order[j]];
+            }
+            small_solve(H, b, n, y);
+            for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
+        }
+        HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma.data(), sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, N, m, ctx->aa_gamma.as<double>(), g,
+                           ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(), ctx->aa_x.as<double>());
+        HIPCHK(hipStreamSynchronize(ctx->stream));  // gamma lives on the host stack
+    }
+    hipLaunchKernelGGL(k_set_density, dim3(gb), dim3(256), 0, ctx->stream, N, bands_of(ctx), ctx->aa_x.as<double>(),
+                       ctx->dens.as<double>());
+    HIPCHK(hipGetLastError());
+    return BTE_OK;
+}
+
 static int check_opts(bte_ctx* ctx, const bte_opts& o) {
     if (o.precision != BTE_PREC_FP64 && o.precision != BTE_PREC_MIXED) SETERR(BTE_ERR_ARG, "unknown precision");
     if (o.precond != BTE_PC_SWEEP && o.precond != BTE_PC_JACOBI) SETERR(BTE_ERR_ARG, "unknown preconditioner");
@@ -873,6 +990,14 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
         ctx->graph_key = key;
     }
     ctx->run_opts = o;
+    if (o.accel == BTE_ACCEL_ANDERSON) {
+        if ((rc = aa_reset(ctx))) return rc;
+        // x_0 = T_ref everywhere: the zero emission density of the first iteration
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->T_ref,
+                           ctx->aa_x.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
     ctx->run_shape_block = shape.block;
     ctx->run_shape_variant = shape.variant;
     ctx->run_graph = use_graph;
@@ -932,6 +1057,8 @@ int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged) {
         if (m <= o.stop_margin) {
             ctx->run_converged = true;
             if (o.verbose) std::printf("\n");
+        } else if (o.accel == BTE_ACCEL_ANDERSON) {
+            if ((rc = aa_step(ctx))) return rc;
         }
     }
     ctx->run_solve_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
