@@ -875,6 +875,35 @@ __global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ parti
     }
 }
 
+// The state of the outer iteration is the temperature field plus the intensities at cells with a
+// reflecting (diffuse or specular) boundary face -- the only parts of the previous iterate the next
+// right-hand sides read.  Intensities are carried as temperature equivalents e * Omega / C_b.
+// z = [T (N) ; e_bnd (S x nb)].
+__global__ void k_state_gather(int N, int D, int S, int nb, const int* __restrict__ bcells, const double* __restrict__ sc,
+                               const double* __restrict__ T, const double* __restrict__ ee, double* __restrict__ z) {
+    const long long total = N + (long long)S * nb;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        if (t < N) {
+            z[t] = T[t];
+        } else {
+            const long long u = t - N;
+            const int s = (int)(u / nb), j = (int)(u % nb);
+            z[t] = ee[(size_t)s * N + bcells[j]] * sc[s / D];
+        }
+    }
+}
+
+__global__ void k_state_scatter(int N, int D, int S, int nb, const int* __restrict__ bcells,
+                                const double* __restrict__ sc, const double* __restrict__ z, double* __restrict__ ee) {
+    const long long total = (long long)S * nb;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(u / nb), j = (int)(u % nb);
+        ee[(size_t)s * N + bcells[j]] = z[N + u] / sc[s / D];
+    }
+}
+
 __global__ void k_fill(int n, double v, double* __restrict__ x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
 }

@@ -93,8 +93,9 @@ struct bte_ctx {
     std::vector<double> run_margins;
     double run_solve_seconds = 0, run_setup_seconds = 0;
     // Anderson mixing state (ring of m difference columns)
-    DBuf aa_x, aa_xprev, aa_fprev, aa_f, aa_dX, aa_dF, aa_part, aa_dots, aa_gamma;
-    int aa_m = 0;
+    DBuf aa_x, aa_xprev, aa_fprev, aa_f, aa_dX, aa_dF, aa_part, aa_dots, aa_gamma, aa_g, aa_bcells, aa_sc;
+    int aa_m = 0, aa_nb = 0;
+    long long aa_Z = 0;  // state length N + S * nb
     bool aa_first = true;
     std::vector<int> aa_order;  // active slots, oldest first
     std::vector<double> aa_H;   // Gram matrix of the dF columns by slot [m*m]
@@ -857,16 +858,39 @@ static void small_solve(std::vector<double> H, std::vector<double> b, int n, std
 }
 
 static int aa_reset(bte_ctx* ctx) {
-    const int N = ctx->N, m = std::max(1, std::min(ctx->run_opts.accel_memory, 64));
+    const int N = ctx->N, F = ctx->F, m = std::max(1, std::min(ctx->run_opts.accel_memory, 64));
     ctx->aa_m = m;
-    for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f}) HIPCHK(b->ensure(sizeof(double) * N));
-    HIPCHK(ctx->aa_dX.ensure(sizeof(double) * (size_t)m * N));
-    HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * N));
+    // cells whose previous intensities enter the next right-hand sides: reflecting boundary faces
+    std::vector<int> bcells;
+    for (int c = 0; c < N; c++) {
+        bool refl = false;
+        for (int f = 0; f < F; f++) {
+            const int nb = ctx->h_nbr[(size_t)c * F + f];
+            if (nb >= 0) continue;
+            for (size_t k = 0; k < ctx->h_bci.size(); k++)
+                if (ctx->h_bci[k] == nb && ctx->h_bct[k] != 1) refl = true;
+        }
+        if (refl) bcells.push_back(c);
+    }
+    ctx->aa_nb = (int)bcells.size();
+    const int S = ctx->B * ctx->D;
+    const long long Z = N + (long long)S * ctx->aa_nb;
+    ctx->aa_Z = Z;
+    std::vector<double> sc(ctx->B);
+    for (int b = 0; b < ctx->B; b++) sc[b] = ctx->solid_angle / ctx->h_C[b];
+    HIPCHK(upload(ctx->aa_bcells, bcells, ctx->stream));
+    HIPCHK(upload(ctx->aa_sc, sc, ctx->stream));
+    for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f, &ctx->aa_g})
+        HIPCHK(b->ensure(sizeof(double) * Z));
+    HIPCHK(ctx->aa_dX.ensure(sizeof(double) * (size_t)m * Z));
+    HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * Z));
     HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
     HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
     HIPCHK(ctx->aa_gamma.ensure(sizeof(double) * m));
-    HIPCHK(hipMemsetAsync(ctx->aa_dX.p, 0, sizeof(double) * (size_t)m * N, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->aa_dF.p, 0, sizeof(double) * (size_t)m * N, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->aa_dX.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->aa_dF.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->aa_x.p, 0, sizeof(double) * Z, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // sc and bcells are host temporaries
     ctx->aa_first = true;
     ctx->aa_order.clear();
     ctx->aa_H.assign((size_t)m * m, 0.0);
@@ -877,8 +901,12 @@ static int aa_reset(bte_ctx* ctx) {
 // One Anderson step after an outer iteration produced g = G(x) in ctx->T: forms the next iterate
 // x in aa_x and the emission density of x for the following iteration.
 static int aa_step(bte_ctx* ctx) {
-    const int N = ctx->N, m = ctx->aa_m;
-    const int gb = grid_for(N, 256);
+    const int N = ctx->N, m = ctx->aa_m, S = ctx->B * ctx->D, nb = ctx->aa_nb;
+    const int Z = (int)ctx->aa_Z;
+    const int gb = grid_for(Z, 256);
+    // g = G(x): the new temperature field and reflecting-boundary intensities
+    hipLaunchKernelGGL(k_state_gather, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb, ctx->aa_bcells.as<int>(),
+                       ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final, ctx->aa_g.as<double>());
     int slot = 0;
     if (!ctx->aa_first) {
         if ((int)ctx->aa_order.size() < m) {
@@ -888,20 +916,20 @@ static int aa_step(bte_ctx* ctx) {
             ctx->aa_order.erase(ctx->aa_order.begin());
         }
     }
-    const double* g = ctx->T.as<double>();
-    hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->aa_first ? 1 : 0, g,
+    const double* g = ctx->aa_g.as<double>();
+    hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, Z, ctx->aa_first ? 1 : 0, g,
                        ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(),
-                       ctx->aa_f.as<double>(), ctx->aa_dX.as<double>() + (size_t)slot * N,
-                       ctx->aa_dF.as<double>() + (size_t)slot * N);
+                       ctx->aa_f.as<double>(), ctx->aa_dX.as<double>() + (size_t)slot * Z,
+                       ctx->aa_dF.as<double>() + (size_t)slot * Z);
     if (ctx->aa_first) {
         // plain step: x_1 = G(x_0)
-        HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * N, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
         ctx->aa_first = false;
     } else {
         ctx->aa_order.push_back(slot);
         const int nq = 2 * m + 1, nblk = 64;
-        hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk), dim3(256), 0, ctx->stream, N, m, ctx->aa_dF.as<double>(),
-                           ctx->aa_dF.as<double>() + (size_t)slot * N, ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
+        hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
+                           ctx->aa_dF.as<double>() + (size_t)slot * Z, ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
         hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
                            ctx->aa_part.as<double>(), ctx->aa_dots.as<double>());
         std::vector<double> dots(nq);
@@ -925,12 +953,17 @@ static int aa_step(bte_ctx* ctx) {
             for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
         }
         HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma.data(), sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, N, m, ctx->aa_gamma.as<double>(), g,
+        hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, Z, m, ctx->aa_gamma.as<double>(), g,
                            ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(), ctx->aa_x.as<double>());
         HIPCHK(hipStreamSynchronize(ctx->stream));  // gamma lives on the host stack
     }
-    hipLaunchKernelGGL(k_set_density, dim3(gb), dim3(256), 0, ctx->stream, N, bands_of(ctx), ctx->aa_x.as<double>(),
-                       ctx->dens.as<double>());
+    // the next iteration reads the mixed state: emission density of its T part, boundary intensities
+    hipLaunchKernelGGL(k_set_density, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, bands_of(ctx),
+                       ctx->aa_x.as<double>(), ctx->dens.as<double>());
+    if (nb > 0)
+        hipLaunchKernelGGL(k_state_scatter, dim3(grid_for((long long)S * nb, 256)), dim3(256), 0, ctx->stream, N,
+                           ctx->D, S, nb, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->aa_x.as<double>(),
+                           ctx->ee_final);
     HIPCHK(hipGetLastError());
     return BTE_OK;
 }
