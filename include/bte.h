@@ -155,6 +155,27 @@ int bte_get_intensity(bte_ctx* ctx, double* ee);
 
 int bte_get_stats(bte_ctx* ctx, bte_stats* stats);
 
+/* ---- learned correction: mesh graph network (no counterpart in the reference) ----
+ * Nodes are cells, edges interior faces.  16 input features per node (zero padded), hidden width
+ * 64, `layers` message-passing layers with residual connections, linear read-out; BF16 MFMA
+ * products with FP32 accumulation.  Parameters are an FP32 vector of bte_gnn_num_params() values
+ * (layout restated in oracle/gnn_ref.py).  Training: bte_gnn_backward (forward + weighted squared
+ * error + gradients) then bte_gnn_adam. */
+typedef struct bte_gnn bte_gnn;
+int bte_gnn_create(int device, int layers, unsigned long long seed, bte_gnn** out);
+void bte_gnn_destroy(bte_gnn* g);
+const char* bte_gnn_last_error(const bte_gnn* g);
+int bte_gnn_num_params(const bte_gnn* g);
+int bte_gnn_get_params(bte_gnn* g, float* params);
+int bte_gnn_set_params(bte_gnn* g, const float* params);  /* also resets the Adam state */
+int bte_gnn_get_grads(bte_gnn* g, float* grads);
+/* nbr: [n_node * n_face] face table as in bte_set_geometry (negative = boundary) */
+int bte_gnn_set_graph(bte_gnn* g, int n_node, int n_face, const int* nbr);
+int bte_gnn_forward(bte_gnn* g, const float* x /*[n_node*16]*/, float* y /*[n_node], may be NULL*/);
+int bte_gnn_backward(bte_gnn* g, const float* x, const float* target, const float* weight /*may be NULL*/,
+                     double* loss);
+int bte_gnn_adam(bte_gnn* g, float lr, float beta1, float beta2, float eps, float weight_decay);
+
 /* ---- piecewise entry points (used by the parity tests; same kernels as bte_solve) ---- */
 
 /* Assembly of system (band, dir), StaticBTESolver.cpp:122-160, in dense face-slot form:

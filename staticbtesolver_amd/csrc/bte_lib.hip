@@ -12,31 +12,12 @@
 #include <vector>
 
 #include "../../include/bte.h"
+#include "bte_internal.hpp"
 #include "bte_kernels.hpp"
 
 using namespace bte;
 
 namespace {
-
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t ensure(size_t n) {
-        if (n <= bytes && p) return hipSuccess;
-        release();
-        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 16));
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    template <class T>
-    T* as() const { return (T*)p; }
-};
 
 const char* kKernelNames[] = {"assemble", "sweep_schedule", "diffuse", "solve", "temperature", "margin", "flux"};
 enum { KA_ASM = 0, KA_SCHED, KA_DIFF, KA_SOLVE, KA_TEMP, KA_MARGIN, KA_FLUX, KA_COUNT };
