@@ -68,7 +68,9 @@ class Stats(C.Structure):
 HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_last_error", "bte_set_geometry",
                "bte_set_quadrature", "bte_set_bands", "bte_set_bcs", "bte_solve", "bte_get_intensity",
                "bte_get_stats", "bte_debug_assemble", "bte_debug_rhs", "bte_debug_inner", "bte_begin", "bte_step",
-               "bte_get_temperature", "bte_set_temperature", "bte_finish"]
+               "bte_get_temperature", "bte_set_temperature", "bte_finish", "bte_gnn_create", "bte_gnn_destroy",
+               "bte_gnn_last_error", "bte_gnn_num_params", "bte_gnn_get_params", "bte_gnn_set_params",
+               "bte_gnn_get_grads", "bte_gnn_set_graph", "bte_gnn_forward", "bte_gnn_backward", "bte_gnn_adam"]
 CASE_SYMBOLS = ["bte_case_open", "bte_case_close", "bte_case_sizes", "bte_case_params", "bte_case_solve",
                 "bte_case_geometry", "bte_case_quadrature", "bte_case_ctx", "bte_case_error"]
 
