@@ -56,7 +56,10 @@ typedef struct bte_opts {
     int use_graph;        /* capture each outer iteration as a hipGraph; default 1 */
     int verbose;          /* print the reference's per-iteration stdout lines; default 0 */
     int timing;           /* per-kernel hipEvent timing into bte_stats; default 0 */
-    int learned;          /* apply the loaded learned correction every learned_every iterations */
+    int learned;          /* learned correction by the network of bte_attach_gnn: proposed after
+                             state steps 1, 1 + learned_every, ... and kept only if the residual of
+                             the iteration from the corrected state does not grow; stops proposing
+                             after two rejections in a row.  Works with either accel; default 0 */
     int learned_every;    /* default 8 */
     int direct_sweep;     /* BTE_PC_SWEEP + FP64: when no upwind edge had to be lagged (acyclic
                              upwind graph), solve each system by one downwind sweep with the RHS
@@ -175,6 +178,18 @@ int bte_gnn_forward(bte_gnn* g, const float* x /*[n_node*16]*/, float* y /*[n_no
 int bte_gnn_backward(bte_gnn* g, const float* x, const float* target, const float* weight /*may be NULL*/,
                      double* loss);
 int bte_gnn_adam(bte_gnn* g, float lr, float beta1, float beta2, float eps, float weight_decay);
+
+/* Attaches the network used by bte_opts.learned (NULL detaches).  Not owned: it must outlive every
+ * learned solve of the context and live on the same device.  bte_begin points its graph at the
+ * context's mesh whenever that changed (which replaces any graph set by bte_gnn_set_graph). */
+int bte_attach_gnn(bte_ctx* ctx, bte_gnn* g);
+
+/* Network inputs of the current iteration state, for building training pairs: X [n_cell * 16]
+ * (node-major), x_next [n_cell] the temperature part of the state the next iteration would start
+ * from (may be NULL), span the wall-temperature span that scales the network output (correction =
+ * span * y; may be NULL).  Valid after a bte_step of an iteration begun with accel or learned that
+ * ran at least one iteration; the target for y is (T_converged - x_next) / span. */
+int bte_learn_features(bte_ctx* ctx, float* X, double* x_next, double* span);
 
 /* ---- piecewise entry points (used by the parity tests; same kernels as bte_solve) ---- */
 

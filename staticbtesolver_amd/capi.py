@@ -70,7 +70,8 @@ HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_las
                "bte_get_stats", "bte_debug_assemble", "bte_debug_rhs", "bte_debug_inner", "bte_begin", "bte_step",
                "bte_get_temperature", "bte_set_temperature", "bte_finish", "bte_gnn_create", "bte_gnn_destroy",
                "bte_gnn_last_error", "bte_gnn_num_params", "bte_gnn_get_params", "bte_gnn_set_params",
-               "bte_gnn_get_grads", "bte_gnn_set_graph", "bte_gnn_forward", "bte_gnn_backward", "bte_gnn_adam"]
+               "bte_gnn_get_grads", "bte_gnn_set_graph", "bte_gnn_forward", "bte_gnn_backward", "bte_gnn_adam",
+               "bte_attach_gnn", "bte_learn_features"]
 CASE_SYMBOLS = ["bte_case_open", "bte_case_close", "bte_case_sizes", "bte_case_params", "bte_case_solve",
                 "bte_case_geometry", "bte_case_quadrature", "bte_case_ctx", "bte_case_error"]
 
@@ -108,6 +109,8 @@ def lib():
     L.bte_get_temperature.argtypes = [vp, dp, dp]
     L.bte_set_temperature.argtypes = [vp, dp]
     L.bte_finish.argtypes = [vp, C.POINTER(Result)]
+    L.bte_attach_gnn.argtypes = [vp, vp]
+    L.bte_learn_features.argtypes = [vp, C.POINTER(C.c_float), dp, dp]
     L.bte_case_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_int,
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
     L.bte_case_close.argtypes = [vp]
@@ -278,7 +281,26 @@ class Case:
         it = r.iterations
         return dict(T=T, T_band=Tb.reshape(N, B), flux=fl[:nbc], band_flux=bfl[:nbc * B].reshape(nbc, B),
                     margins=m[:min(it, cap)].copy(), iters=it, converged=bool(r.converged),
-                    inner_iters=int(r.inner_iters), solve_seconds=r.solve_seconds, status=rc)
+                    inner_iters=int(r.inner_iters), solve_seconds=r.solve_seconds, status=rc,
+                    learned_accepted=r.learned_accepted, learned_proposed=r.learned_proposed)
+
+    # ---- learned correction ----
+    def attach_gnn(self, gnn):
+        """Network used by opts.learned (a staticbtesolver_amd.gnn.Gnn, or None to detach).  The
+        caller keeps the Gnn alive while it is attached."""
+        ctx = self.ctx()
+        self._gnn = gnn
+        self._check(lib().bte_attach_gnn(ctx, None if gnn is None else gnn.h), ctx)
+
+    def learn_features(self):
+        """(X [N_cell, 16] float32, x_next [N_cell], span) of the current iteration state."""
+        ctx = self.ctx()
+        X = np.zeros((self.N_cell, 16), dtype=np.float32)
+        x = np.zeros(self.N_cell)
+        span = C.c_double()
+        self._check(lib().bte_learn_features(ctx, X.ctypes.data_as(C.POINTER(C.c_float)), _dp(x), C.byref(span)),
+                    ctx)
+        return X, x, span.value
 
     def intensity(self):
         ctx = self.ctx()

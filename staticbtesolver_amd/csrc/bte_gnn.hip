@@ -273,7 +273,8 @@ __global__ void k_gnn_adam(int P, float* __restrict__ p, const float* __restrict
 static int grid1(long long n, int b = 256) { return (int)std::max<long long>(1, std::min<long long>((n + b - 1) / b, 65536)); }
 
 int gnn_ensure(GnnNet* net, int n) {
-    if (n <= net->N && net->H.size() == (size_t)net->L + 1 && net->Z0.p) return 0;
+    if (n <= net->cap && net->H.size() == (size_t)net->L + 1 && net->Z0.p) return 0;
+    net->cap = 0;
     const size_t a = sizeof(float) * (size_t)n * kGnnHid;
     net->H.clear();
     net->M.clear();
@@ -296,25 +297,27 @@ int gnn_ensure(GnnNet* net, int n) {
     net->part_elems = (size_t)kGnnHid * kGnnHid * nblk + (size_t)n;
     if (net->part.ensure(sizeof(float) * net->part_elems) != hipSuccess) return BTE_ERR_HIP;
     if (net->red.ensure(sizeof(float) * (kGnnHid * kGnnHid + 64)) != hipSuccess) return BTE_ERR_HIP;
+    net->cap = n;
     return 0;
 }
 
-int gnn_forward_device(GnnNet* net) {
+int gnn_forward_device(GnnNet* net, hipStream_t stream) {
     const int N = net->N, L = net->L;
     const float* p = net->params.as<float>();
     const int gt = (N + 63) / 64;  // 64 rows per work-group
-    hipLaunchKernelGGL(k_gnn_enc, dim3(gt), dim3(256), 0, net->stream, N, net->X.as<float>(), p, p + kGnnIn * kGnnHid,
+    hipStream_t st = stream ? stream : net->stream;
+    hipLaunchKernelGGL(k_gnn_enc, dim3(gt), dim3(256), 0, st, N, net->X.as<float>(), p, p + kGnnIn * kGnnHid,
                        net->Z0.as<float>(), net->H[0].as<float>());
     for (int l = 0; l < L; l++) {
         const float* W1 = p + gnn_off_layer(l);
-        hipLaunchKernelGGL(k_gnn_agg, dim3(grid1((long long)N * kGnnHid)), dim3(256), 0, net->stream, N, net->F,
+        hipLaunchKernelGGL(k_gnn_agg, dim3(grid1((long long)N * kGnnHid)), dim3(256), 0, st, N, net->F,
                            net->nbr.as<int>(), net->H[l].as<float>(), net->M[l].as<float>());
-        hipLaunchKernelGGL(k_gnn_layer, dim3(gt), dim3(256), 0, net->stream, N, net->H[l].as<float>(),
-                           net->M[l].as<float>(), W1, W1 + kGnnHid * kGnnHid, W1 + 2 * kGnnHid * kGnnHid,
-                           net->Z[l].as<float>(), net->H[l + 1].as<float>());
+        hipLaunchKernelGGL(k_gnn_layer, dim3(gt), dim3(256), 0, st, N, net->H[l].as<float>(), net->M[l].as<float>(), W1,
+                           W1 + kGnnHid * kGnnHid, W1 + 2 * kGnnHid * kGnnHid, net->Z[l].as<float>(),
+                           net->H[l + 1].as<float>());
     }
     const float* wo = p + gnn_off_out(L);
-    hipLaunchKernelGGL(k_gnn_dec, dim3(grid1(N)), dim3(256), 0, net->stream, N, net->H[L].as<float>(), wo, wo + kGnnHid,
+    hipLaunchKernelGGL(k_gnn_dec, dim3(grid1(N)), dim3(256), 0, st, N, net->H[L].as<float>(), wo, wo + kGnnHid,
                        net->y.as<float>());
     return hipGetLastError() == hipSuccess ? 0 : BTE_ERR_HIP;
 }
@@ -386,8 +389,6 @@ static int gnn_backward_device(GnnNet* net, float inv_sw, bool weighted, double*
 }  // namespace bte
 
 using namespace bte;
-
-struct bte_gnn : GnnNet {};
 
 #define GCHK(expr)                                                                   \
     do {                                                                             \
@@ -483,6 +484,7 @@ int bte_gnn_set_graph(bte_gnn* g, int n_node, int n_face, const int* nbr) {
         }
     GCHK(hipSetDevice(g->device));
     g->N = 0;
+    g->graph_tag = 0;
     if (gnn_ensure(g, n_node)) {
         g->err = "out of device memory";
         return BTE_ERR_HIP;

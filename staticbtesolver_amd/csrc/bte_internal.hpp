@@ -50,6 +50,8 @@ struct GnnNet {
     int L = 4;       // message-passing layers
     int P = 0;       // number of parameters
     int N = 0, F = 0;
+    int cap = 0;                        // node capacity of the activation buffers
+    unsigned long long graph_tag = 0;   // geometry tag of the solver mesh the graph holds (0: none)
     DBuf params, grads, adam_m, adam_v;
     int adam_t = 0;
     DBuf nbr, deg;                      // graph (face table, interior degree)
@@ -65,10 +67,14 @@ inline int gnn_off_layer(int l) { return kGnnIn * kGnnHid + kGnnHid + l * (2 * k
 inline int gnn_off_out(int L) { return gnn_off_layer(L); }
 inline int gnn_num_params(int L) { return gnn_off_out(L) + kGnnHid + 1; }
 
-// Forward pass on device data already in net->X (N rows); writes net->y.  Used by the solver's
-// learned correction.  Returns 0 or a negative bte_status.
-int gnn_forward_device(GnnNet* net);
+// Forward pass on device data already in net->X (N rows); writes net->y.  Launches on `stream`
+// (the network's own stream when null).  Used by the solver's learned correction.  Returns 0 or
+// a negative bte_status.
+int gnn_forward_device(GnnNet* net, hipStream_t stream = nullptr);
 // Ensures activation buffers for n nodes.
 int gnn_ensure(GnnNet* net, int n);
 
 }  // namespace bte
+
+// The opaque handle of the C ABI is the network state itself.
+struct bte_gnn : bte::GnnNet {};

@@ -904,6 +904,65 @@ __global__ void k_state_scatter(int N, int D, int S, int nb, const int* __restri
     }
 }
 
+// ----------------------------------------------------------------------------------------------
+// Learned correction.  Node features of the state the next outer iteration would start from
+// (temperature part of the fixed-point state; x = last evaluated state, f = G(x) - x its
+// residual, x_next = the mixed next state, s = RMS of f over cells):
+//   0 f / s      1 (x_next - x) / s      2 (g - g_prev) / s (last history column, 0 without one)
+//   3 (x_next - T_mid) / T_span      4 log10(s / T_span) / 4 (uniform)
+//   5 (x_next - T_wall_interp) / T_span      6 progress (outer iterations / 100, capped at 2)   7 0
+//   8..15 static per-cell features (wall flags, graph distances, Knudsen numbers; host-built)
+// The network output y is the correction T_span * y to the temperature part of x_next.
+// ----------------------------------------------------------------------------------------------
+constexpr int kLearnStatic = 8;
+
+__global__ void k_learn_features(int N, const double* __restrict__ f, const double* __restrict__ x_next,
+                                 const double* __restrict__ x, const double* __restrict__ dXl,
+                                 const double* __restrict__ dFl, const float* __restrict__ stat, double inv_s,
+                                 double T_mid, double inv_span, float log_s, float progress, float* __restrict__ X) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        float* o = X + (size_t)c * 16;
+        const float* st = stat + (size_t)c * kLearnStatic;
+        const float t = (float)((x_next[c] - T_mid) * inv_span);
+        o[0] = (float)(f[c] * inv_s);
+        o[1] = (float)((x_next[c] - x[c]) * inv_s);
+        o[2] = dXl ? (float)((dXl[c] + dFl[c]) * inv_s) : 0.f;
+        o[3] = t;
+        o[4] = log_s;
+        o[5] = t - st[3];
+        o[6] = progress;
+        o[7] = 0.f;
+#pragma unroll
+        for (int k = 0; k < kLearnStatic; k++) o[8 + k] = st[k];
+    }
+}
+
+// z += s * y on the temperature part and, as a local equilibrium shift, on the reflecting-boundary
+// intensities of each corrected cell (they are carried as temperature equivalents).
+__global__ void k_apply_correction(int N, int S, int nb, const int* __restrict__ bcells, const float* __restrict__ y,
+                                   double s, double* __restrict__ z) {
+    const long long total = N + (long long)S * nb;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int c = t < N ? (int)t : bcells[(t - N) % nb];
+        z[t] += s * (double)y[c];
+    }
+}
+
+// partial sums of sum (a - b)^2 over [lo, hi)
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sqdiff(long long lo, long long hi, const double* __restrict__ a,
+                                                 const double* __restrict__ b, double* __restrict__ partial) {
+    __shared__ double lds[BLOCK / 64];
+    double s[1] = {0};
+    for (long long i = lo + blockIdx.x * (long long)BLOCK + threadIdx.x; i < hi; i += (long long)gridDim.x * BLOCK) {
+        const double d = a[i] - (b ? b[i] : 0.0);
+        s[0] += d * d;
+    }
+    block_sum<BLOCK, 1>(s, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
 __global__ void k_fill(int n, double v, double* __restrict__ x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
 }

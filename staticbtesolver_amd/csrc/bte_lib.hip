@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -21,6 +22,7 @@ namespace {
 
 const char* kKernelNames[] = {"assemble", "sweep_schedule", "diffuse", "solve", "temperature", "margin", "flux"};
 enum { KA_ASM = 0, KA_SCHED, KA_DIFF, KA_SOLVE, KA_TEMP, KA_MARGIN, KA_FLUX, KA_COUNT };
+std::atomic<unsigned long long> g_geo_tags{0};
 
 }  // namespace
 
@@ -81,6 +83,17 @@ struct bte_ctx {
     std::vector<int> aa_order;  // active slots, oldest first
     std::vector<double> aa_H;   // Gram matrix of the dF columns by slot [m*m]
     double aa_best = 0;
+    double aa_last_ff = 0;      // squared residual norm of the last evaluated Anderson state
+    int aa_steps = 0;
+    unsigned long long geo_tag = 0;  // changes with every bte_set_geometry
+    bool aa_plain = false;      // state bookkeeping without mixing (plain iteration + learned correction)
+    // learned correction
+    bte_gnn* gnn = nullptr;     // attached network (not owned)
+    DBuf learn_stat, learn_backup, learn_X, red_part, red_out;
+    bool learn_stat_ok = false, learn_pending = false;
+    double learn_Tmid = 0, learn_span = 1, learn_ref_ff = 0, learn_ratio = 0;
+    int learn_streak = 0;       // consecutive rejections
+    int run_rejected = 0;
     int tuned_variant[2][2][2] = {};  // sweep variant for the direct form
 };
 
@@ -299,7 +312,7 @@ int bte_ctx_create(int device, bte_ctx** out) {
     }
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_margin_pinned, sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_margin_pinned, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return BTE_ERR_HIP;
     }
@@ -342,6 +355,7 @@ int bte_set_geometry(bte_ctx* ctx, int dim, int n_cell, int n_face, const double
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_geo = true;
     ctx->prepared = false;
+    ctx->geo_tag = ++g_geo_tags;
     drop_graphs(ctx);
     return BTE_OK;
 }
@@ -839,7 +853,8 @@ static void small_solve(std::vector<double> H, std::vector<double> b, int n, std
 }
 
 static int aa_reset(bte_ctx* ctx) {
-    const int N = ctx->N, F = ctx->F, m = std::max(1, std::min(ctx->run_opts.accel_memory, 64));
+    const int N = ctx->N, F = ctx->F;
+    const int m = ctx->aa_plain ? 1 : std::max(1, std::min(ctx->run_opts.accel_memory, 64));
     ctx->aa_m = m;
     // cells whose previous intensities enter the next right-hand sides: reflecting boundary faces
     std::vector<int> bcells;
@@ -873,9 +888,27 @@ static int aa_reset(bte_ctx* ctx) {
     HIPCHK(hipMemsetAsync(ctx->aa_x.p, 0, sizeof(double) * Z, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));  // sc and bcells are host temporaries
     ctx->aa_first = true;
+    ctx->aa_steps = 0;
+    ctx->aa_last_ff = 0;
+    ctx->learn_pending = false;
+    ctx->learn_streak = 0;
     ctx->aa_order.clear();
     ctx->aa_H.assign((size_t)m * m, 0.0);
     ctx->aa_best = 0;
+    return BTE_OK;
+}
+
+// The next iteration reads the state aa_x: emission density of its T part, and its boundary
+// intensities written into the iterate the next iteration takes as ee_prev.
+static int load_state(bte_ctx* ctx) {
+    const int N = ctx->N, S = ctx->B * ctx->D, nb = ctx->aa_nb;
+    hipLaunchKernelGGL(k_set_density, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, bands_of(ctx),
+                       ctx->aa_x.as<double>(), ctx->dens.as<double>());
+    if (nb > 0)
+        hipLaunchKernelGGL(k_state_scatter, dim3(grid_for((long long)S * nb, 256)), dim3(256), 0, ctx->stream, N,
+                           ctx->D, S, nb, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->aa_x.as<double>(),
+                           ctx->ee_final);
+    HIPCHK(hipGetLastError());
     return BTE_OK;
 }
 
@@ -885,9 +918,20 @@ static int aa_step(bte_ctx* ctx) {
     const int N = ctx->N, m = ctx->aa_m, S = ctx->B * ctx->D, nb = ctx->aa_nb;
     const int Z = (int)ctx->aa_Z;
     const int gb = grid_for(Z, 256);
+    ctx->aa_steps++;
     // g = G(x): the new temperature field and reflecting-boundary intensities
     hipLaunchKernelGGL(k_state_gather, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb, ctx->aa_bcells.as<int>(),
                        ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final, ctx->aa_g.as<double>());
+    if (ctx->aa_plain) {
+        // plain iteration: x_{k+1} = G(x_k); the iteration already left T, density and the boundary
+        // intensities of x_{k+1} in place
+        hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, Z, 1, ctx->aa_g.as<double>(),
+                           ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(),
+                           ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>());
+        HIPCHK(hipMemcpyAsync(ctx->aa_x.p, ctx->aa_g.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipGetLastError());
+        return BTE_OK;
+    }
     int slot = 0;
     if (!ctx->aa_first) {
         if ((int)ctx->aa_order.size() < m) {
@@ -918,6 +962,7 @@ static int aa_step(bte_ctx* ctx) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
         const double ff = dots[2 * m];
+        ctx->aa_last_ff = ff;
         // divergence guard: restart the history when the residual grew far beyond the best seen
         if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
         std::vector<double> gamma(m, 0.0);
@@ -938,21 +983,218 @@ static int aa_step(bte_ctx* ctx) {
                            ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(), ctx->aa_x.as<double>());
         HIPCHK(hipStreamSynchronize(ctx->stream));  // gamma lives on the host stack
     }
-    // the next iteration reads the mixed state: emission density of its T part, boundary intensities
-    hipLaunchKernelGGL(k_set_density, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, bands_of(ctx),
-                       ctx->aa_x.as<double>(), ctx->dens.as<double>());
-    if (nb > 0)
-        hipLaunchKernelGGL(k_state_scatter, dim3(grid_for((long long)S * nb, 256)), dim3(256), 0, ctx->stream, N,
-                           ctx->D, S, nb, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->aa_x.as<double>(),
-                           ctx->ee_final);
+    return load_state(ctx);
+}
+
+// ---- learned correction ------------------------------------------------------------------------
+
+// sum over [lo, hi) of (a - b)^2 (b may be null), reduced in a fixed order; synchronises the stream
+static int sq_norm(bte_ctx* ctx, const double* a, const double* b, long long lo, long long hi, double& out) {
+    const int nblk = 64;
+    HIPCHK(ctx->red_part.ensure(sizeof(double) * nblk));
+    HIPCHK(ctx->red_out.ensure(sizeof(double)));
+    hipLaunchKernelGGL(k_sqdiff<256>, dim3(nblk), dim3(256), 0, ctx->stream, lo, hi, a, b, ctx->red_part.as<double>());
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, ctx->stream, 1, nblk, ctx->red_part.as<double>(),
+                       ctx->red_out.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned + 1, ctx->red_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    out = ctx->h_margin_pinned[1];
+    return BTE_OK;
+}
+
+// Static node features (host-built once per problem): per cell
+//   0 isothermal-wall flag   1 reflecting-wall flag   2 log10(mfp / h) / 2
+//   3 (T_interp - T_mid) / T_span, T_interp the inverse-hop-distance blend of the isothermal wall
+//     temperatures (linear between two opposite walls)
+//   4 d_iso / (d_iso + d_refl) in hops (0 without reflecting walls)
+//   5 log10(1 + mean hop distance to the isothermal walls) / 3
+//   6 log10(mfp / (h (1 + mean hop distance))) / 2 (a Knudsen number of the whole domain)
+//   7 log10(1 + d_iso h / mfp) / 2 (distance from the nearest isothermal wall in mean free paths)
+// with mfp = sum_b C v^2 tau / sum_b C v, h the cell size V^(1/dim) and d_iso in hops.
+static int learn_prepare(bte_ctx* ctx) {
+    if (ctx->learn_stat_ok) return BTE_OK;
+    const int N = ctx->N, F = ctx->F, nbc = (int)ctx->h_bci.size();
+    double tmin = 1e300, tmax = -1e300;
+    std::vector<int> iso;
+    for (int k = 0; k < nbc; k++)
+        if (ctx->h_bct[k] == 1) {
+            iso.push_back(k);
+            tmin = std::min(tmin, ctx->h_bcT[k]);
+            tmax = std::max(tmax, ctx->h_bcT[k]);
+        }
+    if (iso.empty()) SETERR(BTE_ERR_BC, "learned correction: the problem has no isothermal wall");
+    ctx->learn_Tmid = 0.5 * (tmin + tmax);
+    ctx->learn_span = tmax > tmin ? tmax - tmin : 1.0;
+    const int INF = 1 << 29;
+    // multi-source breadth-first search over interior faces from the cells of the selected walls
+    auto bfs = [&](auto&& seed_bc, std::vector<int>& d) {
+        d.assign(N, INF);
+        std::vector<int> q;
+        q.reserve(N);
+        for (int c = 0; c < N; c++)
+            for (int f = 0; f < F; f++) {
+                const int nb = ctx->h_nbr[(size_t)c * F + f];
+                if (nb < 0 && seed_bc(nb)) {
+                    d[c] = 0;
+                    q.push_back(c);
+                    break;
+                }
+            }
+        for (size_t h = 0; h < q.size(); h++) {
+            const int c = q[h];
+            for (int f = 0; f < F; f++) {
+                const int nb = ctx->h_nbr[(size_t)c * F + f];
+                if (nb >= 0 && d[nb] == INF) {
+                    d[nb] = d[c] + 1;
+                    q.push_back(nb);
+                }
+            }
+        }
+    };
+    std::vector<std::vector<int>> d_iso(iso.size());
+    for (size_t i = 0; i < iso.size(); i++) {
+        const int idx = ctx->h_bci[iso[i]];
+        bfs([idx](int nb) { return nb == idx; }, d_iso[i]);
+    }
+    std::vector<int> d_refl;
+    bool any_refl = false;
+    bfs(
+        [&](int nb) {
+            for (int k = 0; k < nbc; k++)
+                if (ctx->h_bci[k] == nb && ctx->h_bct[k] != 1) return any_refl = true;
+            return false;
+        },
+        d_refl);
+    double num = 0, den = 0;
+    for (int b = 0; b < ctx->B; b++) {
+        num += ctx->h_C[b] * ctx->h_v[b] * ctx->h_v[b] * ctx->h_tau[b];
+        den += ctx->h_C[b] * ctx->h_v[b];
+    }
+    const double mfp = den > 0 ? num / den : 0;
+    std::vector<float> st((size_t)N * kLearnStatic);
+    for (int c = 0; c < N; c++) {
+        double wsum = 0, tsum = 0, dsum = 0;
+        int nreach = 0, dmin = INF;
+        for (size_t i = 0; i < iso.size(); i++) {
+            const int dk = d_iso[i][c];
+            if (dk >= INF) continue;
+            const double w = 1.0 / (dk + 0.5);
+            wsum += w;
+            tsum += w * ctx->h_bcT[iso[i]];
+            dsum += dk;
+            nreach++;
+            dmin = std::min(dmin, dk);
+        }
+        const double t_int = wsum > 0 ? tsum / wsum : ctx->learn_Tmid;
+        const double dmean = nreach ? dsum / nreach : 0.0;
+        const double vol = std::max(ctx->h_vol[c], 1e-300);
+        const double h = ctx->dim == 1 ? vol : (ctx->dim == 2 ? std::sqrt(vol) : std::cbrt(vol));
+        const double kn = std::max(mfp / h, 1e-12);
+        float* o = st.data() + (size_t)c * kLearnStatic;
+        o[0] = dmin == 0 ? 1.f : 0.f;
+        o[1] = d_refl[c] == 0 ? 1.f : 0.f;
+        o[2] = (float)(std::log10(kn) / 2);
+        o[3] = (float)((t_int - ctx->learn_Tmid) / ctx->learn_span);
+        o[4] = (any_refl && d_refl[c] < INF && dmin < INF) ? (float)((double)dmin / (dmin + d_refl[c] + 1e-30)) : 0.f;
+        o[5] = (float)(std::log10(1.0 + dmean) / 3);
+        o[6] = (float)(std::log10(kn / (1.0 + dmean)) / 2);
+        o[7] = dmin < INF ? (float)(std::log10(1.0 + dmin / kn) / 2) : 4.f;
+    }
+    HIPCHK(upload(ctx->learn_stat, st, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // st is a host temporary
+    ctx->learn_stat_ok = true;
+    return BTE_OK;
+}
+
+// Features of the current state into X (device, [N][16]); returns the residual norms of the last
+// evaluated state: ffT over the temperature part, ffZ over the whole state.
+static int learn_features(bte_ctx* ctx, float* X, double& ffT, double& ffZ) {
+    int rc;
+    if ((rc = learn_prepare(ctx))) return rc;
+    const int N = ctx->N;
+    const long long Z = ctx->aa_Z;
+    if ((rc = sq_norm(ctx, ctx->aa_f.as<double>(), nullptr, 0, N, ffT))) return rc;
+    if ((rc = sq_norm(ctx, ctx->aa_f.as<double>(), nullptr, 0, Z, ffZ))) return rc;
+    const double s = std::sqrt(ffT / N);
+    const double inv_s = s > 0 ? 1.0 / s : 0.0;
+    const double* dXl = nullptr;
+    const double* dFl = nullptr;
+    if (!ctx->aa_plain && !ctx->aa_order.empty()) {
+        const int slot = ctx->aa_order.back();
+        dXl = ctx->aa_dX.as<double>() + (size_t)slot * Z;
+        dFl = ctx->aa_dF.as<double>() + (size_t)slot * Z;
+    }
+    const float log_s = s > 0 ? (float)(std::log10(s / ctx->learn_span) / 4) : -2.f;
+    const float progress = (float)std::min(ctx->run_iter / 100.0, 2.0);
+    hipLaunchKernelGGL(k_learn_features, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->aa_f.as<double>(),
+                       ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), dXl, dFl, ctx->learn_stat.as<float>(), inv_s,
+                       ctx->learn_Tmid, 1.0 / ctx->learn_span, log_s, progress, X);
     HIPCHK(hipGetLastError());
     return BTE_OK;
+}
+
+// Proposes a correction of the next state: x_next += T_span * GNN(features).  The next outer
+// iteration runs from the corrected state and learn_judge decides whether it stands.
+static int learn_propose(bte_ctx* ctx) {
+    GnnNet* net = ctx->gnn;
+    const int N = ctx->N;
+    double ffT = 0, ffZ = 0;
+    int rc;
+    if ((rc = learn_features(ctx, net->X.as<float>(), ffT, ffZ))) return rc;
+    if (!(ffT > 0) || !std::isfinite(ffZ)) return BTE_OK;  // nothing to correct
+    if (gnn_forward_device(net, ctx->stream)) SETERR(BTE_ERR_HIP, "learned correction: network launch failed");
+    const long long Z = ctx->aa_Z;
+    HIPCHK(ctx->learn_backup.ensure(sizeof(double) * Z));
+    HIPCHK(hipMemcpyAsync(ctx->learn_backup.p, ctx->aa_x.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_apply_correction, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->B * ctx->D,
+                       ctx->aa_nb, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span, ctx->aa_x.as<double>());
+    HIPCHK(hipGetLastError());
+    int rc2;
+    if ((rc2 = load_state(ctx))) return rc2;
+    ctx->learn_ref_ff = ffZ;
+    ctx->learn_pending = true;
+    ctx->run_proposed++;
+    return BTE_OK;
+}
+
+// After the outer iteration from a corrected state: accept when its residual |G(x~) - x~| does
+// not exceed the residual of the state evaluated before the correction.
+static int learn_judge(bte_ctx* ctx, bool& accepted) {
+    const int N = ctx->N, S = ctx->B * ctx->D, nb = ctx->aa_nb;
+    const long long Z = ctx->aa_Z;
+    hipLaunchKernelGGL(k_state_gather, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
+                       ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
+                       ctx->aa_g.as<double>());
+    HIPCHK(hipGetLastError());
+    double ff = 0;
+    int rc;
+    if ((rc = sq_norm(ctx, ctx->aa_g.as<double>(), ctx->aa_x.as<double>(), 0, Z, ff))) return rc;
+    accepted = std::isfinite(ff) && ff <= ctx->learn_ref_ff;
+    ctx->learn_ratio = ctx->learn_ref_ff > 0 ? std::sqrt(ff / ctx->learn_ref_ff) : 0.0;
+    return BTE_OK;
+}
+
+// Rejected correction: restore the uncorrected state and undo the outer iteration that ran from it
+// (its input intensities are untouched; the iteration only wrote the other buffer).
+static int learn_undo(bte_ctx* ctx) {
+    HIPCHK(hipMemcpyAsync(ctx->aa_x.p, ctx->learn_backup.p, sizeof(double) * ctx->aa_Z, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+    ctx->run_iter--;
+    ctx->ee_final = (ctx->run_iter & 1) ? ctx->eeB.as<double>() : ctx->eeA.as<double>();
+    return load_state(ctx);
 }
 
 static int check_opts(bte_ctx* ctx, const bte_opts& o) {
     if (o.precision != BTE_PREC_FP64 && o.precision != BTE_PREC_MIXED) SETERR(BTE_ERR_ARG, "unknown precision");
     if (o.precond != BTE_PC_SWEEP && o.precond != BTE_PC_JACOBI) SETERR(BTE_ERR_ARG, "unknown preconditioner");
     if (o.inner_maxit <= 0 || o.inner_tol <= 0) SETERR(BTE_ERR_ARG, "inner_maxit and inner_tol must be positive");
+    if (o.accel != BTE_ACCEL_NONE && o.accel != BTE_ACCEL_ANDERSON) SETERR(BTE_ERR_ARG, "unknown outer acceleration");
+    if (o.learned) {
+        if (o.learned_every <= 0) SETERR(BTE_ERR_ARG, "learned_every must be positive");
+        if (!ctx->gnn) SETERR(BTE_ERR_STATE, "learned correction requested but no network is attached");
+        if (ctx->gnn->device != ctx->device) SETERR(BTE_ERR_ARG, "the attached network lives on another device");
+    }
     return BTE_OK;
 }
 
@@ -1004,7 +1246,9 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
         ctx->graph_key = key;
     }
     ctx->run_opts = o;
-    if (o.accel == BTE_ACCEL_ANDERSON) {
+    ctx->learn_stat_ok = false;
+    ctx->aa_plain = o.accel != BTE_ACCEL_ANDERSON;
+    if (o.accel == BTE_ACCEL_ANDERSON || o.learned) {
         if ((rc = aa_reset(ctx))) return rc;
         // x_0 = T_ref everywhere: the zero emission density of the first iteration
         hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->T_ref,
@@ -1012,6 +1256,16 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
+    if (o.learned) {
+        if ((rc = learn_prepare(ctx))) return rc;
+        bte_gnn* g = ctx->gnn;
+        if (g->graph_tag != ctx->geo_tag || g->N != N) {
+            if ((rc = bte_gnn_set_graph(g, N, ctx->F, ctx->h_nbr.data())))
+                SETERR(rc, std::string("learned correction: ") + g->err);
+            g->graph_tag = ctx->geo_tag;
+        }
+    }
+    ctx->run_accepted = ctx->run_proposed = ctx->run_rejected = 0;
     ctx->run_shape_block = shape.block;
     ctx->run_shape_variant = shape.variant;
     ctx->run_graph = use_graph;
@@ -1058,10 +1312,29 @@ int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged) {
     const bte_opts& o = ctx->run_opts;
     auto t0 = std::chrono::steady_clock::now();
     int done = 0;
+    const bool state_path = o.accel == BTE_ACCEL_ANDERSON || o.learned;
     while (done < n_iter && !ctx->run_converged) {
         double m = 0;
         int rc = run_iteration(ctx, m);
         if (rc) return rc;
+        if (ctx->learn_pending) {
+            ctx->learn_pending = false;
+            bool ok = false;
+            if ((rc = learn_judge(ctx, ok))) return rc;
+            if (o.verbose)
+                std::printf("learned correction %s (residual ratio %.4g)\n", ok ? "accepted" : "rejected",
+                            ctx->learn_ratio);
+            if (ok) {
+                ctx->run_accepted++;
+                ctx->learn_streak = 0;
+            } else {
+                // run the iteration again from the uncorrected state
+                ctx->run_rejected++;
+                ctx->learn_streak++;
+                if ((rc = learn_undo(ctx))) return rc;
+                continue;
+            }
+        }
         done++;
         ctx->run_margins.push_back(m);
         if (o.verbose) {
@@ -1071,13 +1344,42 @@ int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged) {
         if (m <= o.stop_margin) {
             ctx->run_converged = true;
             if (o.verbose) std::printf("\n");
-        } else if (o.accel == BTE_ACCEL_ANDERSON) {
+        } else if (state_path) {
             if ((rc = aa_step(ctx))) return rc;
+            // corrections at state steps 1, 1 + every, 1 + 2 every, ... until two in a row fail
+            if (o.learned && ctx->learn_streak < 2 && (ctx->aa_steps - 1) % o.learned_every == 0)
+                if ((rc = learn_propose(ctx))) return rc;
         }
     }
     ctx->run_solve_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (n_done) *n_done = done;
     if (converged) *converged = ctx->run_converged ? 1 : 0;
+    return BTE_OK;
+}
+
+int bte_attach_gnn(bte_ctx* ctx, bte_gnn* g) {
+    if (!ctx) return BTE_ERR_ARG;
+    if (g && g->device != ctx->device) SETERR(BTE_ERR_ARG, "bte_attach_gnn: the network lives on another device");
+    ctx->gnn = g;
+    ctx->learn_pending = false;
+    return BTE_OK;
+}
+
+int bte_learn_features(bte_ctx* ctx, float* X, double* x_next, double* span) {
+    if (!ctx || !X) return BTE_ERR_ARG;
+    if (!ctx->run_active || ctx->aa_Z == 0 || ctx->aa_steps == 0 ||
+        !(ctx->run_opts.accel == BTE_ACCEL_ANDERSON || ctx->run_opts.learned))
+        SETERR(BTE_ERR_STATE, "bte_learn_features: needs an iteration begun with accel or learned and one bte_step");
+    HIPCHK(hipSetDevice(ctx->device));
+    const int N = ctx->N;
+    HIPCHK(ctx->learn_X.ensure(sizeof(float) * (size_t)N * kGnnIn));
+    double ffT = 0, ffZ = 0;
+    int rc;
+    if ((rc = learn_features(ctx, ctx->learn_X.as<float>(), ffT, ffZ))) return rc;
+    HIPCHK(hipMemcpyAsync(X, ctx->learn_X.p, sizeof(float) * (size_t)N * kGnnIn, hipMemcpyDeviceToHost, ctx->stream));
+    if (x_next) HIPCHK(hipMemcpyAsync(x_next, ctx->aa_x.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (span) *span = ctx->learn_span;
     return BTE_OK;
 }
 
@@ -1184,7 +1486,6 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
         bte_default_opts(&o);
     int rc = bte_begin(ctx, &o);
     if (rc) return rc;
-    ctx->run_accepted = ctx->run_proposed = 0;
     if ((rc = bte_step(ctx, o.max_iter, nullptr, nullptr))) return rc;
     return bte_finish(ctx, res);
 }

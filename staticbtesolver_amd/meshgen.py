@@ -64,33 +64,35 @@ def poisson_disk_square(r: float, seed: int, k: int = 30) -> np.ndarray:
 
     for p in bpts:
         insert(p)
-    P = None
     active = [len(pts)]
     insert((rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7)))
     margin = 0.5 * r
+    # candidates lie within 2r of their base point, so every point closer than r to one of them
+    # sits in the grid window of +-5 cells (3r / cell + 1) around the base
+    win = int(math.ceil(3.0 * r / cell)) + 1
+    parr = np.zeros((len(pts) + 4 * gn * gn, 2))
+    parr[: len(pts)] = np.asarray(pts)
     while active:
         a = active[rng.integers(len(active))]
-        base = np.asarray(pts[a])
+        base = parr[a]
         ang = rng.uniform(0, 2 * math.pi, k)
         rad = r * np.sqrt(rng.uniform(1.0, 4.0, k))
         cand = base + np.stack([np.cos(ang) * rad, np.sin(ang) * rad], axis=1)
-        ok_any = False
-        for c in cand:
-            if not (margin <= c[0] <= 1 - margin and margin <= c[1] <= 1 - margin):
-                continue
-            i, j = int(c[0] / cell), int(c[1] / cell)
-            i0, i1, j0, j1 = max(i - 2, 0), min(i + 3, gn), max(j - 2, 0), min(j + 3, gn)
-            nbrs = grid[i0:i1, j0:j1]
-            nbrs = nbrs[nbrs >= 0]
-            if nbrs.size:
-                P = np.asarray([pts[n] for n in nbrs])
-                if np.min(np.sum((P - c) ** 2, axis=1)) < r * r:
-                    continue
+        ok = (cand[:, 0] >= margin) & (cand[:, 0] <= 1 - margin) & (cand[:, 1] >= margin) & (cand[:, 1] <= 1 - margin)
+        i, j = int(base[0] / cell), int(base[1] / cell)
+        nbrs = grid[max(i - win, 0) : min(i + win + 1, gn), max(j - win, 0) : min(j + win + 1, gn)]
+        nbrs = nbrs[nbrs >= 0]
+        if nbrs.size:
+            P = parr[nbrs]
+            d2 = (cand[:, None, 0] - P[None, :, 0]) ** 2 + (cand[:, None, 1] - P[None, :, 1]) ** 2
+            ok &= d2.min(axis=1) >= r * r
+        hit = np.flatnonzero(ok)
+        if hit.size:
+            c = cand[hit[0]]
             active.append(len(pts))
+            parr[len(pts)] = c
             insert((float(c[0]), float(c[1])))
-            ok_any = True
-            break
-        if not ok_any:
+        else:
             active.remove(a)
     return np.asarray(pts)
 

@@ -1,0 +1,161 @@
+"""GPU: learned correction of the outer iteration -- network inputs, the residual judge, and what
+the trained network gains over Anderson mixing alone on held-out near-diffusive geometries."""
+import os
+
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import dijkstra
+
+from staticbtesolver_amd import capi, meshgen
+from staticbtesolver_amd.gnn import Gnn
+from staticbtesolver_amd.train_gnn import DEFAULT_WEIGHTS
+
+pytestmark = pytest.mark.gpu
+
+L_ND = 1e-5  # near-diffusive square side
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def nd_case(directory, n_cells, seed):
+    p = meshgen.make_square_case(str(directory), n_cells, seed, 10, seed, L=L_ND)
+    return capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=L_ND, Ly=L_ND)
+
+
+def aa(**kw):
+    return dict(accel=capi.ACCEL_ANDERSON, max_iter=20000, **kw)
+
+
+@pytest.fixture(scope="module")
+def case3k(tmp_path_factory):
+    c = nd_case(tmp_path_factory.mktemp("nd3k"), 3000, 11)
+    yield c
+    c.close()
+
+
+def hop_distance(nbr, codes):
+    """Graph distance in interior faces from the cells touching boundary entities `codes`."""
+    N, F = nbr.shape
+    r, f = np.nonzero(nbr >= 0)
+    G = csr_matrix((np.ones(len(r)), (r, nbr[r, f])), shape=(N, N))
+    seeds = np.nonzero(np.isin(nbr, codes).any(axis=1))[0]
+    return dijkstra(G, directed=False, indices=seeds, unweighted=True, min_only=True)
+
+
+def test_learn_features_match_their_definition(case3k):
+    """Dynamic inputs against the exported state; static inputs against graph distances computed
+    independently (scipy's Dijkstra) from the face table."""
+    say("learned: features")
+    c = case3k
+    c.begin(capi.default_opts(**aa()))
+    c.step(3)
+    X, x_next, span = c.learn_features()
+    c.finish()
+    N = c.N_cell
+    assert X.shape == (N, 16) and np.all(np.isfinite(X))
+    assert span == 2.0  # walls at 301 and 299 K
+    # 0: residual over its RMS -> unit RMS
+    assert abs(np.sqrt(np.mean(X[:, 0].astype(np.float64) ** 2)) - 1.0) < 1e-5
+    np.testing.assert_allclose(X[:, 3], (x_next - 300.0) / 2.0, atol=1e-6)
+    assert np.all(X[:, 4] == X[0, 4]) and np.all(X[:, 7] == 0)
+    nbr = c.geometry()["nbr"]
+    # boundary entity k is stored as neighbour -k-1; 0 and 3 isothermal (301 / 299 K), 1 and 2 diffuse
+    d_hot, d_cold, d_refl = hop_distance(nbr, [-1]), hop_distance(nbr, [-4]), hop_distance(nbr, [-2, -3])
+    d_iso = np.minimum(d_hot, d_cold)
+    np.testing.assert_array_equal(X[:, 8], (d_iso == 0).astype(np.float32))
+    np.testing.assert_array_equal(X[:, 9], (d_refl == 0).astype(np.float32))
+    w_h, w_c = 1.0 / (d_hot + 0.5), 1.0 / (d_cold + 0.5)
+    T_int = (301.0 * w_h + 299.0 * w_c) / (w_h + w_c)
+    np.testing.assert_allclose(X[:, 11], (T_int - 300.0) / 2.0, atol=1e-6)
+    np.testing.assert_allclose(X[:, 12], d_iso / (d_iso + d_refl + 1e-30), atol=1e-6)
+    np.testing.assert_allclose(X[:, 13], np.log10(1 + 0.5 * (d_hot + d_cold)) / 3, atol=1e-6)
+    np.testing.assert_allclose(X[:, 5], X[:, 3] - X[:, 11], atol=1e-6)
+    # 15: distance from the nearest isothermal wall in mean free paths (mfp / h from feature 10)
+    kn = 10.0 ** (2 * X[:, 10].astype(np.float64))
+    np.testing.assert_allclose(X[:, 15], np.log10(1 + d_iso / kn) / 2, atol=2e-5)
+
+
+def test_rejected_corrections_leave_the_iteration_unchanged(case3k):
+    """A network that proposes a 2000 K jump everywhere: every proposal raises the residual, is
+    rejected, and the iteration runs as without the network (bit for bit); proposals stop after two
+    rejections in a row."""
+    say("learned: rejection")
+    c = case3k
+    g = Gnn(layers=4, seed=1, device=0)
+    p = np.zeros(g.n_params, np.float32)
+    p[-1] = 1000.0  # output bias: y = 1000 -> correction 2000 K
+    g.set_params(p)
+    c.attach_gnn(g)
+    base = c.solve(**aa())
+    r = c.solve(**aa(learned=1, learned_every=4))
+    c.attach_gnn(None)
+    say("iters", base["iters"], r["iters"], "proposed", r["learned_proposed"], "accepted", r["learned_accepted"])
+    assert r["learned_proposed"] == 2 and r["learned_accepted"] == 0
+    assert r["iters"] == base["iters"] and r["converged"]
+    np.testing.assert_array_equal(r["T"], base["T"])
+    np.testing.assert_array_equal(r["margins"], base["margins"])
+
+
+def test_null_correction_changes_nothing(case3k):
+    """An all-zero network proposes zero corrections: whether accepted or not, the iterates are
+    those of Anderson mixing alone."""
+    say("learned: null network")
+    c = case3k
+    g = Gnn(layers=4, seed=2, device=0)
+    g.set_params(np.zeros(g.n_params, np.float32))
+    c.attach_gnn(g)
+    base = c.solve(**aa())
+    r = c.solve(**aa(learned=1, learned_every=3))
+    c.attach_gnn(None)
+    assert r["learned_proposed"] >= 1
+    assert r["iters"] == base["iters"]
+    np.testing.assert_array_equal(r["T"], base["T"])
+
+
+def test_learned_needs_an_attached_network(case3k):
+    c = case3k
+    c.attach_gnn(None)
+    with pytest.raises(capi.BTEError):
+        c.solve(**aa(learned=1))
+
+
+def test_learned_with_plain_iteration_reaches_the_same_field(case3k):
+    """learned with the reference's plain source iteration (no mixing): a random network's
+    proposals are judged; the result still meets the plain iteration's fixed point."""
+    say("learned: plain iteration")
+    c = case3k
+    ref = c.solve(**aa(stop_margin=1e-12))
+    g = Gnn(layers=4, seed=3, device=0)
+    c.attach_gnn(g)
+    r = c.solve(max_iter=60000, stop_margin=1e-9, learned=1, learned_every=50)
+    c.attach_gnn(None)
+    assert r["converged"] and r["learned_proposed"] >= 1
+    assert np.max(np.abs(r["T"] - ref["T"]) / ref["T"]) < 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(DEFAULT_WEIGHTS), reason="trained weights not present")
+@pytest.mark.parametrize("seed,n_cells", [(2001, 20000), (2002, 50000)])
+def test_trained_network_gains_over_anderson(tmp_path, seed, n_cells):
+    """Held-out geometries of the training family (seeds outside 1000-1255): the trained network's
+    accepted corrections cut the outer iterations of Anderson mixing, at the same accuracy against
+    a tightly converged field (<= 1e-4 relative)."""
+    say("learned: trained network", seed, n_cells)
+    c = nd_case(tmp_path, n_cells, seed)
+    g = Gnn(layers=4, seed=0, device=0)
+    g.set_params(np.load(DEFAULT_WEIGHTS))
+    c.attach_gnn(g)
+    ref = c.solve(**aa(stop_margin=1e-12))
+    base = c.solve(**aa())
+    r = c.solve(**aa(learned=1))
+    c.attach_gnn(None)
+    c.close()
+    err_b = np.max(np.abs(base["T"] - ref["T"]) / ref["T"])
+    err_l = np.max(np.abs(r["T"] - ref["T"]) / ref["T"])
+    say(f"anderson {base['iters']} it {base['solve_seconds']:.3f} s err {err_b:.2e} | learned {r['iters']} it "
+        f"{r['solve_seconds']:.3f} s err {err_l:.2e} accepted {r['learned_accepted']}/{r['learned_proposed']}")
+    assert r["learned_accepted"] >= 1
+    assert r["iters"] < base["iters"]
+    assert err_l <= 1e-4 and err_b <= 1e-4
