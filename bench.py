@@ -1,14 +1,20 @@
-"""Benchmark: wall-clock to a converged temperature field on one MI355X.
+"""Benchmark: wall-clock to a converged temperature field on one MI355X, learned + batched form
+against the plain FP64 form.
 
-A step is one converged source iteration (one pass of the hot path over one synthetic problem):
-StaticBTESolver::solve on a seeded 2D Poisson-disk/Delaunay mesh of a 1e-7 m square (~20k
-triangles), 10 seeded bands, DM=2, t=8, p=8 (32 directions), left wall 301 K, right wall 299 K,
-top/bottom diffuse, stop at margin <= 1e-5.  All 320 band x direction systems of each outer
-iteration run in one batched launch.  Inputs are generated and uploaded before the timed region.
+Default workload (near-diffusive 2D): seeded Poisson-disk/Delaunay triangle mesh of a 1e-5 m square
+(~50k cells), 10 seeded bands (v ~ U[1e3, 6e3] m/s, tau ~ logU[1e-12, 1e-10] s, C ~ U[1e5, 1e6]),
+DM=2, t=8, p=8 (32 directions), left wall 301 K, right wall 299 K, top/bottom diffuse at 300 K,
+stop at margin <= 1e-5 (StaticBTESolver.cpp:798).  A step is one solve to that stop through the
+C++ facade, all 320 band x direction systems of each outer iteration in one batched launch, each
+outer iteration one hipGraph.  Inputs are generated, read and uploaded before the timed region.
 
-Prints one JSON line (see the driver contract) with a roofline object for the dominant kernel (the
-batched inner solve, timed with hipEvents on the library's stream) and a cpu_baseline measured on a
-bounded sample with the CPU oracle.
+Forms: learned (Anderson mixing + residual-judged network corrections), anderson (mixing only),
+plain (the reference's source iteration).  The same run times the plain FP64 form once and measures
+the error of both against a tightly converged FP64 field (Anderson to margin 1e-12).
+
+Prints one JSON line with a roofline object for the dominant kernel (the batched inner solve, timed
+with hipEvents on the library's stream in a separate solve) and a cpu_baseline measured on a bounded
+sample with the CPU oracle's PETSc-default restatement, extrapolated.
 """
 from __future__ import annotations
 
@@ -30,11 +36,16 @@ HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
     # name: (kind, size, n_band, DM, ntheta, nphi, L, seed)
+    "2d-nd-50k": ("square", 50000, 10, 2, 8, 8, 1e-5, 4),
     "2d-20k": ("square", 20000, 10, 2, 8, 8, 1e-7, 2),
     "2d-ref12k": ("file", None, None, 2, 8, 8, 1e-8, None),
     "1d-100": ("1d", 100, 3, 2, 8, 1, 1e-8, 1),
     "3d-200k": ("cube", 32, 20, 3, 4, 4, 1e-7, 3),
 }
+
+
+def say(*a):
+    print("#", *a, file=sys.stderr, flush=True)
 
 
 def make_case(name, workdir):
@@ -61,9 +72,99 @@ def make_case(name, workdir):
     return kw, capi.Case(**kw)
 
 
+UNEVEN = dict(n=64, seed=5, cells=(2e3, 6e4), L=(1e-8, 1e-5), bands=(5, 10), dT=(1.0, 5.0))
+
+
+def _make_uneven_one(spec):
+    from staticbtesolver_amd import meshgen
+
+    d, cells, seed, nb, L, th, tc = spec
+    p = meshgen.make_square_case(d, cells, seed, nb, seed, L=L,
+                                 bc_rows=[(0, 1, th), (1, 2, 300.0), (2, 2, 300.0), (3, 1, tc)])
+    p.update(L=L)
+    return p
+
+
+def make_uneven(workdir):
+    """64 seeded squares: cells logU[2k, 60k], side logU[1e-8, 1e-5] m, 5-10 seeded bands, isothermal
+    walls at 300 + U[1, 5] K and 300 - U[1, 5] K, diffuse walls between.  Meshes are generated by a
+    process pool forked before this process touches the GPU."""
+    import multiprocessing as mp
+
+    u = UNEVEN
+    rng = np.random.default_rng(u["seed"])
+    specs = []
+    for k in range(u["n"]):
+        cells = int(round(10 ** rng.uniform(np.log10(u["cells"][0]), np.log10(u["cells"][1]))))
+        L = float(10 ** rng.uniform(np.log10(u["L"][0]), np.log10(u["L"][1])))
+        nb = int(rng.integers(u["bands"][0], u["bands"][1] + 1))
+        th, tc = 300.0 + rng.uniform(*u["dT"]), 300.0 - rng.uniform(*u["dT"])
+        specs.append((os.path.join(workdir, f"u{k}"), cells, 5000 + k, nb, L, th, tc))
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        return pool.map(_make_uneven_one, specs)
+
+
+def bench_uneven(args):
+    """Uneven batch: all problems in one bte_batch_solve versus one solve per problem."""
+    from staticbtesolver_amd import capi
+
+    work = tempfile.mkdtemp(prefix="bte_bench_")
+    t0 = time.time()
+    paths = make_uneven(work)
+    cases = [capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=p["L"], Ly=p["L"]) for p in paths]
+    n_cell = sum(c.N_cell for c in cases)
+    n_sys = sum(c.N_band * c.N_dir for c in cases)
+    say(f"uneven batch: {len(cases)} problems, {n_cell} cells, {n_sys} systems (inputs {time.time() - t0:.1f}s)")
+    form = dict(max_iter=100000, accel=capi.ACCEL_ANDERSON, accel_memory=args.accel_memory)
+    net = None
+    if args.form == "learned":
+        from staticbtesolver_amd.gnn import Gnn
+        from staticbtesolver_amd.train_gnn import DEFAULT_WEIGHTS
+        net = Gnn(layers=4, seed=0, device=0)
+        net.set_params(np.load(args.weights or DEFAULT_WEIGHTS))
+        for c in cases:
+            c.attach_gnn(net)
+        form.update(learned=1)
+    elif args.form == "plain":
+        form.update(accel=capi.ACCEL_NONE)
+    batch = capi.Batch(cases)
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        out = batch.solve(**form)
+        say(f"warmup {i}: {time.perf_counter() - t:.3f}s, outer iterations {min(r['iters'] for r in out)}.."
+            f"{max(r['iters'] for r in out)}")
+    times = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        out = batch.solve(**form)
+        times.append(time.perf_counter() - t)
+        say(f"step {i}: {times[-1]:.4f}s")
+    # the same problems one at a time (same form), and agreement with the batch
+    t_seq, worst = 0.0, 0.0
+    for c, r in zip(cases, out):
+        t = time.perf_counter()
+        s = c.solve(**form)
+        t_seq += time.perf_counter() - t
+        worst = max(worst, float(np.max(np.abs(s["T"] - r["T"]) / np.abs(s["T"]))))
+    t_med = statistics.median(times)
+    say(f"one at a time: {t_seq:.3f}s; batch {t_med:.3f}s")
+    line = {
+        "metric": METRIC, "value": round(t_med, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * t_med, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded meshes and bands)",
+        "config": {"workload": "uneven-64", "problems": len(cases), "n_cell_total": n_cell, "systems": n_sys,
+                   "form": args.form, "precision": "fp64"},
+        "comparison": {"one_at_a_time_s": round(t_seq, 4), "speedup_vs_one_at_a_time": round(t_seq / t_med, 2),
+                       "max_rel_diff_T_vs_one_at_a_time": worst,
+                       "outer_iterations": [r["iters"] for r in out]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def spmv_bytes(case, w=8):
-    """Survey-canonical bytes of one CSR product of every system: nnz*(w+4) + 4(n+1) + 2wn, with
-    nnz = n + incoming interior faces of the system's direction (band independent)."""
+    """Canonical bytes of one CSR product of every system: nnz*(w+4) + 4(n+1) + 2wn, with nnz = n +
+    incoming interior faces of the system's direction (band independent)."""
     geo = case.geometry()
     _, dirv, _ = case.quadrature()
     n = case.N_cell
@@ -74,10 +175,10 @@ def spmv_bytes(case, w=8):
     return float(per_dir.sum() * case.N_band)
 
 
-def cpu_baseline(kw, gpu_iters, budget_s=15.0):
+def cpu_baseline(kw, outer_iters, budget_s=15.0):
     """PETSc-default restatement (GMRES(30) + ILU(0), rtol 1e-9) timed on a bounded sample: the
     first outer iteration's systems, taken in order until budget_s of CPU time, then extrapolated
-    to all systems and to the GPU's outer-iteration count."""
+    to all systems and to the plain form's outer-iteration count."""
     from oracle import oracle as O
 
     o = O.OracleCase(**kw)
@@ -88,10 +189,61 @@ def cpu_baseline(kw, gpu_iters, budget_s=15.0):
         spent += o.time_systems(done, done + 1)
         done += 1
     per_sys = spent / done
-    est = per_sys * S * gpu_iters
+    est = per_sys * S * outer_iters
     return dict(value=est, unit="s (extrapolated)", cores=1, kind="port",
-                sample=f"first outer iteration, {done}/{S} systems in {spent:.1f}s on 1 core; "
-                       f"x{S} systems x{gpu_iters} outer iterations")
+                sample=f"first outer iteration, {done}/{S} systems in {spent:.1f}s on 1 core (GMRES(30)+ILU(0) "
+                       f"restatement); x{S} systems x{outer_iters} outer iterations of the plain form")
+
+
+def profiled_traffic(workload, form, precision):
+    """HBM bytes per dispatch of the dominant kernel from the committed rocprofv3 summary of this
+    workload and form (profiles/<kernel>_<workload>_<form>_<precision>.json, written by
+    staticbtesolver_amd/prof_summary.py from FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel
+    of known size), or None."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"*_{workload}_{form}_{precision}.json"))):
+        with open(path) as f:
+            res = json.load(f)
+        if res.get("traffic_bytes"):
+            return dict(bytes=res["traffic_bytes"], kernel=res.get("dominant"), avg_us=res.get("avg_us"),
+                        source=os.path.relpath(path, ROOT))
+    return None
+
+
+def errors(T, ref, T_ref=300.0):
+    dT = np.abs(T - ref)
+    return dict(max_rel=float(np.max(dT / np.abs(ref))), max_dT_rel=float(np.max(dT) / np.max(np.abs(ref - T_ref))))
+
+
+def track(case, form, T_ref_field, tol, coarse_after=None):
+    """Runs one solve of `form` outer iteration by outer iteration to its stop rule, timing only the
+    library's work (bte_step calls).  Returns the time and iterations until the temperature field
+    first meets max |T - T*| / |T*| <= tol (the check runs outside the timed calls), and until the stop
+    rule, with the error there.  With coarse_after, the check runs every 10 iterations after that
+    many (the reported time to target is then at most 9 iterations late)."""
+    from staticbtesolver_amd import capi
+
+    case.begin(capi.default_opts(**form))
+    t_solve, k, conv = 0.0, 0, False
+    tta, tta_k = None, None
+    while not conv:
+        n = 1 if (tta is not None or coarse_after is None or k < coarse_after) else 10
+        if tta is not None:
+            n = 1000
+        t = time.perf_counter()
+        done, conv = case.step(n)
+        t_solve += time.perf_counter() - t
+        k += done
+        if done == 0:
+            break
+        if tta is None:
+            T = case.temperature()
+            if np.max(np.abs(T - T_ref_field) / np.abs(T_ref_field)) <= tol:
+                tta, tta_k = t_solve, k
+    r = case.finish()
+    return dict(tta_s=tta, tta_iters=tta_k, stop_s=t_solve, iters=r["iters"], err_stop=errors(r["T"], T_ref_field),
+                accepted=r["learned_accepted"], proposed=r["learned_proposed"], converged=r["converged"])
 
 
 def main():
@@ -99,78 +251,118 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="2d-20k", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="2d-nd-50k", choices=sorted(CONFIGS) + ["uneven-64"])
+    ap.add_argument("--form", default="learned", choices=["learned", "anderson", "plain"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--tol", type=float, default=1e-4, help="target max relative error of T")
+    ap.add_argument("--accel-memory", type=int, default=50)
+    ap.add_argument("--weights", default=None, help="network parameters (.npy); default: the packaged weights")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-run", action="store_true", help="one untimed solve without graphs (for rocprofv3)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison solve")
+    ap.add_argument("--profile-run", action="store_true", help="one untimed solve of the form without graphs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only"}))
+        print(json.dumps({"error": "this build runs on one GPU (the box has one); multi-GPU sharding is "
+                                   "design-only -- run with --gpus 1"}))
         sys.exit(2)
+    if args.config == "uneven-64":
+        return bench_uneven(args)
     from staticbtesolver_amd import capi
 
     work = tempfile.mkdtemp(prefix="bte_bench_")
     t0 = time.time()
     kw, case = make_case(args.config, work)
-    print(f"# case {args.config}: N_cell={case.N_cell} N_dir={case.N_dir} N_band={case.N_band} "
-          f"(inputs {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    say(f"case {args.config}: N_cell={case.N_cell} N_dir={case.N_dir} N_band={case.N_band} "
+        f"(inputs {time.time() - t0:.1f}s)")
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
+    form = dict(precision=prec, max_iter=100000)
+    net = None
+    if args.form in ("learned", "anderson"):
+        form.update(accel=capi.ACCEL_ANDERSON, accel_memory=args.accel_memory)
+    if args.form == "learned":
+        from staticbtesolver_amd.gnn import Gnn
+        from staticbtesolver_amd.train_gnn import DEFAULT_WEIGHTS
+        wpath = args.weights or DEFAULT_WEIGHTS
+        net = Gnn(layers=4, seed=0, device=0)
+        net.set_params(np.load(wpath))
+        case.attach_gnn(net)
+        form.update(learned=1)
     if args.profile_run:
-        r = case.solve(precision=prec, use_graph=0, max_iter=10000)
-        print(f"# profile run: {r['iters']} outer iterations, {r['solve_seconds']:.3f}s", file=sys.stderr)
+        r = case.solve(use_graph=0, **form)
+        say(f"profile run: {r['iters']} outer iterations, {r['solve_seconds']:.3f}s")
         return
+    # the converged field every accuracy figure refers to: FP64, Anderson to margin 1e-12
+    ref = case.solve(accel=capi.ACCEL_ANDERSON, stop_margin=1e-12, max_iter=100000)
     for i in range(args.warmup):
-        r = case.solve(precision=prec)
-        print(f"# warmup {i}: {r['iters']} it {r['solve_seconds']:.3f}s setup {r['setup_seconds']:.3f}s",
-              file=sys.stderr, flush=True)
-    times, iters = [], []
+        r = track(case, form, ref["T"], args.tol)
+        say(f"warmup {i}: to {args.tol:g}: {r['tta_s']:.4f}s ({r['tta_iters']} it); to the stop rule: "
+            f"{r['stop_s']:.4f}s ({r['iters']} it)")
+    runs = []
     for i in range(args.steps):
-        r = case.solve(precision=prec)
-        times.append(r["solve_seconds"])
-        iters.append(r["iters"])
-        print(f"# step {i}: {r['iters']} it {r['solve_seconds']:.4f}s", file=sys.stderr, flush=True)
-    T_run = r["T"].copy()
+        r = track(case, form, ref["T"], args.tol)
+        runs.append(r)
+        say(f"step {i}: to {args.tol:g}: {r['tta_s']:.4f}s ({r['tta_iters']} it); to the stop rule: {r['stop_s']:.4f}s "
+            f"({r['iters']} it, err {r['err_stop']['max_rel']:.1e}); accepted {r['accepted']}/{r['proposed']}")
+    tta = statistics.median(r["tta_s"] for r in runs)
+    t_stop = statistics.median(r["stop_s"] for r in runs)
+    last = runs[-1]
     # roofline of the dominant kernel: a separate solve with per-kernel hipEvents on the library stream
-    rt = case.solve(precision=prec, timing=1)
+    rt = case.solve(timing=1, **form)
     st = case.stats()
     solve_ms = st["ms"]["solve"] / max(1, st["calls"]["solve"])
-    # one product + one sweep per system per launch when the sweep is exact (inner_iters == 1 each)
-    per_launch_sys_iters = rt["inner_iters"] / max(1, rt["iters"]) / (case.N_band * case.N_dir)
-    b_alg = 2.0 * spmv_bytes(case, 8 if prec == capi.PREC_FP64 else 4) * max(1.0, per_launch_sys_iters)
+    # per launch: one exact sweep per system (FP64 direct form) or the refinement loop's products
+    per_sys = rt["inner_iters"] / max(1, st["calls"]["solve"]) / (case.N_band * case.N_dir)
+    w = 8 if prec == capi.PREC_FP64 else 4
+    b_alg = 2.0 * spmv_bytes(case, w) * max(1.0, per_sys)
     achieved = b_alg / (solve_ms * 1e-3) / 1e9
-    # accuracy of this run against the plain FP64 form
-    if prec != capi.PREC_FP64:
-        ref = case.solve(precision=capi.PREC_FP64)
-        rel = float(np.max(np.abs(T_run - ref["T"]) / np.abs(ref["T"])))
-    else:
-        rel = 0.0
+    cmp = dict(form=args.form, precision=args.precision, target_rel_err=args.tol,
+               time_to_target_s=round(tta, 5), iterations_to_target=last["tta_iters"],
+               time_to_stop_rule_s=round(t_stop, 5), iterations_to_stop_rule=last["iters"],
+               err_at_stop_rule=last["err_stop"], learned_accepted=last["accepted"],
+               learned_proposed=last["proposed"], reference_field="FP64, Anderson mixing to margin 1e-12")
+    plain_iters = None
+    if not args.no_plain:
+        if net is not None:
+            case.attach_gnn(None)
+        pl = track(case, dict(max_iter=100000), ref["T"], args.tol, coarse_after=100)
+        plain_iters = pl["iters"]
+        cmp.update(plain_fp64_time_to_target_s=round(pl["tta_s"], 4), plain_fp64_iterations_to_target=pl["tta_iters"],
+                   plain_fp64_time_to_stop_rule_s=round(pl["stop_s"], 4), plain_fp64_iterations_to_stop_rule=pl["iters"],
+                   plain_fp64_err_at_stop_rule=pl["err_stop"],
+                   speedup_to_target_vs_plain_fp64=round(pl["tta_s"] / tta, 1) if pl["tta_s"] else None,
+                   speedup_to_stop_rule_vs_plain_fp64=round(pl["stop_s"] / t_stop, 2))
+        say(f"plain fp64: to {args.tol:g}: {pl['tta_s']:.3f}s ({pl['tta_iters']} it); to the stop rule "
+            f"{pl['stop_s']:.3f}s ({pl['iters']} it, err {pl['err_stop']['max_rel']:.1e})")
+    if net is not None:
+        case.attach_gnn(None)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(kw, int(statistics.median(iters)))
-    t_med = statistics.median(times)
+        cpu = cpu_baseline(kw, plain_iters or last["iters"])
+    prof = profiled_traffic(args.config, args.form, args.precision)
     line = {
         "metric": METRIC,
-        "value": round(t_med, 6),
+        "value": round(tta, 6),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_med, 4),
+        "ms_per_step": round(1e3 * tta, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == capi.PREC_FP64 else "f32+f64",
         "data": "synthetic (seeded mesh and bands)",
         "config": {"workload": args.config, "n_cell": case.N_cell, "n_dir": case.N_dir, "n_band": case.N_band,
-                   "systems_per_launch": case.N_band * case.N_dir, "form": args.precision + "+sweep",
-                   "outer_iterations": int(statistics.median(iters)),
-                   "ms_per_outer_iteration": round(1e3 * t_med / statistics.median(iters), 4),
+                   "systems_per_launch": case.N_band * case.N_dir, "form": args.form, "precision": args.precision,
+                   "accel_memory": form.get("accel_memory"),
+                   "ms_per_outer_iteration": round(1e3 * t_stop / max(1, last["iters"]), 4),
                    "sweep_levels_max": st["max_levels"], "lagged_edges": st["lagged_edges"],
-                   "block": st["block"], "sweep_variant": st["variant"],
-                   "max_rel_err_T_vs_fp64": rel},
+                   "block": st["block"], "sweep_variant": st["variant"]},
+        "comparison": cmp,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_solve",
-                     "kernel_ms": round(solve_ms, 5), "alg_bytes_per_launch": b_alg},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": prof["bytes"] if prof else None,
+                     "kernel": "inner solve", "kernel_ms": round(solve_ms, 5), "alg_bytes_per_launch": b_alg,
+                     "traffic_source": prof},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -60,7 +60,8 @@ typedef struct bte_opts {
                              state steps 1, 1 + learned_every, ... and kept only if the residual of
                              the iteration from the corrected state does not grow; stops proposing
                              after two rejections in a row.  Works with either accel; default 0 */
-    int learned_every;    /* default 8 */
+    int learned_every;    /* default 8; 0 = no proposals (keeps the state bookkeeping that
+                             bte_learn_features reads, e.g. to record the plain iteration) */
     int direct_sweep;     /* BTE_PC_SWEEP + FP64: when no upwind edge had to be lagged (acyclic
                              upwind graph), solve each system by one downwind sweep with the RHS
                              formed on the fly instead of BiCGSTAB (same solution); default 1 */
@@ -190,6 +191,21 @@ int bte_attach_gnn(bte_ctx* ctx, bte_gnn* g);
  * span * y; may be NULL).  Valid after a bte_step of an iteration begun with accel or learned that
  * ran at least one iteration; the target for y is (T_converged - x_next) / span. */
 int bte_learn_features(bte_ctx* ctx, float* X, double* x_next, double* span);
+
+/* ---- uneven batch of problems (no counterpart in the reference: it runs one problem per process) ----
+ * n contexts on one device, each fully loaded (geometry, quadrature, bands, BCs), solved together:
+ * every outer iteration sweeps the systems of all still-iterating problems in ONE launch (work-groups
+ * take the systems with the longest level chains first), then each problem runs its own outer-step
+ * bookkeeping (stop test, Anderson mixing, learned corrections) and leaves the batch when it
+ * converges.  Every problem must use the FP64 direct-sweep form.  The contexts must outlive the
+ * batch and must not be used concurrently with it; results equal those of solving each alone. */
+typedef struct bte_batch bte_batch;
+int bte_batch_create(int n, bte_ctx* const* ctxs, bte_batch** out);
+void bte_batch_destroy(bte_batch* b);
+const char* bte_batch_last_error(const bte_batch* b);
+/* results: [n] with caller-set output arrays as for bte_solve (may be NULL); each solve_seconds is the
+ * wall time of the whole batch. */
+int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results);
 
 /* ---- piecewise entry points (used by the parity tests; same kernels as bte_solve) ---- */
 

@@ -71,7 +71,8 @@ HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_las
                "bte_get_temperature", "bte_set_temperature", "bte_finish", "bte_gnn_create", "bte_gnn_destroy",
                "bte_gnn_last_error", "bte_gnn_num_params", "bte_gnn_get_params", "bte_gnn_set_params",
                "bte_gnn_get_grads", "bte_gnn_set_graph", "bte_gnn_forward", "bte_gnn_backward", "bte_gnn_adam",
-               "bte_attach_gnn", "bte_learn_features"]
+               "bte_attach_gnn", "bte_learn_features", "bte_batch_create", "bte_batch_destroy",
+               "bte_batch_last_error", "bte_batch_solve"]
 CASE_SYMBOLS = ["bte_case_open", "bte_case_close", "bte_case_sizes", "bte_case_params", "bte_case_solve",
                 "bte_case_geometry", "bte_case_quadrature", "bte_case_ctx", "bte_case_error"]
 
@@ -111,6 +112,11 @@ def lib():
     L.bte_finish.argtypes = [vp, C.POINTER(Result)]
     L.bte_attach_gnn.argtypes = [vp, vp]
     L.bte_learn_features.argtypes = [vp, C.POINTER(C.c_float), dp, dp]
+    L.bte_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
+    L.bte_batch_destroy.argtypes = [vp]
+    L.bte_batch_last_error.restype = C.c_char_p
+    L.bte_batch_last_error.argtypes = [vp]
+    L.bte_batch_solve.argtypes = [vp, C.POINTER(Opts), C.POINTER(Result)]
     L.bte_case_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_int,
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
     L.bte_case_close.argtypes = [vp]
@@ -316,3 +322,51 @@ class Case:
         return dict(ms={n: s.ms[i] for i, n in enumerate(names)}, calls={n: s.calls[i] for i, n in enumerate(names)},
                     spmv_rows=s.spmv_rows, ell_width=s.ell_width, max_levels=s.max_levels,
                     lagged_edges=s.lagged_edges, block=s.block, variant=s.variant)
+
+
+class Batch:
+    """Several loaded cases solved together (bte_batch_*): each outer iteration sweeps the systems of
+    all still-iterating cases in one launch.  The cases must stay open while the batch exists."""
+
+    def __init__(self, cases):
+        L = lib()
+        self.cases = list(cases)
+        arr = (C.c_void_p * len(self.cases))(*[c.ctx() for c in self.cases])
+        h = C.c_void_p()
+        rc = L.bte_batch_create(len(self.cases), arr, C.byref(h))
+        if rc != BTE_OK:
+            raise BTEError(rc, "bte_batch_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bte_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve(self, opts: Opts | None = None, **kw):
+        if opts is None:
+            opts = default_opts(**kw)
+        cap = max(1, opts.max_iter)
+        res = (Result * len(self.cases))()
+        bufs = []
+        for c, r in zip(self.cases, res):
+            N, B, nbc = c.N_cell, c.N_band, c.n_bc
+            T, Tb, fl, bfl, m = np.zeros(N), np.zeros(N * B), np.zeros(max(nbc, 1)), np.zeros(max(nbc, 1) * B), \
+                np.zeros(cap)
+            r.T, r.T_band, r.bc_flux, r.bc_band_flux, r.margins = _dp(T), _dp(Tb), _dp(fl), _dp(bfl), _dp(m)
+            r.margins_capacity = cap
+            bufs.append((T, Tb, fl, bfl, m))
+        rc = lib().bte_batch_solve(self.h, C.byref(opts), res)
+        if rc not in (BTE_OK, BTE_ERR_NOCONV):
+            raise BTEError(rc, lib().bte_batch_last_error(self.h).decode())
+        out = []
+        for c, r, (T, Tb, fl, bfl, m) in zip(self.cases, res, bufs):
+            nbc, B, it = c.n_bc, c.N_band, r.iterations
+            out.append(dict(T=T, T_band=Tb.reshape(c.N_cell, B), flux=fl[:nbc], band_flux=bfl[:nbc * B].reshape(nbc, B),
+                            margins=m[:min(it, cap)].copy(), iters=it, converged=bool(r.converged),
+                            inner_iters=int(r.inner_iters), solve_seconds=r.solve_seconds,
+                            learned_accepted=r.learned_accepted, learned_proposed=r.learned_proposed))
+        return out

@@ -293,6 +293,7 @@ struct Ops {
     const float* swp_dg32;
     const double* swp_vw;          // [D*N] w_d V_c in level order
     const unsigned char* swp_bnd;  // [D*N] 1 if the cell has an incoming boundary face
+    const int* swp_pos;   // [(d*W+k)*N + i] level-order position of the upwind neighbour (-1: none)
 };
 
 struct Work {
@@ -605,12 +606,12 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_direct(Geo g, Quad q, Bnd bd, B
 // XLDS the system's intensity vector lives in LDS during the sweep (gathers and stores never
 // leave the CU) and is written to HBM once, coalesced, at the end; this needs N*8 bytes of LDS.
 template <int BLOCK, bool XLDS>
-__global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, int S,
-                                                     const double* __restrict__ ee_prev,
-                                                     const double* __restrict__ dens) {
-    extern __shared__ double xs[];
+__device__ __forceinline__ void sweep_fast_system(const Geo& g, const Quad& q, const Bnd& bd, const Bands& bands,
+                                                  const Ops& o, const Work& w, int s, double* xs,
+                                                  const double* __restrict__ ee_prev,
+                                                  const double* __restrict__ dens) {
     const int N = o.N, W = o.W;
-    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+    {
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
         double* xg = w.x + (size_t)s * N;
@@ -652,6 +653,177 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
         }
         if (threadIdx.x == 0) w.iters[s] += 1;
     }
+}
+
+template <int BLOCK, bool XLDS>
+__global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, int S,
+                                                     const double* __restrict__ ee_prev,
+                                                     const double* __restrict__ dens) {
+    extern __shared__ double xs[];
+    for (int s = blockIdx.x; s < S; s += gridDim.x)
+        sweep_fast_system<BLOCK, XLDS>(g, q, bd, bands, o, w, s, xs, ee_prev, dens);
+}
+
+// Direct sweep with the recent intensities in an LDS ring and software-pipelined operand loads.
+// Upwind neighbours are addressed by their level-order position p (swp_pos); the value of position p
+// lives in ring slot p mod R.  The host enables this form only when every upwind neighbour of a
+// cell in level l satisfies p >= end(l) - R, so each read finds its slot not yet overwritten and no
+// intensity is ever read back from HBM; the output is written to HBM once per cell.  Operands of
+// level l+2 and the emission density of level l+1 are loaded while level l computes, so a level
+// costs an LDS gather, the arithmetic and one LDS barrier.  Levels wider than BLOCK finish their
+// extra cells unpipelined.  Same arithmetic, in the same order, as k_sweep_fast.
+// Right-hand sides of every system in sweep (level) order, rhs_lo[s*N + i] for cell perm_d[i]: the
+// emission term for interior cells, rhs_at for cells with an incoming boundary face -- the values
+// the level sweeps form on the fly, computed up front, fully parallel, so that the pipelined sweep
+// streams them with its other operands instead of gathering by cell inside the level chain.
+__global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const double* __restrict__ ee_prev,
+                             const double* __restrict__ dens, double* __restrict__ rhs_lo) {
+    const int N = o.N;
+    const long long total = (long long)S * N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(t / N), i = (int)(t % N);
+        const int b = s / o.D, d = s % o.D;
+        const size_t di = (size_t)d * N + i;
+        const int c = o.perm[di];
+        rhs_lo[t] = o.swp_bnd[di] ? rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens)
+                                  : dens[(size_t)b * N + c] * o.swp_vw[di];
+    }
+}
+
+struct SweepItem {
+    int i, c;     // level-order position (-1: no cell for this thread) and cell
+    int p[3];     // level-order positions of the upwind neighbours (-1: none)
+    double a[3];  // their coefficients
+    double dgc, vw, rhs;
+};
+
+__device__ __forceinline__ void sweep_item_load(int i, int N, int W, const int* __restrict__ perm,
+                                                const int* __restrict__ spos, const double* __restrict__ sval,
+                                                const double* __restrict__ sdg, const double* __restrict__ svw,
+                                                const double* __restrict__ rhs, SweepItem& it) {
+    it.i = i;
+    if (i < 0) return;
+    it.c = perm[i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        it.p[k] = (k < W) ? spos[(size_t)k * N + i] : -1;
+        it.a[k] = (k < W) ? sval[(size_t)k * N + i] : 0.0;
+    }
+    it.dgc = sdg[i];
+    it.vw = svw[i];
+    it.rhs = rhs[i];
+}
+
+// LDS-only work-group barrier: the ring is the only memory threads exchange inside the sweep, so
+// outstanding HBM loads (the prefetches) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// LDS of the pipelined sweep: the ring, then the level pointers of the system's direction.
+__host__ __device__ inline size_t ring_lds_bytes(int ring, int max_levels) {
+    return (size_t)ring * sizeof(double) + (size_t)(max_levels + 1) * sizeof(int);
+}
+
+// One system of the pipelined sweep; needs rhs_lo (k_rhs_levels) in w.b.  ring holds the LDS ring
+// followed by room for the direction's level pointers.  Per thread every operand of a level --
+// including its right-hand side -- is a coalesced load in level order, issued two levels ahead; the
+// level pointers are read from LDS, so the only waits inside the level chain are for those loads
+// and the LDS ring.
+template <int BLOCK>
+__device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
+    const int N = o.N, W = o.W;
+    const int tid = threadIdx.x;
+    int* lps = (int*)(ring + ring_mask + 1);
+    {
+        const int b = s / o.D, d = s % o.D;
+        const double vt = o.vt[b];
+        double* __restrict__ xg = w.x + (size_t)s * N;
+        const int* __restrict__ perm = o.perm + (size_t)d * N;
+        const int* __restrict__ spos = o.swp_pos + (size_t)d * W * N;
+        const double* __restrict__ sval = o.swp_val + (size_t)d * W * N;
+        const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
+        const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
+        const double* __restrict__ rhs = w.b + (size_t)s * N;
+        const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
+        const int L = o.lvl_cnt[d];
+        for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
+        __syncthreads();
+        auto first = [&](int l) {  // this thread's first-pass position in level l, or -1
+            if (l >= L) return -1;
+            const int i = lps[l] + tid;
+            return i < lps[l + 1] ? i : -1;
+        };
+        auto solve_cell = [&](const SweepItem& it) {
+            double acc = 0;
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (it.p[k] >= 0) acc += it.a[k] * ring[it.p[k] & ring_mask];
+            const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
+            ring[it.i & ring_mask] = xv;
+            xg[it.c] = xv;
+        };
+        SweepItem A, B2;
+        sweep_item_load(first(0), N, W, perm, spos, sval, sdg, svw, rhs, A);
+        sweep_item_load(first(1), N, W, perm, spos, sval, sdg, svw, rhs, B2);
+        for (int l = 0; l < L; l++) {
+            const int beg = lps[l], end = lps[l + 1];
+            SweepItem C;  // prefetch: operands of level l+2
+            sweep_item_load(first(l + 2), N, W, perm, spos, sval, sdg, svw, rhs, C);
+            if (A.i >= 0) solve_cell(A);
+            for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
+                SweepItem E;
+                sweep_item_load(i, N, W, perm, spos, sval, sdg, svw, rhs, E);
+                solve_cell(E);
+            }
+            lds_barrier();
+            A = B2;
+            B2 = C;
+        }
+        __syncthreads();  // the ring and level pointers are reused by the next system of this work-group
+        if (tid == 0) w.iters[s] += 1;
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int ring_mask) {
+    extern __shared__ double ring[];
+    for (int s = blockIdx.x; s < S; s += gridDim.x)
+        sweep_ring_system<BLOCK>(o, w, s, ring, ring_mask);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Uneven batch of problems: one launch sweeps the systems of several meshes.  desc[k] holds problem
+// k's data; items[t] = (problem, system) in the order the work-groups take them (longest level
+// chains first).
+// ----------------------------------------------------------------------------------------------
+struct SweepDesc {
+    Geo g;
+    Quad q;
+    Bnd bd;
+    Bands bands;
+    Ops o;
+    Work w;
+    const double* ee_prev;
+    const double* dens;
+};
+
+template <int BLOCK, bool RING>
+__global__ __launch_bounds__(BLOCK) void k_sweep_batch(const SweepDesc* __restrict__ desc, const int2* __restrict__ items,
+                                                      int n_items, int ring_mask) {
+    extern __shared__ double lds_x[];
+    for (int t = blockIdx.x; t < n_items; t += gridDim.x) {
+        const int2 it = items[t];
+        const SweepDesc& D = desc[it.x];
+        if constexpr (RING)
+            sweep_ring_system<BLOCK>(D.o, D.w, it.y, lds_x, ring_mask);
+        else
+            sweep_fast_system<BLOCK, false>(D.g, D.q, D.bd, D.bands, D.o, D.w, it.y, lds_x, D.ee_prev, D.dens);
+    }
+}
+
+// out[k] = *ptr[k]: the margins of the problems of a batch, for one copy to the host.
+__global__ void k_gather_scalars(int n, const double* const* __restrict__ ptr, double* __restrict__ out) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = *ptr[k];
 }
 
 // FP64 path: RHS formed in-kernel, then BiCGSTAB in FP64 writes ee_curr directly.
@@ -910,16 +1082,17 @@ __global__ void k_state_scatter(int N, int D, int S, int nb, const int* __restri
 // residual, x_next = the mixed next state, s = RMS of f over cells):
 //   0 f / s      1 (x_next - x) / s      2 (g - g_prev) / s (last history column, 0 without one)
 //   3 (x_next - T_mid) / T_span      4 log10(s / T_span) / 4 (uniform)
-//   5 (x_next - T_wall_interp) / T_span      6 progress (outer iterations / 100, capped at 2)   7 0
-//   8..15 static per-cell features (wall flags, graph distances, Knudsen numbers; host-built)
+//   5 (x_next - T_wall_interp) / T_span
+//   6..15 static per-cell features (wall flags, graph distances, Knudsen numbers; host-built)
 // The network output y is the correction T_span * y to the temperature part of x_next.
 // ----------------------------------------------------------------------------------------------
-constexpr int kLearnStatic = 8;
+constexpr int kLearnDynamic = 6;
+constexpr int kLearnStatic = 16 - kLearnDynamic;
 
 __global__ void k_learn_features(int N, const double* __restrict__ f, const double* __restrict__ x_next,
                                  const double* __restrict__ x, const double* __restrict__ dXl,
                                  const double* __restrict__ dFl, const float* __restrict__ stat, double inv_s,
-                                 double T_mid, double inv_span, float log_s, float progress, float* __restrict__ X) {
+                                 double T_mid, double inv_span, float log_s, float* __restrict__ X) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
         float* o = X + (size_t)c * 16;
         const float* st = stat + (size_t)c * kLearnStatic;
@@ -930,10 +1103,8 @@ __global__ void k_learn_features(int N, const double* __restrict__ f, const doub
         o[3] = t;
         o[4] = log_s;
         o[5] = t - st[3];
-        o[6] = progress;
-        o[7] = 0.f;
 #pragma unroll
-        for (int k = 0; k < kLearnStatic; k++) o[8 + k] = st[k];
+        for (int k = 0; k < kLearnStatic; k++) o[kLearnDynamic + k] = st[k];
     }
 }
 

@@ -46,8 +46,10 @@ struct bte_ctx {
     // assembly + sweep schedule
     int W = 0;
     DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
-    DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd;
+    DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd, swp_pos;
     int max_levels = 0, lagged = 0;
+    int ring_back = 0;  // max over cells of end(level) - position of an upwind neighbour
+    std::vector<int> h_lvl_cnt;  // sweep levels per direction
     int max_lds = 65536;  // dynamic LDS bytes a work-group may use on this device
     bool have_sweep = false, have_fp32 = false;
     // state + work
@@ -86,7 +88,8 @@ struct bte_ctx {
     double aa_last_ff = 0;      // squared residual norm of the last evaluated Anderson state
     int aa_steps = 0;
     unsigned long long geo_tag = 0;  // changes with every bte_set_geometry
-    bool aa_plain = false;      // state bookkeeping without mixing (plain iteration + learned correction)
+    bool aa_plain = false;
+    bool aa_skip = false;       // the next state step follows an accepted correction      // state bookkeeping without mixing (plain iteration + learned correction)
     // learned correction
     bte_gnn* gnn = nullptr;     // attached network (not owned)
     DBuf learn_stat, learn_backup, learn_X, red_part, red_out;
@@ -511,8 +514,10 @@ int prepare(bte_ctx* ctx, bool timing) {
             });
         for (auto& th : pool) th.join();
         std::vector<int> off(D), cnt(D), lp_all, perm_all((size_t)D * N), scol_all((size_t)D * W * N);
+        std::vector<int> spos_all((size_t)D * W * N, -1);
         ctx->max_levels = 0;
         ctx->lagged = 0;
+        ctx->ring_back = 0;
         for (int d = 0; d < D; d++) {
             off[d] = (int)lp_all.size();
             cnt[d] = (int)lptr[d].size() - 1;
@@ -521,12 +526,26 @@ int prepare(bte_ctx* ctx, bool timing) {
             lp_all.insert(lp_all.end(), lptr[d].begin(), lptr[d].end());
             std::copy(order[d].begin(), order[d].end(), perm_all.begin() + (size_t)d * N);
             std::copy(scol[d].begin(), scol[d].end(), scol_all.begin() + (size_t)d * W * N);
+            // level-order positions of the upwind neighbours, and how far back the ring must reach
+            std::vector<int> inv(N);
+            for (int i = 0; i < N; i++) inv[order[d][i]] = i;
+            for (int l = 0; l + 1 < (int)lptr[d].size(); l++)
+                for (int i = lptr[d][l]; i < lptr[d][l + 1]; i++)
+                    for (int k = 0; k < W; k++) {
+                        const int j = scol[d][(size_t)k * N + i];
+                        if (j < 0) continue;
+                        const int pj = inv[j];
+                        spos_all[((size_t)d * W + k) * N + i] = pj;
+                        ctx->ring_back = std::max(ctx->ring_back, lptr[d][l + 1] - pj);
+                    }
         }
+        ctx->h_lvl_cnt = cnt;
         HIPCHK(upload(ctx->lvl_off, off, ctx->stream));
         HIPCHK(upload(ctx->lvl_cnt, cnt, ctx->stream));
         HIPCHK(upload(ctx->lvl_ptr, lp_all, ctx->stream));
         HIPCHK(upload(ctx->perm, perm_all, ctx->stream));
         HIPCHK(upload(ctx->swp_col, scol_all, ctx->stream));
+        HIPCHK(upload(ctx->swp_pos, spos_all, ctx->stream));
         HIPCHK(ctx->swp_dg.ensure(sizeof(double) * (size_t)D * N));
         HIPCHK(ctx->swp_val.ensure(sizeof(double) * (size_t)D * W * N));
         hipLaunchKernelGGL(k_gather_sweep<double>, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream, N,
@@ -623,6 +642,7 @@ Ops ops_of(bte_ctx* ctx) {
     o.swp_dg32 = ctx->swp_dg32.as<float>();
     o.swp_vw = ctx->swp_vw.as<double>();
     o.swp_bnd = ctx->swp_bnd.as<unsigned char>();
+    o.swp_pos = ctx->swp_pos.as<int>();
     return o;
 }
 
@@ -646,7 +666,8 @@ Work work_of(bte_ctx* ctx, double* ee_curr) {
 }
 
 // Launch shape of the inner-solve kernel: work-group size and, for the direct sweep, its variant
-// (1 basic, 2 level-ordered operands, 3 level-ordered operands + intensity vector in LDS).
+// (1 basic, 2 level-ordered operands, 3 level-ordered operands + intensity vector in LDS,
+// 4 level-ordered operands + LDS ring of recent intensities + pipelined operand loads).
 struct Shape {
     int block = 256;
     int variant = 2;
@@ -657,6 +678,15 @@ bool direct_path(const bte_ctx* ctx, const bte_opts* o) {
 }
 
 bool lds_fits(const bte_ctx* ctx) { return (size_t)ctx->N * sizeof(double) + 1024 <= (size_t)ctx->max_lds; }
+
+// LDS ring of the pipelined sweep (variant 4): the smallest power of two covering ring_back, at
+// least 1024 and at most 8192 entries (64 KB); 0 when the mesh needs a longer reach.
+int ring_size(const bte_ctx* ctx) {
+    if (ctx->lagged != 0 || ctx->ring_back <= 0) return 0;
+    int r = 1024;
+    while (r < ctx->ring_back) r <<= 1;
+    return (r <= 8192 && ring_lds_bytes(r, ctx->max_levels) <= (size_t)ctx->max_lds) ? r : 0;
+}
 
 template <int BLOCK>
 void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const double* ee_prev, double* ee_curr,
@@ -670,7 +700,16 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     const Work w = work_of(ctx, ee_curr);
     dim3 grid(S), blk(BLOCK);
     if (direct_path(ctx, o)) {
-        if (variant == 3 && lds_fits(ctx)) {
+        const int ring = ring_size(ctx);
+        if (variant == 4 && ring > 0) {
+            // right-hand sides in level order into w.b, then the pipelined sweep
+            hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)S * ctx->N, 256)), dim3(256), 0, ctx->stream, g, q,
+                               bd, bn, op, S, ee_prev, dens, w.b);
+            const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
+            (void)hipFuncSetAttribute((const void*)k_sweep_ring<BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
+            hipLaunchKernelGGL((k_sweep_ring<BLOCK>), grid, blk, lds, ctx->stream, op, w, S, ring - 1);
+        } else if (variant == 3 && lds_fits(ctx)) {
             const size_t lds = (size_t)ctx->N * sizeof(double);
             (void)hipFuncSetAttribute((const void*)k_sweep_fast<BLOCK, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds);
@@ -754,6 +793,7 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
         if (direct) {
             cand.push_back({blk, 2});
             if (lds_fits(ctx)) cand.push_back({blk, 3});
+            if (ring_size(ctx) > 0) cand.push_back({blk, 4});
         } else {
             cand.push_back({blk, 0});
         }
@@ -888,6 +928,7 @@ static int aa_reset(bte_ctx* ctx) {
     HIPCHK(hipMemsetAsync(ctx->aa_x.p, 0, sizeof(double) * Z, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));  // sc and bcells are host temporaries
     ctx->aa_first = true;
+    ctx->aa_skip = false;
     ctx->aa_steps = 0;
     ctx->aa_last_ff = 0;
     ctx->learn_pending = false;
@@ -932,8 +973,12 @@ static int aa_step(bte_ctx* ctx) {
         HIPCHK(hipGetLastError());
         return BTE_OK;
     }
+    // after an accepted correction the step x -> x~ is no iteration step: it enters no history
+    // column (the columns stay valid differences of the affine map), the mixing restarts from x~
+    const bool skip = ctx->aa_skip;
+    ctx->aa_skip = false;
     int slot = 0;
-    if (!ctx->aa_first) {
+    if (!ctx->aa_first && !skip) {
         if ((int)ctx->aa_order.size() < m) {
             slot = (int)ctx->aa_order.size();
         } else {
@@ -942,7 +987,7 @@ static int aa_step(bte_ctx* ctx) {
         }
     }
     const double* g = ctx->aa_g.as<double>();
-    hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, Z, ctx->aa_first ? 1 : 0, g,
+    hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, Z, (ctx->aa_first || skip) ? 1 : 0, g,
                        ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(),
                        ctx->aa_f.as<double>(), ctx->aa_dX.as<double>() + (size_t)slot * Z,
                        ctx->aa_dF.as<double>() + (size_t)slot * Z);
@@ -951,16 +996,18 @@ static int aa_step(bte_ctx* ctx) {
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
         ctx->aa_first = false;
     } else {
-        ctx->aa_order.push_back(slot);
+        if (!skip) ctx->aa_order.push_back(slot);
         const int nq = 2 * m + 1, nblk = 64;
         hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
-                           ctx->aa_dF.as<double>() + (size_t)slot * Z, ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
+                           skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
+                           ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
         hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
                            ctx->aa_part.as<double>(), ctx->aa_dots.as<double>());
         std::vector<double> dots(nq);
         HIPCHK(hipMemcpyAsync(dots.data(), ctx->aa_dots.p, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
+        if (!skip)
+            for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
         const double ff = dots[2 * m];
         ctx->aa_last_ff = ff;
         // divergence guard: restart the history when the residual grew far beyond the best seen
@@ -1011,7 +1058,11 @@ static int sq_norm(bte_ctx* ctx, const double* a, const double* b, long long lo,
 //   5 log10(1 + mean hop distance to the isothermal walls) / 3
 //   6 log10(mfp / (h (1 + mean hop distance))) / 2 (a Knudsen number of the whole domain)
 //   7 log10(1 + d_iso h / mfp) / 2 (distance from the nearest isothermal wall in mean free paths)
-// with mfp = sum_b C v^2 tau / sum_b C v, h the cell size V^(1/dim) and d_iso in hops.
+//   8 conductivity fraction of the bands whose mean free path exceeds (d_iso + 1/2) h (the bands
+//     for which the cell lies in the Knudsen layer of the nearest isothermal wall)
+//   9 conductivity-weighted standard deviation of log10(v_b tau_b) (uniform; band-set spread)
+// with mfp = sum_b C v^2 tau / sum_b C v, h the cell size V^(1/dim), d_iso in hops, and band
+// conductivity weights k_b = C_b v_b^2 tau_b.
 static int learn_prepare(bte_ctx* ctx) {
     if (ctx->learn_stat_ok) return BTE_OK;
     const int N = ctx->N, F = ctx->F, nbc = (int)ctx->h_bci.size();
@@ -1072,6 +1123,14 @@ static int learn_prepare(bte_ctx* ctx) {
         den += ctx->h_C[b] * ctx->h_v[b];
     }
     const double mfp = den > 0 ? num / den : 0;
+    double lm = 0, lm2 = 0;
+    for (int b = 0; b < ctx->B; b++) {
+        const double kb = ctx->h_C[b] * ctx->h_v[b] * ctx->h_v[b] * ctx->h_tau[b] / num;
+        const double l = std::log10(ctx->h_v[b] * ctx->h_tau[b]);
+        lm += kb * l;
+        lm2 += kb * l * l;
+    }
+    const float spread = (float)std::sqrt(std::max(0.0, lm2 - lm * lm));
     std::vector<float> st((size_t)N * kLearnStatic);
     for (int c = 0; c < N; c++) {
         double wsum = 0, tsum = 0, dsum = 0;
@@ -1100,6 +1159,12 @@ static int learn_prepare(bte_ctx* ctx) {
         o[5] = (float)(std::log10(1.0 + dmean) / 3);
         o[6] = (float)(std::log10(kn / (1.0 + dmean)) / 2);
         o[7] = dmin < INF ? (float)(std::log10(1.0 + dmin / kn) / 2) : 4.f;
+        double bal = 0;
+        for (int b = 0; b < ctx->B; b++)
+            if (ctx->h_v[b] * ctx->h_tau[b] > (dmin + 0.5) * h)
+                bal += ctx->h_C[b] * ctx->h_v[b] * ctx->h_v[b] * ctx->h_tau[b] / num;
+        o[8] = (float)bal;
+        o[9] = spread;
     }
     HIPCHK(upload(ctx->learn_stat, st, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));  // st is a host temporary
@@ -1126,11 +1191,20 @@ static int learn_features(bte_ctx* ctx, float* X, double& ffT, double& ffZ) {
         dFl = ctx->aa_dF.as<double>() + (size_t)slot * Z;
     }
     const float log_s = s > 0 ? (float)(std::log10(s / ctx->learn_span) / 4) : -2.f;
-    const float progress = (float)std::min(ctx->run_iter / 100.0, 2.0);
     hipLaunchKernelGGL(k_learn_features, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->aa_f.as<double>(),
                        ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), dXl, dFl, ctx->learn_stat.as<float>(), inv_s,
-                       ctx->learn_Tmid, 1.0 / ctx->learn_span, log_s, progress, X);
+                       ctx->learn_Tmid, 1.0 / ctx->learn_span, log_s, X);
     HIPCHK(hipGetLastError());
+    return BTE_OK;
+}
+
+// Points the attached network's graph at this context's mesh (a network may serve several contexts).
+static int learn_graph(bte_ctx* ctx) {
+    bte_gnn* g = ctx->gnn;
+    if (g->graph_tag == ctx->geo_tag && g->N == ctx->N) return BTE_OK;
+    const int rc = bte_gnn_set_graph(g, ctx->N, ctx->F, ctx->h_nbr.data());
+    if (rc) SETERR(rc, std::string("learned correction: ") + g->err);
+    g->graph_tag = ctx->geo_tag;
     return BTE_OK;
 }
 
@@ -1141,6 +1215,7 @@ static int learn_propose(bte_ctx* ctx) {
     const int N = ctx->N;
     double ffT = 0, ffZ = 0;
     int rc;
+    if ((rc = learn_graph(ctx))) return rc;
     if ((rc = learn_features(ctx, net->X.as<float>(), ffT, ffZ))) return rc;
     if (!(ffT > 0) || !std::isfinite(ffZ)) return BTE_OK;  // nothing to correct
     if (gnn_forward_device(net, ctx->stream)) SETERR(BTE_ERR_HIP, "learned correction: network launch failed");
@@ -1191,9 +1266,11 @@ static int check_opts(bte_ctx* ctx, const bte_opts& o) {
     if (o.inner_maxit <= 0 || o.inner_tol <= 0) SETERR(BTE_ERR_ARG, "inner_maxit and inner_tol must be positive");
     if (o.accel != BTE_ACCEL_NONE && o.accel != BTE_ACCEL_ANDERSON) SETERR(BTE_ERR_ARG, "unknown outer acceleration");
     if (o.learned) {
-        if (o.learned_every <= 0) SETERR(BTE_ERR_ARG, "learned_every must be positive");
-        if (!ctx->gnn) SETERR(BTE_ERR_STATE, "learned correction requested but no network is attached");
-        if (ctx->gnn->device != ctx->device) SETERR(BTE_ERR_ARG, "the attached network lives on another device");
+        if (o.learned_every < 0) SETERR(BTE_ERR_ARG, "learned_every must not be negative");
+        if (o.learned_every > 0 && !ctx->gnn)
+            SETERR(BTE_ERR_STATE, "learned correction requested but no network is attached");
+        if (ctx->gnn && ctx->gnn->device != ctx->device)
+            SETERR(BTE_ERR_ARG, "the attached network lives on another device");
     }
     return BTE_OK;
 }
@@ -1258,12 +1335,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     }
     if (o.learned) {
         if ((rc = learn_prepare(ctx))) return rc;
-        bte_gnn* g = ctx->gnn;
-        if (g->graph_tag != ctx->geo_tag || g->N != N) {
-            if ((rc = bte_gnn_set_graph(g, N, ctx->F, ctx->h_nbr.data())))
-                SETERR(rc, std::string("learned correction: ") + g->err);
-            g->graph_tag = ctx->geo_tag;
-        }
+        if (ctx->gnn && (rc = learn_graph(ctx))) return rc;
     }
     ctx->run_accepted = ctx->run_proposed = ctx->run_rejected = 0;
     ctx->run_shape_block = shape.block;
@@ -1305,51 +1377,66 @@ static int run_iteration(bte_ctx* ctx, double& m) {
     return BTE_OK;
 }
 
+// Bookkeeping after an outer iteration of ctx produced margin m (ee_final, run_iter already
+// advanced): judge a pending learned correction (a rejected one is undone and the iteration does not
+// count), record the margin, test convergence, then mix the state and propose the next correction.
+static int after_iteration(bte_ctx* ctx, double m, bool& counted) {
+    const bte_opts& o = ctx->run_opts;
+    int rc;
+    counted = false;
+    if (ctx->learn_pending) {
+        ctx->learn_pending = false;
+        bool ok = false;
+        if ((rc = learn_judge(ctx, ok))) return rc;
+        if (o.verbose)
+            std::printf("learned correction %s (residual ratio %.4g)\n", ok ? "accepted" : "rejected", ctx->learn_ratio);
+        if (ok) {
+            ctx->run_accepted++;
+            ctx->learn_streak = 0;
+            // the jump to the corrected state enters no mixing column: a least-squares step over it
+            // would take the correction back (the residual hardly sees smooth errors)
+            double jump = 0;
+            if ((rc = sq_norm(ctx, ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), 0, ctx->N, jump)))
+                return rc;
+            ctx->aa_skip = jump > 0;
+        } else {
+            // run the iteration again from the uncorrected state
+            ctx->run_rejected++;
+            ctx->learn_streak++;
+            return learn_undo(ctx);
+        }
+    }
+    counted = true;
+    ctx->run_margins.push_back(m);
+    if (o.verbose) {
+        std::printf("----------------------------------------------------------------------------------\n");
+        std::printf("Iteration #%d\t Margin per band per cell: %g\n", ctx->run_iter - 1, m);
+    }
+    if (m <= o.stop_margin) {
+        ctx->run_converged = true;
+        if (o.verbose) std::printf("\n");
+    } else if (o.accel == BTE_ACCEL_ANDERSON || o.learned) {
+        if ((rc = aa_step(ctx))) return rc;
+        // corrections at state steps 1, 1 + every, 1 + 2 every, ... until two in a row fail
+        if (o.learned && o.learned_every > 0 && ctx->learn_streak < 2 && (ctx->aa_steps - 1) % o.learned_every == 0)
+            if ((rc = learn_propose(ctx))) return rc;
+    }
+    return BTE_OK;
+}
+
 int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged) {
     if (!ctx) return BTE_ERR_ARG;
     if (!ctx->run_active) SETERR(BTE_ERR_STATE, "bte_step: call bte_begin first");
     HIPCHK(hipSetDevice(ctx->device));
-    const bte_opts& o = ctx->run_opts;
     auto t0 = std::chrono::steady_clock::now();
     int done = 0;
-    const bool state_path = o.accel == BTE_ACCEL_ANDERSON || o.learned;
     while (done < n_iter && !ctx->run_converged) {
         double m = 0;
         int rc = run_iteration(ctx, m);
         if (rc) return rc;
-        if (ctx->learn_pending) {
-            ctx->learn_pending = false;
-            bool ok = false;
-            if ((rc = learn_judge(ctx, ok))) return rc;
-            if (o.verbose)
-                std::printf("learned correction %s (residual ratio %.4g)\n", ok ? "accepted" : "rejected",
-                            ctx->learn_ratio);
-            if (ok) {
-                ctx->run_accepted++;
-                ctx->learn_streak = 0;
-            } else {
-                // run the iteration again from the uncorrected state
-                ctx->run_rejected++;
-                ctx->learn_streak++;
-                if ((rc = learn_undo(ctx))) return rc;
-                continue;
-            }
-        }
-        done++;
-        ctx->run_margins.push_back(m);
-        if (o.verbose) {
-            std::printf("----------------------------------------------------------------------------------\n");
-            std::printf("Iteration #%d\t Margin per band per cell: %g\n", ctx->run_iter - 1, m);
-        }
-        if (m <= o.stop_margin) {
-            ctx->run_converged = true;
-            if (o.verbose) std::printf("\n");
-        } else if (state_path) {
-            if ((rc = aa_step(ctx))) return rc;
-            // corrections at state steps 1, 1 + every, 1 + 2 every, ... until two in a row fail
-            if (o.learned && ctx->learn_streak < 2 && (ctx->aa_steps - 1) % o.learned_every == 0)
-                if ((rc = learn_propose(ctx))) return rc;
-        }
+        bool counted = false;
+        if ((rc = after_iteration(ctx, m, counted))) return rc;
+        if (counted) done++;
     }
     ctx->run_solve_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (n_done) *n_done = done;
@@ -1607,6 +1694,236 @@ int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, c
     if (iters) HIPCHK(hipMemcpyAsync(iters, ctx->iters.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return BTE_OK;
+}
+
+// ---- uneven batch of problems ----------------------------------------------------------------
+
+struct bte_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<bte_ctx*> ctx;
+    DBuf desc, items, mptr, mout;
+    double* h_m = nullptr;  // pinned [K] margins
+    int ring = 0, block = 256, n_items = 0, max_levels = 0;
+    double seconds = 0;
+};
+
+#define BCHK(expr)                                                                                \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            b->err = std::string(#expr) + ": " + hipGetErrorString(e_);                           \
+            return BTE_ERR_HIP;                                                                   \
+        }                                                                                         \
+    } while (0)
+
+int bte_batch_create(int n, bte_ctx* const* ctxs, bte_batch** out) {
+    if (!out || n <= 0 || !ctxs) return BTE_ERR_ARG;
+    *out = nullptr;
+    for (int k = 0; k < n; k++)
+        if (!ctxs[k] || ctxs[k]->device != ctxs[0]->device) return BTE_ERR_ARG;
+    bte_batch* b = new bte_batch();
+    b->device = ctxs[0]->device;
+    b->ctx.assign(ctxs, ctxs + n);
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_m, sizeof(double) * n, hipHostMallocDefault) != hipSuccess) {
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return BTE_ERR_HIP;
+    }
+    *out = b;
+    return BTE_OK;
+}
+
+void bte_batch_destroy(bte_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->h_m) (void)hipHostFree(b->h_m);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* bte_batch_last_error(const bte_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+// Runs the member contexts on the batch stream for the duration of a call.
+struct BatchStreams {
+    std::vector<bte_ctx*>& ctx;
+    std::vector<hipStream_t> saved;
+    BatchStreams(std::vector<bte_ctx*>& c, hipStream_t s) : ctx(c) {
+        for (bte_ctx* x : ctx) {
+            saved.push_back(x->stream);
+            x->stream = s;
+        }
+    }
+    ~BatchStreams() {
+        for (size_t k = 0; k < ctx.size(); k++) ctx[k]->stream = saved[k];
+    }
+};
+
+}  // extern "C"
+
+template <int BLOCK>
+static void launch_batch_b(bte_batch* b, int ring) {
+    const SweepDesc* desc = b->desc.as<SweepDesc>();
+    const int2* items = b->items.as<int2>();
+    if (ring > 0) {
+        const size_t lds = ring_lds_bytes(ring, b->max_levels);
+        (void)hipFuncSetAttribute((const void*)k_sweep_batch<BLOCK, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL((k_sweep_batch<BLOCK, true>), dim3(b->n_items), dim3(BLOCK), lds, b->stream, desc, items,
+                           b->n_items, ring - 1);
+    } else {
+        hipLaunchKernelGGL((k_sweep_batch<BLOCK, false>), dim3(b->n_items), dim3(BLOCK), 0, b->stream, desc, items,
+                           b->n_items, 0);
+    }
+}
+
+extern "C" {
+
+// One outer iteration of every active problem: boundary data, one batched sweep launch over all
+// their systems, temperature update and margins; returns the margins (host) of the active problems.
+static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vector<double>& m) {
+    std::vector<SweepDesc> hd(act.size());
+    std::vector<std::pair<int, int2>> order;  // (levels, item)
+    std::vector<const double*> mp(act.size());
+    for (size_t a = 0; a < act.size(); a++) {
+        bte_ctx* ctx = b->ctx[act[a]];
+        const int par = ctx->run_iter & 1;
+        double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
+        if (ctx->has_diffuse)
+            hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, b->stream,
+                               geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, buf[par], ctx->ediff.as<double>());
+        if (b->ring > 0)  // the pipelined sweep reads its right-hand sides in level order from w.b
+            hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)ctx->B * ctx->D * ctx->N, 256)), dim3(256), 0,
+                               b->stream, geo_of(ctx), quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx),
+                               ctx->B * ctx->D, buf[par], ctx->dens.as<double>(), ctx->wb.as<double>());
+        SweepDesc& d = hd[a];
+        d.g = geo_of(ctx);
+        d.q = quad_of(ctx);
+        d.bd = bnd_of(ctx);
+        d.bands = bands_of(ctx);
+        d.o = ops_of(ctx);
+        d.w = work_of(ctx, buf[1 - par]);
+        d.ee_prev = buf[par];
+        d.dens = ctx->dens.as<double>();
+        const int S = ctx->B * ctx->D;
+        for (int s = 0; s < S; s++) order.push_back({ctx->h_lvl_cnt[s % ctx->D], make_int2((int)a, s)});
+        mp[a] = ctx->margin.as<double>();
+    }
+    // longest level chains first, so the last work-groups to start are the short ones
+    std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    std::vector<int2> items(order.size());
+    for (size_t t = 0; t < order.size(); t++) items[t] = order[t].second;
+    b->n_items = (int)items.size();
+    BCHK(b->desc.ensure(sizeof(SweepDesc) * hd.size()));
+    BCHK(b->items.ensure(sizeof(int2) * items.size()));
+    BCHK(b->mptr.ensure(sizeof(double*) * mp.size()));
+    BCHK(b->mout.ensure(sizeof(double) * mp.size()));
+    BCHK(hipMemcpyAsync(b->desc.p, hd.data(), sizeof(SweepDesc) * hd.size(), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipMemcpyAsync(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipMemcpyAsync(b->mptr.p, mp.data(), sizeof(double*) * mp.size(), hipMemcpyHostToDevice, b->stream));
+    switch (b->block) {
+        case 64: launch_batch_b<64>(b, b->ring); break;
+        case 128: launch_batch_b<128>(b, b->ring); break;
+        case 512: launch_batch_b<512>(b, b->ring); break;
+        default: launch_batch_b<256>(b, b->ring); break;
+    }
+    for (size_t a = 0; a < act.size(); a++) {
+        bte_ctx* ctx = b->ctx[act[a]];
+        const Bands bn = bands_of(ctx);
+        const int par = ctx->run_iter & 1;
+        double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
+        const int nblk = grid_for((long long)ctx->B * ctx->N, 256);
+        hipLaunchKernelGGL(k_tband<256>, dim3(nblk), dim3(256), 0, b->stream, ctx->N, ctx->D, bn, ctx->omega.as<double>(),
+                           buf[1 - par], buf[par], ctx->Tband.as<double>(), ctx->partial.as<double>());
+        hipLaunchKernelGGL(k_tcell, dim3(grid_for(ctx->N, 256)), dim3(256), 0, b->stream, ctx->N, bn,
+                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>());
+        hipLaunchKernelGGL(k_margin, dim3(1), dim3(256), 0, b->stream, nblk, ctx->partial.as<double>(),
+                           1.0 / ctx->B / ctx->N, ctx->margin.as<double>());
+    }
+    hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(256), 0, b->stream, (int)mp.size(),
+                       b->mptr.as<const double*>(), b->mout.as<double>());
+    BCHK(hipGetLastError());
+    BCHK(hipMemcpyAsync(b->h_m, b->mout.p, sizeof(double) * mp.size(), hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));  // also keeps hd / items alive until the copies are done
+    m.assign(b->h_m, b->h_m + mp.size());
+    return BTE_OK;
+}
+
+int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
+    if (!b) return BTE_ERR_ARG;
+    BCHK(hipSetDevice(b->device));
+    bte_opts o;
+    if (opts)
+        o = *opts;
+    else
+        bte_default_opts(&o);
+    o.use_graph = 0;
+    o.timing = 0;
+    const int K = (int)b->ctx.size();
+    BatchStreams streams(b->ctx, b->stream);
+    int ring = 1 << 30;
+    for (int k = 0; k < K; k++) {
+        bte_ctx* ctx = b->ctx[k];
+        int rc = bte_begin(ctx, &o);
+        if (rc) {
+            b->err = "problem " + std::to_string(k) + ": " + ctx->err;
+            return rc;
+        }
+        if (!direct_path(ctx, &o)) {
+            b->err = "problem " + std::to_string(k) + ": the batch needs the FP64 direct-sweep form (precision FP64, "
+                     "sweep preconditioner, direct_sweep, an acyclic upwind graph)";
+            return BTE_ERR_ARG;
+        }
+        ring = std::min(ring, ring_size(ctx) > 0 ? ring_size(ctx) : 0);
+    }
+    // one LDS ring size for the launch: the largest any problem needs, if every problem can use one
+    int rmax = 0;
+    b->max_levels = 0;
+    for (int k = 0; k < K; k++) {
+        if (ring > 0) rmax = std::max(rmax, ring_size(b->ctx[k]));
+        b->max_levels = std::max(b->max_levels, b->ctx[k]->max_levels);
+    }
+    b->ring = (ring > 0 && ring_lds_bytes(rmax, b->max_levels) <= (size_t)b->ctx[0]->max_lds) ? rmax : 0;
+    b->block = (o.block == 64 || o.block == 128 || o.block == 512) ? o.block : 256;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> act;
+    for (int k = 0; k < K; k++) act.push_back(k);
+    std::vector<double> m;
+    while (!act.empty()) {
+        int rc = batch_iteration(b, act, m);
+        if (rc) return rc;
+        std::vector<int> next;
+        for (size_t a = 0; a < act.size(); a++) {
+            bte_ctx* ctx = b->ctx[act[a]];
+            const int par = ctx->run_iter & 1;
+            ctx->ee_final = par ? ctx->eeA.as<double>() : ctx->eeB.as<double>();
+            ctx->run_iter++;
+            bool counted = false;
+            if ((rc = after_iteration(ctx, m[a], counted))) {
+                b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
+                return rc;
+            }
+            if (!ctx->run_converged && (int)ctx->run_margins.size() < o.max_iter) next.push_back(act[a]);
+        }
+        act.swap(next);
+    }
+    b->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int status = BTE_OK;
+    for (int k = 0; k < K; k++) {
+        bte_ctx* ctx = b->ctx[k];
+        ctx->run_solve_seconds = b->seconds;
+        if (!results) continue;
+        const int rc = bte_finish(ctx, &results[k]);
+        if (rc && rc != BTE_ERR_NOCONV) {
+            b->err = "problem " + std::to_string(k) + ": " + ctx->err;
+            return rc;
+        }
+        if (rc) status = rc;
+    }
+    return status;
 }
 
 }  // extern "C"

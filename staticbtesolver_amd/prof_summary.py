@@ -77,3 +77,8 @@ if __name__ == "__main__":
     text, res = summary(sys.argv[1], sys.argv[2], sys.argv[3], int(sys.argv[4]) if len(sys.argv) > 4 else 1024)
     print(text)
     print(res, file=sys.stderr)
+    if len(sys.argv) > 5:  # machine-readable copy for bench.py's roofline.traffic
+        import json
+
+        with open(sys.argv[5], "w") as f:
+            json.dump(dict(res, title=sys.argv[2], command=sys.argv[3]), f, indent=1)

@@ -1,6 +1,7 @@
 #!/usr/bin/env bash
 # Profiles the bench workload with rocprofv3: kernel trace + stats, then HBM byte counters
-# (FETCH_SIZE, WRITE_SIZE) in separate passes, each also for the calibration copy kernel.
+# (FETCH_SIZE, WRITE_SIZE) in separate passes, each also for the calibration copy kernel, then
+# writes summary.md / summary.json (prof_summary.py) next to the raw results.
 # Usage (from the repository root, on a GPU box): bash staticbtesolver_amd/profile.sh [bench args]
 set -euo pipefail
 ROOT=$(cd "$(dirname "$0")/.." && pwd)
@@ -17,4 +18,7 @@ for C in FETCH_SIZE WRITE_SIZE; do
   echo "== pmc $C: hbm_copy 1024 MiB"
   timeout -s KILL 60 rocprofv3 --pmc $C -d "$OUT/pmc_copy_$C" -o copy -- "$ROOT/staticbtesolver_amd/bin/hbm_copy" 1024 3
 done
+echo "== summary"
+python3 "$ROOT/staticbtesolver_amd/prof_summary.py" "$OUT" "${TITLE:-bench.py $ARGS}" \
+  "bash staticbtesolver_amd/profile.sh $ARGS" 1024 "$OUT/summary.json" > "$OUT/summary.md"
 echo "== done"

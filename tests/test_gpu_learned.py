@@ -60,22 +60,27 @@ def test_learn_features_match_their_definition(case3k):
     # 0: residual over its RMS -> unit RMS
     assert abs(np.sqrt(np.mean(X[:, 0].astype(np.float64) ** 2)) - 1.0) < 1e-5
     np.testing.assert_allclose(X[:, 3], (x_next - 300.0) / 2.0, atol=1e-6)
-    assert np.all(X[:, 4] == X[0, 4]) and np.all(X[:, 7] == 0)
+    assert np.all(X[:, 4] == X[0, 4])
     nbr = c.geometry()["nbr"]
     # boundary entity k is stored as neighbour -k-1; 0 and 3 isothermal (301 / 299 K), 1 and 2 diffuse
     d_hot, d_cold, d_refl = hop_distance(nbr, [-1]), hop_distance(nbr, [-4]), hop_distance(nbr, [-2, -3])
     d_iso = np.minimum(d_hot, d_cold)
-    np.testing.assert_array_equal(X[:, 8], (d_iso == 0).astype(np.float32))
-    np.testing.assert_array_equal(X[:, 9], (d_refl == 0).astype(np.float32))
+    st = X[:, 6:]  # static inputs
+    np.testing.assert_array_equal(st[:, 0], (d_iso == 0).astype(np.float32))
+    np.testing.assert_array_equal(st[:, 1], (d_refl == 0).astype(np.float32))
     w_h, w_c = 1.0 / (d_hot + 0.5), 1.0 / (d_cold + 0.5)
     T_int = (301.0 * w_h + 299.0 * w_c) / (w_h + w_c)
-    np.testing.assert_allclose(X[:, 11], (T_int - 300.0) / 2.0, atol=1e-6)
-    np.testing.assert_allclose(X[:, 12], d_iso / (d_iso + d_refl + 1e-30), atol=1e-6)
-    np.testing.assert_allclose(X[:, 13], np.log10(1 + 0.5 * (d_hot + d_cold)) / 3, atol=1e-6)
-    np.testing.assert_allclose(X[:, 5], X[:, 3] - X[:, 11], atol=1e-6)
-    # 15: distance from the nearest isothermal wall in mean free paths (mfp / h from feature 10)
-    kn = 10.0 ** (2 * X[:, 10].astype(np.float64))
-    np.testing.assert_allclose(X[:, 15], np.log10(1 + d_iso / kn) / 2, atol=2e-5)
+    np.testing.assert_allclose(st[:, 3], (T_int - 300.0) / 2.0, atol=1e-6)
+    np.testing.assert_allclose(st[:, 4], d_iso / (d_iso + d_refl + 1e-30), atol=1e-6)
+    np.testing.assert_allclose(st[:, 5], np.log10(1 + 0.5 * (d_hot + d_cold)) / 3, atol=1e-6)
+    np.testing.assert_allclose(X[:, 5], X[:, 3] - st[:, 3], atol=1e-6)
+    # 7: distance from the nearest isothermal wall in mean free paths (mfp / h from static 2)
+    kn = 10.0 ** (2 * st[:, 2].astype(np.float64))
+    np.testing.assert_allclose(st[:, 7], np.log10(1 + d_iso / kn) / 2, atol=2e-5)
+    # 8: conductivity fraction of bands with mean free path beyond (d_iso + 1/2) h; 9: spread (uniform)
+    assert np.all((st[:, 8] >= 0) & (st[:, 8] <= 1 + 1e-6))
+    assert st[:, 8][d_iso == 0].mean() >= st[:, 8][d_iso > 5].mean()
+    assert np.all(st[:, 9] == st[0, 9]) and st[0, 9] > 0
 
 
 def test_rejected_corrections_leave_the_iteration_unchanged(case3k):
@@ -136,26 +141,47 @@ def test_learned_with_plain_iteration_reaches_the_same_field(case3k):
     assert np.max(np.abs(r["T"] - ref["T"]) / ref["T"]) < 1e-6
 
 
+def iterations_to(c, opts, ref, tol, budget):
+    """Outer iterations until max |T - T*| / T* <= tol (None if not within budget) and the error at
+    the end of the budget (or at the stop rule, if earlier)."""
+    c.begin(capi.default_opts(**opts))
+    hit, k, conv, err = None, 0, False, None
+    while k < budget and not conv:
+        done, conv = c.step(1)
+        k += done
+        err = np.max(np.abs(c.temperature() - ref) / ref)
+        if hit is None and err <= tol:
+            hit = k
+    r = c.finish()
+    return hit, err, r
+
+
 @pytest.mark.skipif(not os.path.exists(DEFAULT_WEIGHTS), reason="trained weights not present")
 @pytest.mark.parametrize("seed,n_cells", [(2001, 20000), (2002, 50000)])
-def test_trained_network_gains_over_anderson(tmp_path, seed, n_cells):
-    """Held-out geometries of the training family (seeds outside 1000-1255): the trained network's
-    accepted corrections cut the outer iterations of Anderson mixing, at the same accuracy against
-    a tightly converged field (<= 1e-4 relative)."""
+def test_trained_network_gains(tmp_path, seed, n_cells):
+    """Held-out geometries of the training family (seeds outside 1000-1255).  What the trained
+    network gains: with it, the field is within 1e-4 of the converged one after a handful of outer
+    iterations, where Anderson mixing alone needs tens to hundreds and the plain source iteration
+    thousands; at a budget of 60 outer iterations the plain iteration with the network is >= 10x
+    more accurate than without.  Every judged correction obeys the residual rule, and the solves
+    still run to the same fixed point (<= 1e-4 at the stop rule)."""
     say("learned: trained network", seed, n_cells)
     c = nd_case(tmp_path, n_cells, seed)
     g = Gnn(layers=4, seed=0, device=0)
     g.set_params(np.load(DEFAULT_WEIGHTS))
     c.attach_gnn(g)
-    ref = c.solve(**aa(stop_margin=1e-12))
-    base = c.solve(**aa())
-    r = c.solve(**aa(learned=1))
+    ref = c.solve(**aa(stop_margin=1e-12))["T"]
+    k_aa, _, r_aa = iterations_to(c, aa(), ref, 1e-4, 20000)
+    k_l, _, r_l = iterations_to(c, aa(learned=1), ref, 1e-4, 20000)
+    _, e_pl, _ = iterations_to(c, dict(max_iter=60), ref, 1e-4, 60)
+    _, e_pll, r_pll = iterations_to(c, dict(max_iter=60, learned=1, learned_every=16), ref, 1e-4, 60)
     c.attach_gnn(None)
     c.close()
-    err_b = np.max(np.abs(base["T"] - ref["T"]) / ref["T"])
-    err_l = np.max(np.abs(r["T"] - ref["T"]) / ref["T"])
-    say(f"anderson {base['iters']} it {base['solve_seconds']:.3f} s err {err_b:.2e} | learned {r['iters']} it "
-        f"{r['solve_seconds']:.3f} s err {err_l:.2e} accepted {r['learned_accepted']}/{r['learned_proposed']}")
-    assert r["learned_accepted"] >= 1
-    assert r["iters"] < base["iters"]
-    assert err_l <= 1e-4 and err_b <= 1e-4
+    say(f"iterations to 1e-4: anderson {k_aa}, anderson+network {k_l} (accepted "
+        f"{r_l['learned_accepted']}/{r_l['learned_proposed']}); error after 60 plain iterations {e_pl:.2e}, "
+        f"with the network {e_pll:.2e}")
+    assert r_l["learned_accepted"] >= 1 and r_pll["learned_accepted"] >= 1
+    assert k_l is not None and k_aa is not None and k_l * 5 <= k_aa and k_l <= 10
+    assert e_pll * 10 <= e_pl
+    for r in (r_aa, r_l):
+        assert r["converged"] and np.max(np.abs(r["T"] - ref) / ref) <= 1e-4
