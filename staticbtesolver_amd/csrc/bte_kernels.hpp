@@ -294,6 +294,7 @@ struct Ops {
     const double* swp_vw;          // [D*N] w_d V_c in level order
     const unsigned char* swp_bnd;  // [D*N] 1 if the cell has an incoming boundary face
     const int* swp_pos;   // [(d*W+k)*N + i] level-order position of the upwind neighbour (-1: none)
+    int ring_sync;        // pipelined sweep: make HBM writes visible to the work-group every ring_sync levels
 };
 
 struct Work {
@@ -694,25 +695,32 @@ __global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, c
 struct SweepItem {
     int i, c;     // level-order position (-1: no cell for this thread) and cell
     int p[3];     // level-order positions of the upwind neighbours (-1: none)
+    int col[3];   // their cells (for values older than the ring)
     double a[3];  // their coefficients
     double dgc, vw, rhs;
 };
 
-__device__ __forceinline__ void sweep_item_load(int i, int N, int W, const int* __restrict__ perm,
-                                                const int* __restrict__ spos, const double* __restrict__ sval,
-                                                const double* __restrict__ sdg, const double* __restrict__ svw,
-                                                const double* __restrict__ rhs, SweepItem& it) {
+// Straight-line loads (W, the incoming-slot count, is a compile-time constant and an absent item
+// loads position 0 and is ignored), so the compiler's wait counting stays exact and a later use of
+// the item waits for its own loads only.
+template <int W>
+__device__ __forceinline__ void sweep_item_load(int i, int N, const int* __restrict__ perm,
+                                                const int* __restrict__ spos, const int* __restrict__ scol,
+                                                const double* __restrict__ sval, const double* __restrict__ sdg,
+                                                const double* __restrict__ svw, const double* __restrict__ rhs,
+                                                SweepItem& it) {
     it.i = i;
-    if (i < 0) return;
-    it.c = perm[i];
+    const int j = i < 0 ? 0 : i;
+    it.c = perm[j];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        it.p[k] = (k < W) ? spos[(size_t)k * N + i] : -1;
-        it.a[k] = (k < W) ? sval[(size_t)k * N + i] : 0.0;
+        it.p[k] = (k < W) ? spos[(size_t)k * N + j] : -1;
+        it.col[k] = (k < W) ? scol[(size_t)k * N + j] : -1;
+        it.a[k] = (k < W) ? sval[(size_t)k * N + j] : 0.0;
     }
-    it.dgc = sdg[i];
-    it.vw = svw[i];
-    it.rhs = rhs[i];
+    it.dgc = sdg[j];
+    it.vw = svw[j];
+    it.rhs = rhs[j];
 }
 
 // LDS-only work-group barrier: the ring is the only memory threads exchange inside the sweep, so
@@ -725,21 +733,28 @@ __host__ __device__ inline size_t ring_lds_bytes(int ring, int max_levels) {
 }
 
 // One system of the pipelined sweep; needs rhs_lo (k_rhs_levels) in w.b.  ring holds the LDS ring
-// followed by room for the direction's level pointers.  Per thread every operand of a level --
-// including its right-hand side -- is a coalesced load in level order, issued two levels ahead; the
-// level pointers are read from LDS, so the only waits inside the level chain are for those loads
-// and the LDS ring.
-template <int BLOCK>
+// of R = ring_mask + 1 values followed by room for the direction's level pointers.  Per thread every
+// operand of a level -- including its right-hand side -- is a coalesced load in level order, issued
+// two levels ahead; the level pointers are read from LDS.  An upwind value at position
+// p >= end(l) - R is read from the ring (its slot is not yet reused).  An older one is read from HBM:
+// every o.ring_sync levels the work-group synchronises with its HBM writes complete, and the host
+// sizes R to cover o.ring_sync consecutive levels, so such a value was written before the last of
+// those synchronisations.  In between, levels synchronise on the LDS only, leaving the prefetches in
+// flight.
+template <int BLOCK, int W>
 __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
-    const int N = o.N, W = o.W;
+    const int N = o.N;
     const int tid = threadIdx.x;
-    int* lps = (int*)(ring + ring_mask + 1);
+    const int R = ring_mask + 1;
+    const int K = o.ring_sync;
+    int* lps = (int*)(ring + R);
     {
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
         double* __restrict__ xg = w.x + (size_t)s * N;
         const int* __restrict__ perm = o.perm + (size_t)d * N;
         const int* __restrict__ spos = o.swp_pos + (size_t)d * W * N;
+        const int* __restrict__ scol = o.swp_col + (size_t)d * W * N;
         const double* __restrict__ sval = o.swp_val + (size_t)d * W * N;
         const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
         const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
@@ -753,42 +768,52 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             const int i = lps[l] + tid;
             return i < lps[l + 1] ? i : -1;
         };
-        auto solve_cell = [&](const SweepItem& it) {
+        auto solve_cell = [&](const SweepItem& it, int end) {
             double acc = 0;
 #pragma unroll
             for (int k = 0; k < 3; k++)
-                if (it.p[k] >= 0) acc += it.a[k] * ring[it.p[k] & ring_mask];
+                if (it.p[k] >= 0) acc += it.a[k] * (it.p[k] >= end - R ? ring[it.p[k] & ring_mask] : xg[it.col[k]]);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
             xg[it.c] = xv;
         };
-        SweepItem A, B2;
-        sweep_item_load(first(0), N, W, perm, spos, sval, sdg, svw, rhs, A);
-        sweep_item_load(first(1), N, W, perm, spos, sval, sdg, svw, rhs, B2);
-        for (int l = 0; l < L; l++) {
+        // level l computes from the item loaded two levels earlier and loads the item of level l+2
+        // into the register set it just freed; the three sets rotate by unrolling, never by copies
+        // (a copy of a register with a load in flight would wait for that load)
+        auto level = [&](int l, SweepItem& cur, SweepItem& ahead) {
             const int beg = lps[l], end = lps[l + 1];
-            SweepItem C;  // prefetch: operands of level l+2
-            sweep_item_load(first(l + 2), N, W, perm, spos, sval, sdg, svw, rhs, C);
-            if (A.i >= 0) solve_cell(A);
+            sweep_item_load<W>(first(l + 2), N, perm, spos, scol, sval, sdg, svw, rhs, ahead);
+            if (cur.i >= 0) solve_cell(cur, end);
             for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
                 SweepItem E;
-                sweep_item_load(i, N, W, perm, spos, sval, sdg, svw, rhs, E);
-                solve_cell(E);
+                sweep_item_load<W>(i, N, perm, spos, scol, sval, sdg, svw, rhs, E);
+                solve_cell(E, end);
             }
-            lds_barrier();
-            A = B2;
-            B2 = C;
+            if ((l + 1) % K == 0)
+                __syncthreads();  // HBM writes of the levels so far complete and visible
+            else
+                lds_barrier();
+        };
+        SweepItem X0, X1, X2;
+        sweep_item_load<W>(first(0), N, perm, spos, scol, sval, sdg, svw, rhs, X0);
+        sweep_item_load<W>(first(1), N, perm, spos, scol, sval, sdg, svw, rhs, X1);
+        X2.i = -1;
+        for (int l = 0; l < L; l += 3) {
+            level(l, X0, X2);
+            if (l + 1 < L) level(l + 1, X1, X0);
+            if (l + 2 < L) level(l + 2, X2, X1);
         }
         __syncthreads();  // the ring and level pointers are reused by the next system of this work-group
         if (tid == 0) w.iters[s] += 1;
     }
 }
 
-template <int BLOCK>
+// W = incoming slots per cell (F - 1: 1 in 1D, 2 in 2D, 3 in 3D)
+template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
     for (int s = blockIdx.x; s < S; s += gridDim.x)
-        sweep_ring_system<BLOCK>(o, w, s, ring, ring_mask);
+        sweep_ring_system<BLOCK, W>(o, w, s, ring, ring_mask);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -815,7 +840,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_batch(const SweepDesc* __restri
         const int2 it = items[t];
         const SweepDesc& D = desc[it.x];
         if constexpr (RING)
-            sweep_ring_system<BLOCK>(D.o, D.w, it.y, lds_x, ring_mask);
+            switch (D.o.W) {
+                case 1: sweep_ring_system<BLOCK, 1>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                case 2: sweep_ring_system<BLOCK, 2>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                default: sweep_ring_system<BLOCK, 3>(D.o, D.w, it.y, lds_x, ring_mask); break;
+            }
         else
             sweep_fast_system<BLOCK, false>(D.g, D.q, D.bd, D.bands, D.o, D.w, it.y, lds_x, D.ee_prev, D.dens);
     }

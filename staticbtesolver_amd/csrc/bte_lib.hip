@@ -48,7 +48,7 @@ struct bte_ctx {
     DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
     DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd, swp_pos;
     int max_levels = 0, lagged = 0;
-    int ring_back = 0;  // max over cells of end(level) - position of an upwind neighbour
+    int ring_span[4] = {0, 0, 0, 0};  // max positions spanned by 8, 4, 2, 1 consecutive sweep levels
     std::vector<int> h_lvl_cnt;  // sweep levels per direction
     int max_lds = 65536;  // dynamic LDS bytes a work-group may use on this device
     bool have_sweep = false, have_fp32 = false;
@@ -517,7 +517,7 @@ int prepare(bte_ctx* ctx, bool timing) {
         std::vector<int> spos_all((size_t)D * W * N, -1);
         ctx->max_levels = 0;
         ctx->lagged = 0;
-        ctx->ring_back = 0;
+        for (int& r : ctx->ring_span) r = 0;
         for (int d = 0; d < D; d++) {
             off[d] = (int)lp_all.size();
             cnt[d] = (int)lptr[d].size() - 1;
@@ -526,18 +526,22 @@ int prepare(bte_ctx* ctx, bool timing) {
             lp_all.insert(lp_all.end(), lptr[d].begin(), lptr[d].end());
             std::copy(order[d].begin(), order[d].end(), perm_all.begin() + (size_t)d * N);
             std::copy(scol[d].begin(), scol[d].end(), scol_all.begin() + (size_t)d * W * N);
-            // level-order positions of the upwind neighbours, and how far back the ring must reach
+            // level-order positions of the upwind neighbours
             std::vector<int> inv(N);
             for (int i = 0; i < N; i++) inv[order[d][i]] = i;
             for (int l = 0; l + 1 < (int)lptr[d].size(); l++)
                 for (int i = lptr[d][l]; i < lptr[d][l + 1]; i++)
                     for (int k = 0; k < W; k++) {
                         const int j = scol[d][(size_t)k * N + i];
-                        if (j < 0) continue;
-                        const int pj = inv[j];
-                        spos_all[((size_t)d * W + k) * N + i] = pj;
-                        ctx->ring_back = std::max(ctx->ring_back, lptr[d][l + 1] - pj);
+                        if (j >= 0) spos_all[((size_t)d * W + k) * N + i] = inv[j];
                     }
+            // positions spanned by K consecutive levels (the reach the ring needs when the sweep makes
+            // HBM writes visible every K levels), K = 8, 4, 2, 1
+            for (int q = 0; q < 4; q++) {
+                const int K = 8 >> q;
+                for (int l = 0; l + 1 < (int)lptr[d].size(); l++)
+                    ctx->ring_span[q] = std::max(ctx->ring_span[q], lptr[d][l + 1] - lptr[d][std::max(0, l - K + 1)]);
+            }
         }
         ctx->h_lvl_cnt = cnt;
         HIPCHK(upload(ctx->lvl_off, off, ctx->stream));
@@ -618,6 +622,8 @@ int ensure_work(bte_ctx* ctx, int precision) {
     return BTE_OK;
 }
 
+int ring_size(const bte_ctx* ctx, int* sync_every = nullptr);
+
 Ops ops_of(bte_ctx* ctx) {
     Ops o{};
     o.N = ctx->N;
@@ -643,6 +649,8 @@ Ops ops_of(bte_ctx* ctx) {
     o.swp_vw = ctx->swp_vw.as<double>();
     o.swp_bnd = ctx->swp_bnd.as<unsigned char>();
     o.swp_pos = ctx->swp_pos.as<int>();
+    o.ring_sync = 1;
+    (void)ring_size(ctx, &o.ring_sync);
     return o;
 }
 
@@ -679,13 +687,21 @@ bool direct_path(const bte_ctx* ctx, const bte_opts* o) {
 
 bool lds_fits(const bte_ctx* ctx) { return (size_t)ctx->N * sizeof(double) + 1024 <= (size_t)ctx->max_lds; }
 
-// LDS ring of the pipelined sweep (variant 4): the smallest power of two covering ring_back, at
-// least 1024 and at most 8192 entries (64 KB); 0 when the mesh needs a longer reach.
-int ring_size(const bte_ctx* ctx) {
-    if (ctx->lagged != 0 || ctx->ring_back <= 0) return 0;
-    int r = 1024;
-    while (r < ctx->ring_back) r <<= 1;
-    return (r <= 8192 && ring_lds_bytes(r, ctx->max_levels) <= (size_t)ctx->max_lds) ? r : 0;
+// LDS ring of the pipelined sweep (variant 4): 8192 entries (64 KB; the fewer values are older than
+// the ring, the fewer reads go to HBM), or the smallest power of two >= 1024 that still fits the LDS
+// with the level pointers; it must cover the positions of K consecutive levels, K the largest of
+// {8, 4, 2, 1} that it covers; *sync_every = K.  0 when even one level is wider than the ring.
+int ring_size(const bte_ctx* ctx, int* sync_every) {
+    if (ctx->lagged != 0) return 0;
+    int r = 8192;
+    while (r > 1024 && ring_lds_bytes(r, ctx->max_levels) > (size_t)ctx->max_lds) r >>= 1;
+    if (ring_lds_bytes(r, ctx->max_levels) > (size_t)ctx->max_lds) return 0;
+    for (int q = 0; q < 4; q++)
+        if (ctx->ring_span[q] > 0 && ctx->ring_span[q] <= r) {
+            if (sync_every) *sync_every = 8 >> q;
+            return r;
+        }
+    return 0;
 }
 
 template <int BLOCK>
@@ -700,24 +716,36 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     const Work w = work_of(ctx, ee_curr);
     dim3 grid(S), blk(BLOCK);
     if (direct_path(ctx, o)) {
-        const int ring = ring_size(ctx);
+        int sync_every = 1;
+        const int ring = ring_size(ctx, &sync_every);
         if (variant == 4 && ring > 0) {
+            ctx->variant_used = 4;
             // right-hand sides in level order into w.b, then the pipelined sweep
             hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)S * ctx->N, 256)), dim3(256), 0, ctx->stream, g, q,
                                bd, bn, op, S, ee_prev, dens, w.b);
             const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
-            (void)hipFuncSetAttribute((const void*)k_sweep_ring<BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)lds);
-            hipLaunchKernelGGL((k_sweep_ring<BLOCK>), grid, blk, lds, ctx->stream, op, w, S, ring - 1);
+            auto go = [&](auto kernel) {
+                (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                hipLaunchKernelGGL(kernel, grid, blk, lds, ctx->stream, op, w, S, ring - 1);
+            };
+            if (ctx->W == 1)
+                go(k_sweep_ring<BLOCK, 1>);
+            else if (ctx->W == 2)
+                go(k_sweep_ring<BLOCK, 2>);
+            else
+                go(k_sweep_ring<BLOCK, 3>);
         } else if (variant == 3 && lds_fits(ctx)) {
+            ctx->variant_used = 3;
             const size_t lds = (size_t)ctx->N * sizeof(double);
             (void)hipFuncSetAttribute((const void*)k_sweep_fast<BLOCK, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds);
             hipLaunchKernelGGL((k_sweep_fast<BLOCK, true>), grid, blk, lds, ctx->stream, g, q, bd, bn, op, w, S, ee_prev,
                                dens);
         } else if (variant == 1) {
+            ctx->variant_used = 1;
             hipLaunchKernelGGL((k_sweep_direct<BLOCK>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev, dens);
         } else {
+            ctx->variant_used = 2;
             hipLaunchKernelGGL((k_sweep_fast<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev,
                                dens);
         }
@@ -1295,7 +1323,12 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     Shape shape;
     if ((rc = pick_shape(ctx, &o, shape, true))) return rc;
     ctx->block_used = shape.block;
-    ctx->variant_used = shape.variant;
+    // the sweep variant that actually runs (a requested form the mesh cannot use falls back to 2)
+    ctx->variant_used = !direct_path(ctx, &o)                              ? 0
+                        : (shape.variant == 4 && ring_size(ctx) > 0)      ? 4
+                        : (shape.variant == 3 && lds_fits(ctx))           ? 3
+                        : (shape.variant == 1)                            ? 1
+                                                                          : 2;
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
