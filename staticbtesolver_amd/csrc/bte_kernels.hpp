@@ -1049,19 +1049,31 @@ __global__ void k_aa_diff(int N, int first, const double* __restrict__ g, const 
 
 // partial[q * nblk + blk] for q < ncols: dF_q . dF_new, q in [ncols, 2 ncols): dF_{q-ncols} . f,
 // q = 2 ncols: f . f
+// Grid (nblk, ncols + 1): work-group (x, j < ncols) reads column dF_j once for both of its dots,
+// (x, ncols) forms f . f; each sums its chunk in the same fixed order for every call.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const double* __restrict__ dF,
                                                   const double* __restrict__ dF_new, const double* __restrict__ f,
                                                   double* __restrict__ partial) {
-    __shared__ double lds[BLOCK / 64];
-    const int nq = 2 * ncols + 1;
-    for (int q = 0; q < nq; q++) {
-        const double* a = (q < 2 * ncols) ? dF + (size_t)(q % ncols) * N : f;
-        const double* b = (q < ncols) ? dF_new : f;
-        double s[1] = {0};
-        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += gridDim.x * BLOCK) s[0] += a[c] * b[c];
+    __shared__ double lds[2 * (BLOCK / 64)];
+    const int j = blockIdx.y, nblk = gridDim.x;
+    double s[2] = {0, 0};
+    if (j < ncols) {
+        const double* a = dF + (size_t)j * N;
+        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
+            const double v = a[c];
+            s[0] += v * dF_new[c];
+            s[1] += v * f[c];
+        }
+        block_sum<BLOCK, 2>(s, lds);
+        if (threadIdx.x == 0) {
+            partial[(size_t)j * nblk + blockIdx.x] = s[0];
+            partial[(size_t)(ncols + j) * nblk + blockIdx.x] = s[1];
+        }
+    } else {
+        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];
         block_sum<BLOCK, 1>(s, lds);
-        if (threadIdx.x == 0) partial[(size_t)q * gridDim.x + blockIdx.x] = s[0];
+        if (threadIdx.x == 0) partial[(size_t)(2 * ncols) * nblk + blockIdx.x] = s[0];
     }
 }
 

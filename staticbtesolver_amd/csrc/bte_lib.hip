@@ -59,6 +59,7 @@ struct bte_ctx {
     int work_prec = -1;
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
     double* h_margin_pinned = nullptr;
+    double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients
     // graphs
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_key = -1;
@@ -315,7 +316,8 @@ int bte_ctx_create(int device, bte_ctx** out) {
     }
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_margin_pinned, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_margin_pinned, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_gamma, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return BTE_ERR_HIP;
     }
@@ -329,6 +331,7 @@ void bte_ctx_destroy(bte_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graphs(ctx);
     if (ctx->h_margin_pinned) (void)hipHostFree(ctx->h_margin_pinned);
+    if (ctx->h_gamma) (void)hipHostFree(ctx->h_gamma);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1026,7 +1029,7 @@ static int aa_step(bte_ctx* ctx) {
     } else {
         if (!skip) ctx->aa_order.push_back(slot);
         const int nq = 2 * m + 1, nblk = 64;
-        hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
+        hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk, m + 1), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
                            skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
                            ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
         hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
@@ -1040,7 +1043,10 @@ static int aa_step(bte_ctx* ctx) {
         ctx->aa_last_ff = ff;
         // divergence guard: restart the history when the residual grew far beyond the best seen
         if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
-        std::vector<double> gamma(m, 0.0);
+        // mixing coefficients go to the device from pinned memory: the copy runs in stream order and
+        // the buffer is rewritten only after the next step's synchronising read of its dots
+        double* gamma = ctx->h_gamma;
+        for (int j = 0; j < m; j++) gamma[j] = 0.0;
         if (ff > 1e6 * ctx->aa_best) {
             ctx->aa_order.clear();
         } else {
@@ -1053,10 +1059,9 @@ static int aa_step(bte_ctx* ctx) {
             small_solve(H, b, n, y);
             for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
         }
-        HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma.data(), sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, Z, m, ctx->aa_gamma.as<double>(), g,
                            ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(), ctx->aa_x.as<double>());
-        HIPCHK(hipStreamSynchronize(ctx->stream));  // gamma lives on the host stack
     }
     return load_state(ctx);
 }
