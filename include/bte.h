@@ -67,7 +67,10 @@ typedef struct bte_opts {
                              formed on the fly instead of BiCGSTAB (same solution); default 1 */
     int sweep_variant;    /* direct sweep kernel: 0 autotuned, 1 basic, 2 level-ordered operands,
                              3 level-ordered operands with the intensity vector held in LDS
-                             (needs n_cell * 8 bytes of LDS); default 0 */
+                             (needs n_cell * 8 bytes of LDS), 4 pipelined (operands loaded two
+                             levels ahead, recent intensities in an LDS ring), 5 as 4 with one
+                             wave per system (no work-group barriers); a form the mesh cannot
+                             use falls back to 2; default 0 */
     int accel;            /* outer iteration: BTE_ACCEL_NONE (the reference's source iteration) or
                              BTE_ACCEL_ANDERSON (Anderson mixing of the temperature field over the
                              last accel_memory iterates); default BTE_ACCEL_NONE */

@@ -677,18 +677,34 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
 // emission term for interior cells, rhs_at for cells with an incoming boundary face -- the values
 // the level sweeps form on the fly, computed up front, fully parallel, so that the pipelined sweep
 // streams them with its other operands instead of gathering by cell inside the level chain.
+// One thread per (direction, level position) forms the values of every band, so the level-ordered
+// operands are read once per direction rather than once per system.
 __global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const double* __restrict__ ee_prev,
                              const double* __restrict__ dens, double* __restrict__ rhs_lo) {
-    const int N = o.N;
-    const long long total = (long long)S * N;
+    const int N = o.N, D = o.D, B = S / D;
+    const long long total = (long long)D * N;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
-        const int s = (int)(t / N), i = (int)(t % N);
-        const int b = s / o.D, d = s % o.D;
-        const size_t di = (size_t)d * N + i;
-        const int c = o.perm[di];
-        rhs_lo[t] = o.swp_bnd[di] ? rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens)
-                                  : dens[(size_t)b * N + c] * o.swp_vw[di];
+        const int d = (int)(t / N), i = (int)(t % N);
+        if (o.swp_bnd[t]) continue;  // cells with an incoming boundary face: k_rhs_levels_bnd
+        const int c = o.perm[t];
+        const double vw = o.swp_vw[t];
+        for (int b = 0; b < B; b++) rhs_lo[((size_t)b * D + d) * N + i] = dens[(size_t)b * N + c] * vw;
+    }
+}
+
+// The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
+// their face loops run in work-groups of their own instead of stalling the interior waves.
+__global__ void k_rhs_levels_bnd(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const int* __restrict__ list,
+                                 int n_list, const double* __restrict__ ee_prev, const double* __restrict__ dens,
+                                 double* __restrict__ rhs_lo) {
+    const int N = o.N, D = o.D, B = S / D;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_list; k += gridDim.x * blockDim.x) {
+        const int t = list[k];
+        const int d = t / N, i = t % N;
+        const int c = o.perm[t];
+        for (int b = 0; b < B; b++)
+            rhs_lo[((size_t)b * D + d) * N + i] = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
     }
 }
 
@@ -814,6 +830,89 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int 
     extern __shared__ double ring[];
     for (int s = blockIdx.x; s < S; s += gridDim.x)
         sweep_ring_system<BLOCK, W>(o, w, s, ring, ring_mask);
+}
+
+// Single-wave form of the pipelined sweep (variant 5): one 64-lane wave per system, IPL cells of a
+// level per lane, all loaded two levels ahead.  A wave's LDS instructions execute in order, so the
+// ring needs no work-group barrier between levels: waiting for the wave's own LDS writes is enough.
+// Values older than the ring come from HBM under the same rule as sweep_ring_system (every
+// o.ring_sync levels the wave waits for its HBM writes).  Levels wider than 64 * IPL finish their
+// extra cells unpipelined.  Same arithmetic, in the same order, as the other sweeps.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+template <int W, int IPL>
+__device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
+    const int N = o.N;
+    const int lane = threadIdx.x;
+    const int R = ring_mask + 1;
+    const int K = o.ring_sync;
+    int* lps = (int*)(ring + R);
+    const int b = s / o.D, d = s % o.D;
+    const double vt = o.vt[b];
+    double* __restrict__ xg = w.x + (size_t)s * N;
+    const int* __restrict__ perm = o.perm + (size_t)d * N;
+    const int* __restrict__ spos = o.swp_pos + (size_t)d * W * N;
+    const int* __restrict__ scol = o.swp_col + (size_t)d * W * N;
+    const double* __restrict__ sval = o.swp_val + (size_t)d * W * N;
+    const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
+    const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
+    const double* __restrict__ rhs = w.b + (size_t)s * N;
+    const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
+    const int L = o.lvl_cnt[d];
+    for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
+    __syncthreads();
+    auto first = [&](int l, int k) {
+        if (l >= L) return -1;
+        const int i = lps[l] + lane + 64 * k;
+        return i < lps[l + 1] ? i : -1;
+    };
+    auto solve_cell = [&](const SweepItem& it, int end) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (it.p[k] >= 0) acc += it.a[k] * (it.p[k] >= end - R ? ring[it.p[k] & ring_mask] : xg[it.col[k]]);
+        const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
+        ring[it.i & ring_mask] = xv;
+        xg[it.c] = xv;
+    };
+    auto level = [&](int l, SweepItem (&cur)[IPL], SweepItem (&ahead)[IPL]) {
+        const int beg = lps[l], end = lps[l + 1];
+#pragma unroll
+        for (int k = 0; k < IPL; k++)
+            sweep_item_load<W>(first(l + 2, k), N, perm, spos, scol, sval, sdg, svw, rhs, ahead[k]);
+#pragma unroll
+        for (int k = 0; k < IPL; k++)
+            if (cur[k].i >= 0) solve_cell(cur[k], end);
+        for (int i = beg + 64 * IPL + lane; i < end; i += 64) {
+            SweepItem E;
+            sweep_item_load<W>(i, N, perm, spos, scol, sval, sdg, svw, rhs, E);
+            solve_cell(E, end);
+        }
+        if ((l + 1) % K == 0)
+            __syncthreads();  // HBM writes so far complete
+        else
+            wave_lds_sync();
+    };
+    SweepItem X0[IPL], X1[IPL], X2[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; k++) {
+        sweep_item_load<W>(first(0, k), N, perm, spos, scol, sval, sdg, svw, rhs, X0[k]);
+        sweep_item_load<W>(first(1, k), N, perm, spos, scol, sval, sdg, svw, rhs, X1[k]);
+        X2[k].i = -1;
+    }
+    for (int l = 0; l < L; l += 3) {
+        level(l, X0, X2);
+        if (l + 1 < L) level(l + 1, X1, X0);
+        if (l + 2 < L) level(l + 2, X2, X1);
+    }
+    __syncthreads();
+    if (lane == 0) w.iters[s] += 1;
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void k_sweep_wave(Ops o, Work w, int S, int ring_mask) {
+    extern __shared__ double ring[];
+    for (int s = blockIdx.x; s < S; s += gridDim.x) sweep_wave_system<W, 2>(o, w, s, ring, ring_mask);
 }
 
 // ----------------------------------------------------------------------------------------------

@@ -50,6 +50,8 @@ struct bte_ctx {
     int max_levels = 0, lagged = 0;
     int ring_span[4] = {0, 0, 0, 0};  // max positions spanned by 8, 4, 2, 1 consecutive sweep levels
     std::vector<int> h_lvl_cnt;  // sweep levels per direction
+    DBuf bnd_list;               // level-order entries d*N+i of cells with an incoming boundary face
+    int n_bnd_list = 0;
     int max_lds = 65536;  // dynamic LDS bytes a work-group may use on this device
     bool have_sweep = false, have_fp32 = false;
     // state + work
@@ -564,6 +566,15 @@ int prepare(bte_ctx* ctx, bool timing) {
                            quad_of(ctx), ctx->perm.as<int>(), ctx->w_eff.as<double>(), ctx->swp_vw.as<double>(),
                            ctx->swp_bnd.as<unsigned char>());
         HIPCHK(hipGetLastError());
+        // compact list of the level-order entries with an incoming boundary face
+        std::vector<unsigned char> hb((size_t)D * N);
+        HIPCHK(hipMemcpyAsync(hb.data(), ctx->swp_bnd.p, hb.size(), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::vector<int> blist;
+        for (size_t t = 0; t < hb.size(); t++)
+            if (hb[t]) blist.push_back((int)t);
+        ctx->n_bnd_list = (int)blist.size();
+        HIPCHK(upload(ctx->bnd_list, blist, ctx->stream));
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
             ctx->max_lds = lds;
@@ -707,6 +718,17 @@ int ring_size(const bte_ctx* ctx, int* sync_every) {
     return 0;
 }
 
+// Right-hand sides of all systems in level order (interior cells, then the boundary-cell list).
+void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out) {
+    const int S = ctx->B * ctx->D;
+    hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)ctx->D * ctx->N, 256)), dim3(256), 0, st, geo_of(ctx),
+                       quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx), S, ee_prev, dens, out);
+    if (ctx->n_bnd_list > 0)
+        hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(grid_for(ctx->n_bnd_list, 64)), dim3(64), 0, st, geo_of(ctx),
+                           quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx), S, ctx->bnd_list.as<int>(),
+                           ctx->n_bnd_list, ee_prev, dens, out);
+}
+
 template <int BLOCK>
 void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const double* ee_prev, double* ee_curr,
                     const double* dens) {
@@ -721,11 +743,24 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     if (direct_path(ctx, o)) {
         int sync_every = 1;
         const int ring = ring_size(ctx, &sync_every);
-        if (variant == 4 && ring > 0) {
+        if (variant == 5 && ring > 0) {
+            ctx->variant_used = 5;
+            enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b);
+            const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
+            auto go = [&](auto kernel) {
+                (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                hipLaunchKernelGGL(kernel, grid, dim3(64), lds, ctx->stream, op, w, S, ring - 1);
+            };
+            if (ctx->W == 1)
+                go(k_sweep_wave<1>);
+            else if (ctx->W == 2)
+                go(k_sweep_wave<2>);
+            else
+                go(k_sweep_wave<3>);
+        } else if (variant == 4 && ring > 0) {
             ctx->variant_used = 4;
             // right-hand sides in level order into w.b, then the pipelined sweep
-            hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)S * ctx->N, 256)), dim3(256), 0, ctx->stream, g, q,
-                               bd, bn, op, S, ee_prev, dens, w.b);
+            enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b);
             const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
             auto go = [&](auto kernel) {
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -825,6 +860,7 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
             cand.push_back({blk, 2});
             if (lds_fits(ctx)) cand.push_back({blk, 3});
             if (ring_size(ctx) > 0) cand.push_back({blk, 4});
+            if (blk == 64 && ring_size(ctx) > 0) cand.push_back({64, 5});  // one wave per system
         } else {
             cand.push_back({blk, 0});
         }
@@ -1330,6 +1366,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     ctx->block_used = shape.block;
     // the sweep variant that actually runs (a requested form the mesh cannot use falls back to 2)
     ctx->variant_used = !direct_path(ctx, &o)                              ? 0
+                        : (shape.variant == 5 && ring_size(ctx) > 0)      ? 5
                         : (shape.variant == 4 && ring_size(ctx) > 0)      ? 4
                         : (shape.variant == 3 && lds_fits(ctx))           ? 3
                         : (shape.variant == 1)                            ? 1
@@ -1834,9 +1871,7 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
             hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, b->stream,
                                geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, buf[par], ctx->ediff.as<double>());
         if (b->ring > 0)  // the pipelined sweep reads its right-hand sides in level order from w.b
-            hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)ctx->B * ctx->D * ctx->N, 256)), dim3(256), 0,
-                               b->stream, geo_of(ctx), quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx),
-                               ctx->B * ctx->D, buf[par], ctx->dens.as<double>(), ctx->wb.as<double>());
+            enqueue_rhs_levels(ctx, b->stream, buf[par], ctx->dens.as<double>(), ctx->wb.as<double>());
         SweepDesc& d = hd[a];
         d.g = geo_of(ctx);
         d.q = quad_of(ctx);

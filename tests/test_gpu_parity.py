@@ -123,7 +123,7 @@ def test_sweep_variants_and_block_sizes_agree(name):
     say("variants", name)
     c = capi.Case(**case_kwargs(name))
     base = c.solve(sweep_variant=1, block=256, stop_margin=1e-4)
-    for variant in (2, 3, 4):
+    for variant in (2, 3, 4, 5):
         for block in (64, 128, 256, 512):
             r = c.solve(sweep_variant=variant, block=block, stop_margin=1e-4)
             st = c.stats()
@@ -133,21 +133,23 @@ def test_sweep_variants_and_block_sizes_agree(name):
             np.testing.assert_allclose(r["flux"], base["flux"], rtol=1e-9, atol=1e-12 * np.abs(base["flux"]).max())
     r = c.solve(stop_margin=1e-4)  # autotuned
     st = c.stats()
-    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3, 4)
+    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3, 4, 5)
     np.testing.assert_allclose(r["T"], base["T"], rtol=1e-13)
 
 
-@pytest.mark.parametrize("block", [64, 256])
-def test_ring_sweep_matches_level_sweep(tmp_path, block):
-    """The pipelined LDS-ring sweep (variant 4) performs the level-ordered sweep's arithmetic in the
-    same order: identical iterates on a 20k-cell mesh, including levels wider than the work-group
-    (block 64), which the ring kernel finishes unpipelined."""
-    say("ring sweep", block)
-    p = meshgen.make_square_case(str(tmp_path), 20000, 2, 10, 2, L=1e-7)
+@pytest.mark.parametrize("variant,block,n_cells", [(4, 64, 20000), (4, 256, 20000), (5, 64, 20000),
+                                                   (4, 256, 50000), (5, 64, 50000)])
+def test_ring_sweep_matches_level_sweep(tmp_path, variant, block, n_cells):
+    """The pipelined LDS-ring sweeps (variant 4: work-group per system; 5: one wave per system)
+    perform the level-ordered sweep's arithmetic in the same order: identical iterates, including
+    levels wider than the pipelined lanes (finished unpipelined) and, at 50k cells, upwind values
+    older than the ring (read back from HBM)."""
+    say("ring sweep", variant, block, n_cells)
+    p = meshgen.make_square_case(str(tmp_path), n_cells, 2, 10, 2, L=1e-7)
     c = capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=1e-7, Ly=1e-7)
     a = c.solve(sweep_variant=2, block=block)
-    b = c.solve(sweep_variant=4, block=block)
-    assert c.stats()["variant"] == 4
+    b = c.solve(sweep_variant=variant, block=block)
+    assert c.stats()["variant"] == variant
     assert a["iters"] == b["iters"]
     np.testing.assert_array_equal(a["T"], b["T"])
     np.testing.assert_array_equal(a["margins"], b["margins"])
