@@ -199,6 +199,40 @@ def cpu_baseline(kw, outer_iters, budget_s=15.0):
                        f"restatement); x{S} systems x{outer_iters} outer iterations of the plain form")
 
 
+def roofline_only(case, form, k, prec):
+    """K outer iterations with per-kernel hipEvents; roofline of the inner solve.  Algorithmic bytes
+    per launch: 2 B_spmv per system for an exact sweep, and per BiCGSTAB iteration the fused lower
+    bound 2 B_spmv + 21 w n per active system, times the mean iterations per system.  With G bands
+    per work-group the operator (columns, values, row pointers) is read once per group: B_spmv's
+    operator part counts 1/G per system (B_shared), its two vectors 2 w n in full."""
+    from staticbtesolver_amd import capi
+
+    f = dict(form, max_iter=k, stop_margin=0.0, timing=1)
+    r = case.solve(**f)
+    st = case.stats()
+    launches = max(1, st["calls"]["solve"])
+    solve_ms = st["ms"]["solve"] / launches
+    S = case.N_band * case.N_dir
+    per_sys = r["inner_iters"] / launches / S
+    w = 8 if prec == capi.PREC_FP64 else 4
+    direct = st["variant"] > 0
+    G = max(1, st.get("band_group", 1))
+    n = case.N_cell
+    if direct:
+        b_alg = 2.0 * spmv_bytes(case, w)
+    else:
+        vec = 2.0 * w * n * S  # the product's x and y of every system
+        op = spmv_bytes(case, w) - vec
+        b_alg = (2.0 * (op / G + vec) + 21.0 * w * n * S) * per_sys
+    achieved = b_alg / (solve_ms * 1e-3) / 1e9
+    return {"roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel_ms": round(solve_ms, 4),
+                         "alg_bytes_per_launch": b_alg, "inner_iterations_per_system": round(per_sys, 2),
+                         "form": "direct sweep" if direct else "BiCGSTAB"},
+            "config": {"n_cell": case.N_cell, "systems": S, "outer_iterations": r["iters"],
+                       "sweep_variant": st["variant"], "block": st["block"], "band_group": G}}
+
+
 def profiled_traffic(workload, form, precision):
     """HBM bytes per dispatch of the dominant kernel from the committed rocprofv3 summary of this
     workload and form (profiles/<kernel>_<workload>_<form>_<precision>.json, written by
@@ -264,6 +298,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison solve")
     ap.add_argument("--profile-run", action="store_true", help="one untimed solve of the form without graphs")
+    ap.add_argument("--precond", default="sweep", choices=["sweep", "jacobi"],
+                    help="jacobi: the reference GPU solver's right-Jacobi BiCGSTAB (Krylov inner solves)")
+    ap.add_argument("--roofline-only", type=int, default=0, metavar="K",
+                    help="only time K outer iterations of the form's inner solve and print its roofline")
+    ap.add_argument("--block", type=int, default=0, choices=[0, 64, 128, 256, 512, 1024],
+                    help="work-group size of the inner solve (0: autotuned)")
+    ap.add_argument("--band-group", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="jacobi: bands per work-group of the batched BiCGSTAB (0: autotuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (the box has one); multi-GPU sharding is "
@@ -279,7 +321,11 @@ def main():
     say(f"case {args.config}: N_cell={case.N_cell} N_dir={case.N_dir} N_band={case.N_band} "
         f"(inputs {time.time() - t0:.1f}s)")
     prec = capi.PREC_MIXED if args.precision == "mixed" else capi.PREC_FP64
-    form = dict(precision=prec, max_iter=100000)
+    form = dict(precision=prec, max_iter=100000, precond=capi.PC_JACOBI if args.precond == "jacobi" else capi.PC_SWEEP)
+    if args.block:
+        form.update(block=args.block)
+    if args.band_group:
+        form.update(band_group=args.band_group)
     net = None
     if args.form in ("learned", "anderson"):
         form.update(accel=capi.ACCEL_ANDERSON, accel_memory=args.accel_memory)
@@ -291,6 +337,9 @@ def main():
         net.set_params(np.load(wpath))
         case.attach_gnn(net)
         form.update(learned=1)
+    if args.roofline_only:
+        print(json.dumps(roofline_only(case, form, args.roofline_only, prec)), flush=True)
+        return
     if args.profile_run:
         r = case.solve(use_graph=0, **form)
         say(f"profile run: {r['iters']} outer iterations, {r['solve_seconds']:.3f}s")

@@ -75,6 +75,10 @@ typedef struct bte_opts {
                              BTE_ACCEL_ANDERSON (Anderson mixing of the temperature field over the
                              last accel_memory iterates); default BTE_ACCEL_NONE */
     int accel_memory;     /* Anderson history length; default 20 */
+    int band_group;       /* BiCGSTAB with BTE_PC_JACOBI: bands of one direction solved together by
+                             one work-group, each operator row loaded once for all of them (each
+                             band keeps its own scalars and stopping test): 1 (one system per
+                             work-group), 2 or 4; 0 autotuned.  Other forms use 1; default 0 */
 } bte_opts;
 
 enum bte_accel { BTE_ACCEL_NONE = 0, BTE_ACCEL_ANDERSON = 1 };
@@ -109,7 +113,8 @@ typedef struct bte_stats {
     int max_levels;       /* deepest sweep schedule over directions */
     int lagged_edges;     /* upwind edges dropped from the sweep to break cycles */
     int block;            /* work-group size used */
-    int variant;          /* direct-sweep variant used (1..3), 0 for the Krylov paths */
+    int variant;          /* direct-sweep variant used (1..5), 0 for the Krylov paths */
+    int band_group;       /* Krylov paths: bands per work-group used (opts.band_group); 0 direct */
 } bte_stats;
 
 void bte_default_opts(bte_opts* opts);

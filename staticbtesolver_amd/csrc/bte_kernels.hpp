@@ -345,19 +345,95 @@ __device__ __forceinline__ R diag_of(const SysView<R>& sv, R dgc, int c) {
     return sv.vt * dgc + sv.wv * (R)sv.vol[c];
 }
 
-// y = A x (+ returns nothing; caller syncs)
-template <typename R, int BLOCK>
-__device__ __forceinline__ void spmv(const SysView<R>& sv, const R* x, R* __restrict__ y) {
-    for (int c = threadIdx.x; c < sv.N; c += BLOCK) {
-        R acc = diag_of(sv, sv.dg[c], c) * x[c];
-        R off = 0;
-#pragma unroll 3
-        for (int k = 0; k < sv.W; k++) {
-            const int j = sv.col[(size_t)k * sv.N + c];
-            if (j >= 0) off += sv.val[(size_t)k * sv.N + c] * x[j];
+// Streaming loops of the Krylov solver.  Thread t visits cells t, t + BLOCK, t + 2 BLOCK, ... in
+// that order (so per-thread partial sums are accumulated in a fixed order); full groups of U cells
+// are handed to the body at once, which issues all their loads before any arithmetic: U times the
+// bytes in flight per wave for the same occupancy.
+template <int K>
+struct Unroll {
+    static constexpr int value = K;
+};
+
+template <int BLOCK, int U, class F>
+__device__ __forceinline__ void for_cells(int N, F&& f) {
+    int c = threadIdx.x;
+    for (; c + (U - 1) * BLOCK < N; c += U * BLOCK) f(Unroll<U>{}, c);
+    for (; c < N; c += BLOCK) f(Unroll<1>{}, c);
+}
+
+// Off-diagonal part of (A x) for K cells c0 + k BLOCK: off[k] = sum_f a_f x_{j_f} (without the
+// v_b tau_b factor).  Operator slots beyond the system's width and missing upwind neighbours
+// (column -1) contribute nothing; the gather of a missing neighbour reads the cell itself, so no
+// load is conditional.
+template <typename R, int BLOCK, int K>
+__device__ __forceinline__ void offdiag_cells(const SysView<R>& sv, const R* x, int c0, R (&off)[K]) {
+    const int N = sv.N, W = sv.W;
+    int j[K][3];
+    R a[K][3];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int c = c0 + k * BLOCK;
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            j[k][f] = (f < W) ? sv.col[(size_t)f * N + c] : -1;
+            a[k][f] = (f < W) ? sv.val[(size_t)f * N + c] : (R)0;
         }
-        y[c] = acc + sv.vt * off;
     }
+    R g[K][3];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int f = 0; f < 3; f++) g[k][f] = x[j[k][f] >= 0 ? j[k][f] : c0 + k * BLOCK];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        R o = 0;
+#pragma unroll
+        for (int f = 0; f < 3; f++)
+            if (j[k][f] >= 0) o += a[k][f] * g[k][f];
+        off[k] = o;
+    }
+}
+
+// y = A x for K cells c0 + k BLOCK; acc(k, c, y_c) sees each result in order.  With the Jacobi
+// preconditioner x = z / diag, and the diagonal term diag_c x_c is taken as z_c (dz = z: one
+// stream instead of the diagonal's two and x_c; equal to diag_c x_c up to the rounding of the
+// division); otherwise (DZ false) it is formed from the operator.
+template <typename R, int BLOCK, int K, bool DZ, class Acc>
+__device__ __forceinline__ void spmv_cells(const SysView<R>& sv, const R* x, const R* dz, R* __restrict__ y, int c0,
+                                           Acc&& acc) {
+    R dt[K];
+    if constexpr (DZ) {
+#pragma unroll
+        for (int k = 0; k < K; k++) dt[k] = dz[c0 + k * BLOCK];
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int c = c0 + k * BLOCK;
+            dt[k] = (sv.vt * sv.dg[c] + sv.wv * (R)sv.vol[c]) * x[c];
+        }
+    }
+    R off[K];
+    offdiag_cells<R, BLOCK, K>(sv, x, c0, off);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int c = c0 + k * BLOCK;
+        const R yc = dt[k] + sv.vt * off[k];
+        y[c] = yc;
+        acc(k, c, yc);
+    }
+}
+
+// (A x)[c]
+template <typename R>
+__device__ __forceinline__ R spmv_row(const SysView<R>& sv, const R* x, int c) {
+    R acc = diag_of(sv, sv.dg[c], c) * x[c];
+    R off = 0;
+#pragma unroll 3
+    for (int k = 0; k < sv.W; k++) {
+        const int j = sv.col[(size_t)k * sv.N + c];
+        if (j >= 0) off += sv.val[(size_t)k * sv.N + c] * x[j];
+    }
+    return acc + sv.vt * off;
 }
 
 // y = M^-1 v: forward substitution in downwind level order (exact inverse when no edge is lagged).
@@ -379,19 +455,6 @@ __device__ __forceinline__ void sweep(const SysView<R>& sv, const R* v, R* y) {
     }
 }
 
-template <typename R, int BLOCK>
-__device__ __forceinline__ void jacobi(const SysView<R>& sv, const R* v, R* __restrict__ y) {
-    for (int c = threadIdx.x; c < sv.N; c += BLOCK) y[c] = v[c] / diag_of(sv, sv.dg[c], c);
-}
-
-template <typename R, int BLOCK, bool SWEEP>
-__device__ __forceinline__ void precond(const SysView<R>& sv, const R* v, R* y) {
-    if constexpr (SWEEP)
-        sweep<R, BLOCK>(sv, v, y);
-    else
-        jacobi<R, BLOCK>(sv, v, y);
-}
-
 // BiCGSTAB on A x = rhs for one system, right-preconditioned.  On entry r holds rhs; x is
 // overwritten.  Two variants:
 //  - reference semantics (Jacobi path): bicgstab.hpp:146-298 -- p starts as r, q = 0, the first
@@ -400,22 +463,39 @@ __device__ __forceinline__ void precond(const SysView<R>& sv, const R* v, R* y) 
 //  - standard form (sweep path): p = 0, v = 0 so the first step is p = r; an early exit when
 //    ||s|| / ||r0|| < tol (with the exact sweep this ends after one sweep and one product).
 // Returns the iterations used (maxit + 1 if the limit was hit without convergence).
+//
+// Five passes over the vectors per iteration, each fusing what needs no global reduction in
+// between: (1) p update, with the Jacobi preconditioner applied in the same pass; (2) v = A p^
+// with <r_hat, v>; (3) s = r - alpha v with ||s||^2, and s^ = s / diag (Jacobi); (4) t = A s^ with
+// <t, s> and <t, t>; (5) the x and r updates with ||r||^2 and <r_hat, r>.  The arithmetic of every
+// element and the order of every per-thread sum are those of the unfused sequence.
 template <typename R, int BLOCK, bool SWEEP>
 __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R* p, R* v, R* ph, R* sh, R* t,
                         int maxit, double tol, bool refsem, double* lds) {
+    // cells per group in the streaming passes and in the operator products (1024-wide work-groups
+    // get half: their 128-register budget would otherwise spill)
+    constexpr int US = (BLOCK >= 1024) ? 2 : 4;
+    constexpr int UM = (BLOCK >= 1024) ? 1 : 2;
     const int N = sv.N;
     double red[2];
     // setup: x = 0, r = rhs, r_hat = r, p = r (reference) or 0, v = 0
     red[0] = 0;
-    for (int c = threadIdx.x; c < N; c += BLOCK) {
-        const R bc = rhs[c];
-        x[c] = 0;
-        r[c] = bc;
-        rh[c] = bc;
-        p[c] = refsem ? bc : (R)0;
-        v[c] = 0;
-        red[0] += (double)bc * (double)bc;
-    }
+    for_cells<BLOCK, US>(N, [&](auto kc, int c0) {
+        constexpr int K = decltype(kc)::value;
+        R bc[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) bc[k] = rhs[c0 + k * BLOCK];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int c = c0 + k * BLOCK;
+            x[c] = 0;
+            r[c] = bc[k];
+            rh[c] = bc[k];
+            p[c] = refsem ? bc[k] : (R)0;
+            v[c] = 0;
+            red[0] += (double)bc[k] * (double)bc[k];
+        }
+    });
     block_sum<BLOCK, 1>(red, lds);
     const double nrm0 = sqrt(red[0]);
     double nrm = nrm0;
@@ -450,14 +530,45 @@ __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R
             continue;
         }
         const double beta = (rho / rhop) * (alpha / omega);
-        // p = (p - omega v) * beta + r
-        for (int c = threadIdx.x; c < N; c += BLOCK) p[c] = (p[c] - (R)omega * v[c]) * (R)beta + r[c];
+        // (1) p = (p - omega v) * beta + r; Jacobi: p^ = p / diag
+        for_cells<BLOCK, US>(N, [&](auto kc, int c0) {
+            constexpr int K = decltype(kc)::value;
+            R pv[K], vv[K], rv[K], dgv[K];
+            double vl[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                pv[k] = p[c];
+                vv[k] = v[c];
+                rv[k] = r[c];
+                if constexpr (!SWEEP) {
+                    dgv[k] = sv.dg[c];
+                    vl[k] = sv.vol[c];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                const R pn = (pv[k] - (R)omega * vv[k]) * (R)beta + rv[k];
+                p[c] = pn;
+                if constexpr (!SWEEP) ph[c] = pn / (sv.vt * dgv[k] + sv.wv * (R)vl[k]);
+            }
+        });
         __syncthreads();
-        precond<R, BLOCK, SWEEP>(sv, p, ph);
-        __syncthreads();
-        spmv<R, BLOCK>(sv, ph, v);
+        if constexpr (SWEEP) {
+            sweep<R, BLOCK>(sv, p, ph);
+            __syncthreads();
+        }
+        // (2) v = A p^, <r_hat, v>
         red[0] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK) red[0] += (double)rh[c] * (double)v[c];
+        for_cells<BLOCK, UM>(N, [&](auto kc, int c0) {
+            constexpr int K = decltype(kc)::value;
+            R rhv[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) rhv[k] = rh[c0 + k * BLOCK];
+            spmv_cells<R, BLOCK, K, !SWEEP>(sv, ph, p, v, c0,
+                                            [&](int k, int, R vc) { red[0] += (double)rhv[k] * (double)vc; });
+        });
         block_sum<BLOCK, 1>(red, lds);
         const double rv = red[0];
         if (rv < 1e-200 && rv > -1e-200) {
@@ -479,13 +590,31 @@ __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R
             continue;
         }
         alpha = rho / rv;
-        // s = r - alpha v (stored in r)
+        // (3) s = r - alpha v (stored in r), ||s||^2; Jacobi: s^ = s / diag
         red[0] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK) {
-            const R s = r[c] - (R)alpha * v[c];
-            r[c] = s;
-            red[0] += (double)s * (double)s;
-        }
+        for_cells<BLOCK, US>(N, [&](auto kc, int c0) {
+            constexpr int K = decltype(kc)::value;
+            R rv[K], vv[K], dgv[K];
+            double vl[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                rv[k] = r[c];
+                vv[k] = v[c];
+                if constexpr (!SWEEP) {
+                    dgv[k] = sv.dg[c];
+                    vl[k] = sv.vol[c];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                const R s = rv[k] - (R)alpha * vv[k];
+                r[c] = s;
+                if constexpr (!SWEEP) sh[c] = s / (sv.vt * dgv[k] + sv.wv * (R)vl[k]);
+                red[0] += (double)s * (double)s;
+            }
+        });
         block_sum<BLOCK, 1>(red, lds);
         if (!refsem && sqrt(red[0]) < tol * nrm0) {
             for (int c = threadIdx.x; c < N; c += BLOCK) x[c] += (R)alpha * ph[c];
@@ -493,26 +622,51 @@ __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R
             converged = true;
             break;
         }
-        precond<R, BLOCK, SWEEP>(sv, r, sh);
-        __syncthreads();
-        spmv<R, BLOCK>(sv, sh, t);
+        if constexpr (SWEEP) {
+            sweep<R, BLOCK>(sv, r, sh);
+            __syncthreads();
+        }
+        // (4) t = A s^, <t, s>, <t, t>
         red[0] = 0;
         red[1] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK) {
-            red[0] += (double)t[c] * (double)r[c];
-            red[1] += (double)t[c] * (double)t[c];
-        }
+        for_cells<BLOCK, UM>(N, [&](auto kc, int c0) {
+            constexpr int K = decltype(kc)::value;
+            R sv_[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) sv_[k] = r[c0 + k * BLOCK];
+            spmv_cells<R, BLOCK, K, !SWEEP>(sv, sh, r, t, c0, [&](int k, int, R tc) {
+                red[0] += (double)tc * (double)sv_[k];
+                red[1] += (double)tc * (double)tc;
+            });
+        });
         block_sum<BLOCK, 2>(red, lds);
         omega = (red[1] != 0) ? red[0] / red[1] : 0.0;
+        // (5) x += alpha p^ + omega s^, r = s - omega t, ||r||^2, <r_hat, r>
         red[0] = 0;
         red[1] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK) {
-            x[c] += (R)alpha * ph[c] + (R)omega * sh[c];
-            const R rn = r[c] - (R)omega * t[c];
-            r[c] = rn;
-            red[0] += (double)rn * (double)rn;
-            red[1] += (double)rh[c] * (double)rn;
-        }
+        for_cells<BLOCK, US>(N, [&](auto kc, int c0) {
+            constexpr int K = decltype(kc)::value;
+            R xv[K], phv[K], shv[K], rv[K], tv[K], rhv[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                xv[k] = x[c];
+                phv[k] = ph[c];
+                shv[k] = sh[c];
+                rv[k] = r[c];
+                tv[k] = t[c];
+                rhv[k] = rh[c];
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                x[c] = xv[k] + ((R)alpha * phv[k] + (R)omega * shv[k]);
+                const R rn = rv[k] - (R)omega * tv[k];
+                r[c] = rn;
+                red[0] += (double)rn * (double)rn;
+                red[1] += (double)rhv[k] * (double)rn;
+            }
+        });
         block_sum<BLOCK, 2>(red, lds);
         nrm = sqrt(red[0]);
         rho_next = red[1];
@@ -954,17 +1108,14 @@ __global__ void k_gather_scalars(int n, const double* const* __restrict__ ptr, d
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) out[k] = *ptr[k];
 }
 
-// FP64 path: RHS formed in-kernel, then BiCGSTAB in FP64 writes ee_curr directly.
+// FP64 path: BiCGSTAB in FP64 on the right-hand sides in w.b (k_rhs) writes ee_curr directly.
 template <int BLOCK, bool SWEEP>
-__global__ __launch_bounds__(BLOCK) void k_solve64(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w, SolveCfg cfg,
-                                                  const double* __restrict__ ee_prev, const double* __restrict__ dens) {
+__global__ __launch_bounds__(BLOCK) void k_solve64(Ops o, Work w, SolveCfg cfg) {
     __shared__ double lds[2 * (BLOCK / 64)];
     for (int s = blockIdx.x; s < cfg.S; s += gridDim.x) {
         const int b = s / o.D, d = s % o.D;
         const size_t base = (size_t)s * o.N;
-        double* rhs = w.b + base;
-        for (int c = threadIdx.x; c < o.N; c += BLOCK) rhs[c] = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
-        __syncthreads();
+        const double* rhs = w.b + base;
         const SysView<double> sv = make_view<double>(o, b, d);
         const int it = bicgstab<double, BLOCK, SWEEP>(
             sv, rhs, w.x + base, (double*)w.r + base, (double*)w.rh + base, (double*)w.p + base,
@@ -981,20 +1132,17 @@ __global__ __launch_bounds__(BLOCK) void k_solve64(Geo g, Quad q, Bnd bd, Bands 
 // Mixed precision: FP64 RHS and residual, FP32 BiCGSTAB on the scaled residual equation,
 // x += ||r|| * y per refinement step, until ||b - A x|| / ||b|| < tol (FP64, on the GPU).
 template <int BLOCK, bool SWEEP>
-__global__ __launch_bounds__(BLOCK) void k_solve_mixed(Geo g, Quad q, Bnd bd, Bands bands, Ops o, Work w,
-                                                      SolveCfg cfg, const double* __restrict__ ee_prev,
-                                                      const double* __restrict__ dens) {
+__global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg cfg) {
     __shared__ double lds[2 * (BLOCK / 64)];
     for (int s = blockIdx.x; s < cfg.S; s += gridDim.x) {
         const int b = s / o.D, d = s % o.D;
         const size_t base = (size_t)s * o.N;
-        double* rhs = w.b + base;
+        const double* rhs = w.b + base;
         double* x = w.x + base;
         double* r64 = w.r64 + base;
         double red[1] = {0};
         for (int c = threadIdx.x; c < o.N; c += BLOCK) {
-            const double bc = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
-            rhs[c] = bc;
+            const double bc = rhs[c];
             r64[c] = bc;
             x[c] = 0;
             red[0] += bc * bc;

@@ -15,6 +15,7 @@
 #include "../../include/bte.h"
 #include "bte_internal.hpp"
 #include "bte_kernels.hpp"
+#include "bte_bands.hpp"
 
 using namespace bte;
 
@@ -70,9 +71,9 @@ struct bte_ctx {
     long long calls[KA_COUNT] = {0};
     double spmv_bytes = 0;
     long long spmv_rows = 0;
-    int block_used = 0, variant_used = 0;
+    int block_used = 0, variant_used = 0, group_used = 0;
     // autotuned launch shape per solver form: work-group size and sweep variant (0 = not tuned)
-    int tuned_block[2][2][2] = {};    // [precision][precond][direct]
+    int tuned_block[2][2][2][4] = {};  // [precision][precond][direct][requested band group: auto, 1, 2, 4]
     // iteration state between bte_begin and bte_finish
     bte_opts run_opts{};
     int run_shape_block = 256, run_shape_variant = 2;
@@ -100,7 +101,7 @@ struct bte_ctx {
     double learn_Tmid = 0, learn_span = 1, learn_ref_ff = 0, learn_ratio = 0;
     int learn_streak = 0;       // consecutive rejections
     int run_rejected = 0;
-    int tuned_variant[2][2][2] = {};  // sweep variant for the direct form
+    int tuned_variant[2][2][2][4] = {};  // sweep variant (direct form) or band group (Krylov forms)
 };
 
 #define HIPCHK(expr)                                                                              \
@@ -303,6 +304,7 @@ void bte_default_opts(bte_opts* o) {
     o->sweep_variant = 0;
     o->accel = BTE_ACCEL_NONE;
     o->accel_memory = 20;
+    o->band_group = 0;
 }
 
 int bte_ctx_create(int device, bte_ctx** out) {
@@ -692,8 +694,23 @@ Work work_of(bte_ctx* ctx, double* ee_curr) {
 // 4 level-ordered operands + LDS ring of recent intensities + pipelined operand loads).
 struct Shape {
     int block = 256;
-    int variant = 2;
+    int variant = 2;  // direct form: sweep variant; Krylov forms: bands per work-group (1, 2, 4)
 };
+
+// Bands per work-group a Krylov form uses when it is not autotuned: opts.band_group if it is a
+// supported group of the Jacobi path, else one system per work-group.
+int krylov_group(const bte_opts* o) {
+    if (o->precond != BTE_PC_JACOBI) return 1;
+    return (o->band_group == 2 || o->band_group == 4) ? o->band_group : 1;
+}
+
+// Whether the band-batched kernels run G bands per work-group of `block` threads: the Jacobi path
+// only, G in {2, 4}, work-groups of at most 512 (wider ones would spill), and G bands of D*N FP64
+// elements below 4 GiB (the kernels address a group's vectors through 32-bit buffer offsets).
+bool group_ok(const bte_ctx* ctx, const bte_opts* o, int G, int block) {
+    return o->precond == BTE_PC_JACOBI && (G == 2 || G == 4) && block <= 512 &&
+           (double)G * ctx->D * ctx->N * sizeof(double) < 4294967295.0;
+}
 
 bool direct_path(const bte_ctx* ctx, const bte_opts* o) {
     return o->precision == BTE_PREC_FP64 && o->precond == BTE_PC_SWEEP && o->direct_sweep && ctx->lagged == 0;
@@ -789,19 +806,38 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
         }
         return;
     }
+    // Krylov forms: the right-hand sides of all systems first (keeps the face loop's registers out
+    // of the solve kernel, whose occupancy sets its bandwidth)
+    hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)S * ctx->N, 256)), dim3(256), 0, ctx->stream, g, q, bd, bn,
+                       ee_prev, dens, w.b);
+    const int G = group_ok(ctx, o, variant, BLOCK) ? variant : 1;
+    if constexpr (BLOCK <= 512) if (G > 1) {
+        // band-batched: one work-group per (direction, group of G bands)
+        const dim3 gg(ctx->D * ((ctx->B + G - 1) / G));
+        const bool mixed = o->precision == BTE_PREC_MIXED;
+        if (G == 2) {
+            if (mixed)
+                hipLaunchKernelGGL((k_solve_mixed_bands<BLOCK, 2>), gg, blk, 0, ctx->stream, op, w, cfg);
+            else
+                hipLaunchKernelGGL((k_solve64_bands<BLOCK, 2>), gg, blk, 0, ctx->stream, op, w, cfg);
+        } else {
+            if (mixed)
+                hipLaunchKernelGGL((k_solve_mixed_bands<BLOCK, 4>), gg, blk, 0, ctx->stream, op, w, cfg);
+            else
+                hipLaunchKernelGGL((k_solve64_bands<BLOCK, 4>), gg, blk, 0, ctx->stream, op, w, cfg);
+        }
+        return;
+    }
     if (o->precision == BTE_PREC_MIXED) {
         if (o->precond == BTE_PC_JACOBI)
-            hipLaunchKernelGGL((k_solve_mixed<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
-                               dens);
+            hipLaunchKernelGGL((k_solve_mixed<BLOCK, false>), grid, blk, 0, ctx->stream, op, w, cfg);
         else
-            hipLaunchKernelGGL((k_solve_mixed<BLOCK, true>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
-                               dens);
+            hipLaunchKernelGGL((k_solve_mixed<BLOCK, true>), grid, blk, 0, ctx->stream, op, w, cfg);
     } else {
         if (o->precond == BTE_PC_JACOBI)
-            hipLaunchKernelGGL((k_solve64<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev,
-                               dens);
+            hipLaunchKernelGGL((k_solve64<BLOCK, false>), grid, blk, 0, ctx->stream, op, w, cfg);
         else
-            hipLaunchKernelGGL((k_solve64<BLOCK, true>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, cfg, ee_prev, dens);
+            hipLaunchKernelGGL((k_solve64<BLOCK, true>), grid, blk, 0, ctx->stream, op, w, cfg);
     }
 }
 
@@ -855,16 +891,26 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
 int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
     const bool direct = direct_path(ctx, o);
     std::vector<Shape> cand;
-    for (int blk : {64, 128, 256, 512}) {
+    for (int blk : {64, 128, 256, 512, 1024}) {
         if (direct) {
+            if (blk == 1024) continue;
             cand.push_back({blk, 2});
             if (lds_fits(ctx)) cand.push_back({blk, 3});
             if (ring_size(ctx) > 0) cand.push_back({blk, 4});
             if (blk == 64 && ring_size(ctx) > 0) cand.push_back({64, 5});  // one wave per system
-        } else {
-            cand.push_back({blk, 0});
+        } else if (blk >= 128) {
+            for (int G : {1, 2, 4}) {
+                const bool allowed = (o->band_group == 0) ? (G == 1 || (G <= ctx->B && group_ok(ctx, o, G, blk)))
+                                                          : G == krylov_group(o);
+                if (allowed) cand.push_back({blk, G});
+            }
         }
     }
+    // Krylov forms are timed on a fixed number of iterations per system: the rate per iteration is
+    // what the shape changes, and a full solve from the zero state can take seconds per candidate.
+    bte_opts ot = *o;
+    if (!direct) ot.inner_maxit = std::min(o->inner_maxit, 12);
+    o = &ot;
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)ctx->B * ctx->N, ctx->stream));
@@ -901,17 +947,18 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
 // that state is initialised afterwards (bte_solve); elsewhere an untuned form uses {256, 2}.
 int pick_shape(bte_ctx* ctx, const bte_opts* o, Shape& sh, bool allow_tune) {
     const bool direct = direct_path(ctx, o);
-    int& tb = ctx->tuned_block[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct];
-    int& tv = ctx->tuned_variant[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct];
+    const int gkey = direct ? 0 : (o->band_group == 1 ? 1 : o->band_group == 2 ? 2 : o->band_group == 4 ? 3 : 0);
+    int& tb = ctx->tuned_block[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct][gkey];
+    int& tv = ctx->tuned_variant[o->precision == BTE_PREC_MIXED][o->precond == BTE_PC_JACOBI][direct][gkey];
     const bool fixed_block = o->block == 64 || o->block == 128 || o->block == 256 || o->block == 512 || o->block == 1024;
     if (fixed_block && (!direct || o->sweep_variant != 0)) {
         sh.block = o->block;
-        sh.variant = direct ? o->sweep_variant : 0;
+        sh.variant = direct ? o->sweep_variant : krylov_group(o);
         return BTE_OK;
     }
     if (!tb && !allow_tune) {
         sh.block = fixed_block ? o->block : 256;
-        sh.variant = direct ? (o->sweep_variant ? o->sweep_variant : 2) : 0;
+        sh.variant = direct ? (o->sweep_variant ? o->sweep_variant : 2) : krylov_group(o);
         return BTE_OK;
     }
     if (!tb) {
@@ -1371,6 +1418,9 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
                         : (shape.variant == 3 && lds_fits(ctx))           ? 3
                         : (shape.variant == 1)                            ? 1
                                                                           : 2;
+    ctx->group_used = direct_path(ctx, &o) ? 0
+                      : group_ok(ctx, &o, shape.variant, shape.block) ? shape.variant
+                                                                       : 1;
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
@@ -1675,6 +1725,7 @@ int bte_get_stats(bte_ctx* ctx, bte_stats* st) {
     st->lagged_edges = ctx->lagged;
     st->block = ctx->block_used;
     st->variant = ctx->variant_used;
+    st->band_group = ctx->group_used;
     return BTE_OK;
 }
 
