@@ -200,11 +200,18 @@ def cpu_baseline(kw, outer_iters, budget_s=15.0):
 
 
 def roofline_only(case, form, k, prec):
-    """K outer iterations with per-kernel hipEvents; roofline of the inner solve.  Algorithmic bytes
-    per launch: 2 B_spmv per system for an exact sweep, and per BiCGSTAB iteration the fused lower
-    bound 2 B_spmv + 21 w n per active system, times the mean iterations per system.  With G bands
-    per work-group the operator (columns, values, row pointers) is read once per group: B_spmv's
-    operator part counts 1/G per system (B_shared), its two vectors 2 w n in full."""
+    """K outer iterations with per-kernel hipEvents; roofline of the inner solve (all kernels of
+    the solve step: right-hand sides, inner solves and, in the mixed form, FP64 re-solves).
+
+    Algorithmic bytes per launch.  The stored operator of a direction is shared by its bands (the
+    band enters as the scalar v_b tau_b), so a product of all systems needs the operator of each
+    direction once: B_shared = sum_d [nnz_d (w+4) + 4 (n+1)] + sum_(b,d) 2 w n.  Per BiCGSTAB
+    iteration of all systems: 2 B_shared + 21 w n per system (the fused vector passes' lower
+    bound) + the per-band diagonal reads of the Jacobi preconditioner, 2 (w + 8) n per system;
+    times the mean iterations per system (FP64 re-solve iterations of the mixed form counted at
+    w = 4: a lower bound).  For an exact sweep: 2 B_spmv per system.  achieved_canonical charges
+    every system its own operator (2 B_spmv + 21 w n per iteration); with the operator shared
+    through the caches it can exceed the HBM traffic actually moved (rocprofv3 counters)."""
     from staticbtesolver_amd import capi
 
     f = dict(form, max_iter=k, stop_margin=0.0, timing=1)
@@ -218,16 +225,22 @@ def roofline_only(case, form, k, prec):
     direct = st["variant"] > 0
     G = max(1, st.get("band_group", 1))
     n = case.N_cell
+    canon = spmv_bytes(case, w)           # sum over systems of B_spmv
+    vec = 2.0 * w * n * S                 # the products' x and y of every system
+    op_dir = canon / case.N_band - 2.0 * w * n * case.N_dir  # the operator of every direction once
     if direct:
-        b_alg = 2.0 * spmv_bytes(case, w)
+        b_alg = b_canon = 2.0 * canon
     else:
-        vec = 2.0 * w * n * S  # the product's x and y of every system
-        op = spmv_bytes(case, w) - vec
-        b_alg = (2.0 * (op / G + vec) + 21.0 * w * n * S) * per_sys
+        jac = 2.0 * (w + 8) * n * S if form.get("precond") == capi.PC_JACOBI else 0.0
+        b_alg = (2.0 * (op_dir + vec) + 21.0 * w * n * S + jac) * per_sys
+        b_canon = (2.0 * canon + 21.0 * w * n * S) * per_sys
     achieved = b_alg / (solve_ms * 1e-3) / 1e9
     return {"roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel_ms": round(solve_ms, 4),
-                         "alg_bytes_per_launch": b_alg, "inner_iterations_per_system": round(per_sys, 2),
+                         "alg_bytes_per_launch": b_alg,
+                         "achieved_canonical": round(b_canon / (solve_ms * 1e-3) / 1e9, 2),
+                         "inner_iterations_per_system": round(per_sys, 2),
+                         "inner_limit_hits": r["inner_failures"],
                          "form": "direct sweep" if direct else "BiCGSTAB"},
             "config": {"n_cell": case.N_cell, "systems": S, "outer_iterations": r["iters"],
                        "sweep_variant": st["variant"], "block": st["block"], "band_group": G}}
@@ -306,6 +319,7 @@ def main():
                     help="work-group size of the inner solve (0: autotuned)")
     ap.add_argument("--band-group", type=int, default=0, choices=[0, 1, 2, 4],
                     help="jacobi: bands per work-group of the batched BiCGSTAB (0: autotuned)")
+    ap.add_argument("--tol32", type=float, default=0.0, help="mixed: FP32 inner tolerance (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU (the box has one); multi-GPU sharding is "
@@ -326,6 +340,8 @@ def main():
         form.update(block=args.block)
     if args.band_group:
         form.update(band_group=args.band_group)
+    if args.tol32:
+        form.update(inner_tol32=args.tol32)
     net = None
     if args.form in ("learned", "anderson"):
         form.update(accel=capi.ACCEL_ANDERSON, accel_memory=args.accel_memory)

@@ -75,10 +75,12 @@ typedef struct bte_opts {
                              BTE_ACCEL_ANDERSON (Anderson mixing of the temperature field over the
                              last accel_memory iterates); default BTE_ACCEL_NONE */
     int accel_memory;     /* Anderson history length; default 20 */
-    int band_group;       /* BiCGSTAB with BTE_PC_JACOBI: bands of one direction solved together by
+    int band_group;       /* FP64 BiCGSTAB with BTE_PC_JACOBI: bands of one direction solved together by
                              one work-group, each operator row loaded once for all of them (each
                              band keeps its own scalars and stopping test): 1 (one system per
                              work-group), 2 or 4; 0 autotuned.  Other forms use 1; default 0 */
+    double inner_tol32;   /* BTE_PREC_MIXED: relative residual each FP32 inner solve aims for before
+                             the FP64 refinement step (near FP32 rounding it can stall); 1e-5 */
 } bte_opts;
 
 enum bte_accel { BTE_ACCEL_NONE = 0, BTE_ACCEL_ANDERSON = 1 };
@@ -95,7 +97,9 @@ typedef struct bte_result {
     int iterations;           /* outer iterations run */
     int converged;            /* 1 if the last margin <= stop_margin */
     long long inner_iters;    /* total inner BiCGSTAB iterations over all systems */
-    int inner_failures;       /* systems that hit inner_maxit (summed over outer iterations) */
+    int inner_failures;       /* systems that hit inner_maxit (summed over outer iterations); in
+                                 BTE_PREC_MIXED, systems whose FP64 re-solve (after an FP32 inner
+                                 solve stalled) hit it */
     double solve_seconds;     /* wall time of the outer loop (like StaticBTESolver.cpp:865-876) */
     double setup_seconds;     /* assembly + sweep schedule + uploads */
     int learned_accepted;     /* learned corrections accepted / proposed */

@@ -45,7 +45,8 @@ class Opts(C.Structure):
                 ("refine_steps", C.c_int), ("block", C.c_int), ("use_graph", C.c_int), ("verbose", C.c_int),
                 ("timing", C.c_int), ("learned", C.c_int), ("learned_every", C.c_int),
                 ("direct_sweep", C.c_int), ("sweep_variant", C.c_int), ("accel", C.c_int),
-                ("accel_memory", C.c_int), ("band_group", C.c_int)]
+                ("accel_memory", C.c_int), ("band_group", C.c_int),
+                ("inner_tol32", C.c_double)]
 
 
 class Result(C.Structure):

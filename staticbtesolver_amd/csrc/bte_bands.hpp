@@ -123,38 +123,84 @@ struct KrylovVecs {
     GVec<R> rhs, x, r, rh, p, v, ph, sh, t;
 };
 
-// Off-diagonal sums of the G bands' products at cell c: off[g] = sum_f a_f x_g[j_f], the row
-// loaded once.  Slots beyond the width and missing neighbours (column -1) contribute nothing; the
-// gather of a missing neighbour reads the cell itself, so no load is conditional.
-template <typename R, int G>
-__device__ __forceinline__ void group_offdiag(const BandGroup<R, G>& bg, unsigned m, const GVec<R>& x, int c,
-                                              R (&off)[G]) {
+// Off-diagonal sums of the G bands' products at the K cells c0 + k BLOCK: off[k][g] = sum_f a_f
+// x_g[j_f], each row loaded once.  Slots beyond the width and missing neighbours (column -1)
+// contribute nothing; the gather of a missing neighbour reads the cell itself, so no load is
+// conditional.
+template <typename R, int G, int BLOCK, int K>
+__device__ __forceinline__ void group_offdiag(const BandGroup<R, G>& bg, unsigned m, const GVec<R>& x, int c0,
+                                              R (&off)[K][G]) {
     const int W = bg.W;
-    int j[3];
-    R a[3];
+    int j[K][3];
+    R a[K][3];
 #pragma unroll
-    for (int f = 0; f < 3; f++) {
-        const unsigned fo = (unsigned)f * (unsigned)bg.N;
-        j[f] = (f < W) ? buf_ld<int>(bg.col, (unsigned)c * 4u, fo * 4u) : -1;
-        a[f] = (f < W) ? buf_ld<R>(bg.val, (unsigned)c * sizeof(R), fo * sizeof(R)) : (R)0;
+    for (int k = 0; k < K; k++) {
+        const unsigned c = (unsigned)(c0 + k * BLOCK);
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const unsigned fo = (unsigned)f * (unsigned)bg.N;
+            j[k][f] = (f < W) ? buf_ld<int>(bg.col, c * 4u, fo * 4u) : -1;
+            a[k][f] = (f < W) ? buf_ld<R>(bg.val, c * (unsigned)sizeof(R), fo * (unsigned)sizeof(R)) : (R)0;
+        }
     }
-    R gx[G][3];
-    each_band<G>(m, [&](int g) {
+    R gx[K][G][3];
 #pragma unroll
-        for (int f = 0; f < 3; f++) gx[g][f] = x.ld(j[f] >= 0 ? j[f] : c, bg.so[g]);
-    });
-    each_band<G>(m, [&](int g) {
-        R o = 0;
+    for (int k = 0; k < K; k++)
+        each_band<G>(m, [&](int g) {
 #pragma unroll
-        for (int f = 0; f < 3; f++)
-            if (j[f] >= 0) o += a[f] * gx[g][f];
-        off[g] = o;
-    });
+            for (int f = 0; f < 3; f++) gx[k][g][f] = x.ld(j[k][f] >= 0 ? j[k][f] : c0 + k * BLOCK, bg.so[g]);
+        });
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        each_band<G>(m, [&](int g) {
+            R o = 0;
+#pragma unroll
+            for (int f = 0; f < 3; f++)
+                if (j[k][f] >= 0) o += a[k][f] * gx[k][g][f];
+            off[k][g] = o;
+        });
 }
 
 template <typename R, int G>
 __device__ __forceinline__ R group_diag(const BandGroup<R, G>& bg, int g, R dgc, double vl) {
     return bg.vt[g] * dgc + bg.wv * (R)vl;
+}
+
+// Cells per thread and step of the band-batched passes: about 8 (FP64: 4) values per streamed
+// array in flight per lane, over cells and bands together.
+template <typename R, int G>
+constexpr int band_unroll() {
+    return ((sizeof(R) == 4 ? 8 : 4) / G) > 0 ? ((sizeof(R) == 4 ? 8 : 4) / G) : 1;
+}
+
+template <unsigned M>
+struct MaskC {
+    static constexpr unsigned value = M;
+};
+
+// for_cells over the bands in m: the loop is instantiated for the full mask (and, for G = 2, for
+// each single band) with the mask a compile-time constant, so the per-band tests fold away and the
+// loads of all bands and cells of a step issue back to back; other masks run a generic copy whose
+// per-band branches split the loads.  body(kc, c0, mc): mc::value is the mask, 0 in the generic copy.
+template <int BLOCK, int U, int G, class F>
+__device__ __forceinline__ void band_cells(int N, unsigned m, F&& body) {
+    constexpr unsigned full = (1u << G) - 1;
+    auto go = [&](auto mc) { for_cells<BLOCK, U>(N, [&](auto kc, int c0) { body(kc, c0, mc); }); };
+    if (m == full) {
+        go(MaskC<full>{});
+        return;
+    }
+    if constexpr (G == 2) {
+        if (m == 1) {
+            go(MaskC<1>{});
+            return;
+        }
+        if (m == 2) {
+            go(MaskC<2>{});
+            return;
+        }
+    }
+    go(MaskC<0>{});
 }
 
 // Per band, its[g] = iterations used (maxit + 1: limit hit) for the bands in act.
@@ -167,6 +213,7 @@ __device__ __forceinline__ R group_diag(const BandGroup<R, G>& bg, int g, R dgc,
 template <typename R, int BLOCK, int G>
 __device__ void bicgstab_bands(const BandGroup<R, G>& bg, unsigned act, const KrylovVecs<R>& V, int maxit,
                                double tol, bool refsem, double* lds, double* st, int (&its)[G]) {
+    constexpr int U = band_unroll<R, G>();
     const int N = bg.N;
     double* const s_rho = st;            // [2][G] by iteration parity
     double* const s_alpha = st + 2 * G;  // [G]
@@ -178,19 +225,25 @@ __device__ void bicgstab_bands(const BandGroup<R, G>& bg, unsigned act, const Kr
     // setup: x = 0, r = rhs, r_hat = r, p = r (reference) or 0, v = 0
 #pragma unroll
     for (int g = 0; g < G; g++) red[g] = 0;
-    for (int c = threadIdx.x; c < N; c += BLOCK) {
-        R bc[G];
-        each_band<G>(act, [&](int g) { bc[g] = V.rhs.ld(c, bg.so[g]); });
-        each_band<G>(act, [&](int g) {
-            const unsigned so = bg.so[g];
-            V.x.st(c, so, (R)0);
-            V.r.st(c, so, bc[g]);
-            V.rh.st(c, so, bc[g]);
-            V.p.st(c, so, refsem ? bc[g] : (R)0);
-            V.v.st(c, so, (R)0);
-            red[g] += (double)bc[g] * (double)bc[g];
-        });
-    }
+    band_cells<BLOCK, U, G>(N, act, [&](auto kc, int c0, auto mc) {
+        const unsigned msk = decltype(mc)::value ? decltype(mc)::value : act;
+        constexpr int K = decltype(kc)::value;
+        R bc[K][G];
+#pragma unroll
+        for (int k = 0; k < K; k++) each_band<G>(msk, [&](int g) { bc[k][g] = V.rhs.ld(c0 + k * BLOCK, bg.so[g]); });
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            each_band<G>(msk, [&](int g) {
+                const int c = c0 + k * BLOCK;
+                const unsigned so = bg.so[g];
+                V.x.st(c, so, (R)0);
+                V.r.st(c, so, bc[k][g]);
+                V.rh.st(c, so, bc[k][g]);
+                V.p.st(c, so, refsem ? bc[k][g] : (R)0);
+                V.v.st(c, so, (R)0);
+                red[g] += (double)bc[k][g] * (double)bc[k][g];
+            });
+    });
     block_sum<BLOCK, G>(red, lds);
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -250,38 +303,54 @@ __device__ void bicgstab_bands(const BandGroup<R, G>& bg, unsigned act, const Kr
         if (rs) restart(rs, cur);
         if (!wm) continue;
         // (1) p = (p - omega v) * beta + r, p^ = p / diag
-        for (int c = threadIdx.x; c < N; c += BLOCK) {
-            const R dgc = buf_ld<R>(bg.dg, (unsigned)c * sizeof(R), 0);
-            const double vl = buf_ld<double>(bg.vol, (unsigned)c * 8u, 0);
-            R pv[G], vv[G], rv[G];
-            each_band<G>(wm, [&](int g) {
-                pv[g] = V.p.ld(c, bg.so[g]);
-                vv[g] = V.v.ld(c, bg.so[g]);
-                rv[g] = V.r.ld(c, bg.so[g]);
-            });
-            each_band<G>(wm, [&](int g) {
-                const R pn = (pv[g] - om[g] * vv[g]) * be[g] + rv[g];
-                V.p.st(c, bg.so[g], pn);
-                V.ph.st(c, bg.so[g], pn / group_diag(bg, g, dgc, vl));
-            });
-        }
+        band_cells<BLOCK, U, G>(N, wm, [&](auto kc, int c0, auto mc) {
+            const unsigned msk = decltype(mc)::value ? decltype(mc)::value : wm;
+            constexpr int K = decltype(kc)::value;
+            R dgc[K], pv[K][G], vv[K][G], rv[K][G];
+            double vl[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int c = c0 + k * BLOCK;
+                dgc[k] = buf_ld<R>(bg.dg, (unsigned)c * sizeof(R), 0);
+                vl[k] = buf_ld<double>(bg.vol, (unsigned)c * 8u, 0);
+                each_band<G>(msk, [&](int g) {
+                    pv[k][g] = V.p.ld(c, bg.so[g]);
+                    vv[k][g] = V.v.ld(c, bg.so[g]);
+                    rv[k][g] = V.r.ld(c, bg.so[g]);
+                });
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                each_band<G>(msk, [&](int g) {
+                    const int c = c0 + k * BLOCK;
+                    const R pn = (pv[k][g] - om[g] * vv[k][g]) * be[g] + rv[k][g];
+                    V.p.st(c, bg.so[g], pn);
+                    V.ph.st(c, bg.so[g], pn / group_diag(bg, g, dgc[k], vl[k]));
+                });
+        });
         __syncthreads();
         // (2) v = A p^ (diagonal term p), <r_hat, v>
 #pragma unroll
         for (int g = 0; g < G; g++) red[g] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK) {
-            R dz[G], rhv[G], off[G];
-            each_band<G>(wm, [&](int g) {
-                dz[g] = V.p.ld(c, bg.so[g]);
-                rhv[g] = V.rh.ld(c, bg.so[g]);
-            });
-            group_offdiag<R, G>(bg, wm, V.ph, c, off);
-            each_band<G>(wm, [&](int g) {
-                const R vc = dz[g] + bg.vt[g] * off[g];
-                V.v.st(c, bg.so[g], vc);
-                red[g] += (double)rhv[g] * (double)vc;
-            });
-        }
+        band_cells<BLOCK, U, G>(N, wm, [&](auto kc, int c0, auto mc) {
+            const unsigned msk = decltype(mc)::value ? decltype(mc)::value : wm;
+            constexpr int K = decltype(kc)::value;
+            R dz[K][G], rhv[K][G], off[K][G];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                each_band<G>(msk, [&](int g) {
+                    dz[k][g] = V.p.ld(c0 + k * BLOCK, bg.so[g]);
+                    rhv[k][g] = V.rh.ld(c0 + k * BLOCK, bg.so[g]);
+                });
+            group_offdiag<R, G, BLOCK, K>(bg, msk, V.ph, c0, off);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                each_band<G>(msk, [&](int g) {
+                    const R vc = dz[k][g] + bg.vt[g] * off[k][g];
+                    V.v.st(c0 + k * BLOCK, bg.so[g], vc);
+                    red[g] += (double)rhv[k][g] * (double)vc;
+                });
+        });
         block_sum<BLOCK, G>(red, lds);
         rs = 0;
         each_band<G>(wm, [&](int g) {
@@ -300,21 +369,31 @@ __device__ void bicgstab_bands(const BandGroup<R, G>& bg, unsigned act, const Kr
             each_band<G>(wm, [&](int g) { al[g] = (R)s_alpha[g]; });
 #pragma unroll
             for (int g = 0; g < G; g++) red[g] = 0;
-            for (int c = threadIdx.x; c < N; c += BLOCK) {
-                const R dgc = buf_ld<R>(bg.dg, (unsigned)c * sizeof(R), 0);
-                const double vl = buf_ld<double>(bg.vol, (unsigned)c * 8u, 0);
-                R rv[G], vv[G];
-                each_band<G>(wm, [&](int g) {
-                    rv[g] = V.r.ld(c, bg.so[g]);
-                    vv[g] = V.v.ld(c, bg.so[g]);
-                });
-                each_band<G>(wm, [&](int g) {
-                    const R s = rv[g] - al[g] * vv[g];
-                    V.r.st(c, bg.so[g], s);
-                    V.sh.st(c, bg.so[g], s / group_diag(bg, g, dgc, vl));
-                    red[g] += (double)s * (double)s;
-                });
-            }
+            band_cells<BLOCK, U, G>(N, wm, [&](auto kc, int c0, auto mc) {
+                const unsigned msk = decltype(mc)::value ? decltype(mc)::value : wm;
+                constexpr int K = decltype(kc)::value;
+                R dgc[K], rv[K][G], vv[K][G];
+                double vl[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int c = c0 + k * BLOCK;
+                    dgc[k] = buf_ld<R>(bg.dg, (unsigned)c * sizeof(R), 0);
+                    vl[k] = buf_ld<double>(bg.vol, (unsigned)c * 8u, 0);
+                    each_band<G>(msk, [&](int g) {
+                        rv[k][g] = V.r.ld(c, bg.so[g]);
+                        vv[k][g] = V.v.ld(c, bg.so[g]);
+                    });
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    each_band<G>(msk, [&](int g) {
+                        const int c = c0 + k * BLOCK;
+                        const R s = rv[k][g] - al[g] * vv[k][g];
+                        V.r.st(c, bg.so[g], s);
+                        V.sh.st(c, bg.so[g], s / group_diag(bg, g, dgc[k], vl[k]));
+                        red[g] += (double)s * (double)s;
+                    });
+            });
         }
         block_sum<BLOCK, G>(red, lds);
         if (!refsem) {
@@ -339,17 +418,22 @@ __device__ void bicgstab_bands(const BandGroup<R, G>& bg, unsigned act, const Kr
         // (4) t = A s^ (diagonal term s), <t, s>, <t, t>
 #pragma unroll
         for (int g = 0; g < 2 * G; g++) red[g] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK) {
-            R sv_[G], off[G];
-            each_band<G>(wm, [&](int g) { sv_[g] = V.r.ld(c, bg.so[g]); });
-            group_offdiag<R, G>(bg, wm, V.sh, c, off);
-            each_band<G>(wm, [&](int g) {
-                const R tc = sv_[g] + bg.vt[g] * off[g];
-                V.t.st(c, bg.so[g], tc);
-                red[g] += (double)tc * (double)sv_[g];
-                red[G + g] += (double)tc * (double)tc;
-            });
-        }
+        band_cells<BLOCK, U, G>(N, wm, [&](auto kc, int c0, auto mc) {
+            const unsigned msk = decltype(mc)::value ? decltype(mc)::value : wm;
+            constexpr int K = decltype(kc)::value;
+            R sv_[K][G], off[K][G];
+#pragma unroll
+            for (int k = 0; k < K; k++) each_band<G>(msk, [&](int g) { sv_[k][g] = V.r.ld(c0 + k * BLOCK, bg.so[g]); });
+            group_offdiag<R, G, BLOCK, K>(bg, msk, V.sh, c0, off);
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                each_band<G>(msk, [&](int g) {
+                    const R tc = sv_[k][g] + bg.vt[g] * off[k][g];
+                    V.t.st(c0 + k * BLOCK, bg.so[g], tc);
+                    red[g] += (double)tc * (double)sv_[k][g];
+                    red[G + g] += (double)tc * (double)tc;
+                });
+        });
         block_sum<BLOCK, 2 * G>(red, lds);
         each_band<G>(wm, [&](int g) { s_omega[g] = (red[G + g] != 0) ? red[g] / red[G + g] : 0.0; });
         // (5) x += alpha p^ + omega s^, r = s - omega t, ||r||^2, <r_hat, r>
@@ -361,26 +445,34 @@ __device__ void bicgstab_bands(const BandGroup<R, G>& bg, unsigned act, const Kr
             });
 #pragma unroll
             for (int g = 0; g < 2 * G; g++) red[g] = 0;
-            for (int c = threadIdx.x; c < N; c += BLOCK) {
-                R xv[G], phv[G], shv[G], rv[G], tv[G], rhv[G];
-                each_band<G>(wm, [&](int g) {
-                    const unsigned so = bg.so[g];
-                    xv[g] = V.x.ld(c, so);
-                    phv[g] = V.ph.ld(c, so);
-                    shv[g] = V.sh.ld(c, so);
-                    rv[g] = V.r.ld(c, so);
-                    tv[g] = V.t.ld(c, so);
-                    rhv[g] = V.rh.ld(c, so);
-                });
-                each_band<G>(wm, [&](int g) {
-                    const unsigned so = bg.so[g];
-                    V.x.st(c, so, xv[g] + (al[g] * phv[g] + om5[g] * shv[g]));
-                    const R rn = rv[g] - om5[g] * tv[g];
-                    V.r.st(c, so, rn);
-                    red[g] += (double)rn * (double)rn;
-                    red[G + g] += (double)rhv[g] * (double)rn;
-                });
-            }
+            band_cells<BLOCK, U, G>(N, wm, [&](auto kc, int c0, auto mc) {
+                const unsigned msk = decltype(mc)::value ? decltype(mc)::value : wm;
+                constexpr int K = decltype(kc)::value;
+                R xv[K][G], phv[K][G], shv[K][G], rv[K][G], tv[K][G], rhv[K][G];
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    each_band<G>(msk, [&](int g) {
+                        const int c = c0 + k * BLOCK;
+                        const unsigned so = bg.so[g];
+                        xv[k][g] = V.x.ld(c, so);
+                        phv[k][g] = V.ph.ld(c, so);
+                        shv[k][g] = V.sh.ld(c, so);
+                        rv[k][g] = V.r.ld(c, so);
+                        tv[k][g] = V.t.ld(c, so);
+                        rhv[k][g] = V.rh.ld(c, so);
+                    });
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    each_band<G>(msk, [&](int g) {
+                        const int c = c0 + k * BLOCK;
+                        const unsigned so = bg.so[g];
+                        V.x.st(c, so, xv[k][g] + (al[g] * phv[k][g] + om5[g] * shv[k][g]));
+                        const R rn = rv[k][g] - om5[g] * tv[k][g];
+                        V.r.st(c, so, rn);
+                        red[g] += (double)rn * (double)rn;
+                        red[G + g] += (double)rhv[k][g] * (double)rn;
+                    });
+            });
         }
         block_sum<BLOCK, 2 * G>(red, lds);
         each_band<G>(wm, [&](int g) {
@@ -416,100 +508,6 @@ __global__ __launch_bounds__(BLOCK) void k_solve64_bands(Ops o, Work w, SolveCfg
             each_band<G>(bg.present, [&](int g) {
                 w.iters[bg.sys[g]] += (its[g] > cfg.maxit) ? cfg.maxit : its[g];
                 w.fails[bg.sys[g]] += (its[g] > cfg.maxit) ? 1 : 0;
-            });
-        __syncthreads();
-    }
-}
-
-// Band-batched mixed precision (Jacobi): k_solve_mixed's refinement for G bands of a direction,
-// each band refined until its own FP64 residual criterion holds.
-template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void k_solve_mixed_bands(Ops o, Work w, SolveCfg cfg) {
-    // reductions, the inner solver's per-band scalars, then ||b|| and ||r64|| per band
-    __shared__ double lds[2 * G * (BLOCK / 64) + 9 * G];
-    double* const st = lds + 2 * G * (BLOCK / 64);
-    double* const s_nb = st + 7 * G;
-    double* const s_nr = st + 8 * G;
-    const int B = cfg.S / o.D;
-    const int ngroups = o.D * ((B + G - 1) / G);
-    for (int gid = blockIdx.x; gid < ngroups; gid += gridDim.x) {
-        const int d = gid % o.D, b0 = (gid / o.D) * G;
-        const BandGroup<double, G> b64 = make_group<double, G>(o, B, d, b0);
-        const BandGroup<float, G> b32 = make_group<float, G>(o, B, d, b0);
-        const unsigned P = b64.present;
-        const int N = o.N;
-        const GVec<double> vb = group_vec(b64, w.b), vx = group_vec(b64, w.x), vr64 = group_vec(b64, w.r64);
-        const KrylovVecs<float> V{group_vec(b32, w.f), group_vec(b32, w.y),  group_vec(b32, w.r),
-                                  group_vec(b32, w.rh), group_vec(b32, w.p), group_vec(b32, w.v),
-                                  group_vec(b32, w.ph), group_vec(b32, w.sh), group_vec(b32, w.t)};
-        double red[G];
-#pragma unroll
-        for (int g = 0; g < G; g++) red[g] = 0;
-        for (int c = threadIdx.x; c < N; c += BLOCK)
-            each_band<G>(P, [&](int g) {
-                const double bc = vb.ld(c, b64.so[g]);
-                vr64.st(c, b64.so[g], bc);
-                vx.st(c, b64.so[g], 0.0);
-                red[g] += bc * bc;
-            });
-        block_sum<BLOCK, G>(red, lds);
-        int total[G], fail[G];
-#pragma unroll
-        for (int g = 0; g < G; g++) {
-            s_nb[g] = sqrt(red[g]);
-            s_nr[g] = sqrt(red[g]);
-            total[g] = fail[g] = 0;
-        }
-        for (int step = 0; step < cfg.refine_steps; step++) {
-            unsigned rm = 0;
-            each_band<G>(P, [&](int g) {
-                if (s_nr[g] > cfg.tol * s_nb[g] && s_nr[g] > 1e-300) rm |= 1u << g;
-            });
-            if (!rm) break;
-            double inv[G];
-            each_band<G>(rm, [&](int g) { inv[g] = 1.0 / s_nr[g]; });
-            for (int c = threadIdx.x; c < N; c += BLOCK)
-                each_band<G>(rm, [&](int g) { V.rhs.st(c, b32.so[g], (float)(vr64.ld(c, b64.so[g]) * inv[g])); });
-            __syncthreads();
-            int its[G];
-            bicgstab_bands<float, BLOCK, G>(b32, rm, V, cfg.maxit, 1e-6, false, lds, st, its);
-            each_band<G>(rm, [&](int g) {
-                total[g] += (its[g] > cfg.maxit) ? cfg.maxit : its[g];
-                fail[g] += (its[g] > cfg.maxit) ? 1 : 0;
-            });
-            double nrv[G];
-            each_band<G>(rm, [&](int g) { nrv[g] = s_nr[g]; });
-            for (int c = threadIdx.x; c < N; c += BLOCK)
-                each_band<G>(rm, [&](int g) {
-                    vx.st(c, b64.so[g], vx.ld(c, b64.so[g]) + nrv[g] * (double)V.x.ld(c, b32.so[g]));
-                });
-            __syncthreads();
-            // r64 = b - A x in FP64
-#pragma unroll
-            for (int g = 0; g < G; g++) red[g] = 0;
-            for (int c = threadIdx.x; c < N; c += BLOCK) {
-                const double dgc = buf_ld<double>(b64.dg, (unsigned)c * 8u, 0);
-                const double vl = buf_ld<double>(b64.vol, (unsigned)c * 8u, 0);
-                double xc[G], bc[G], off[G];
-                each_band<G>(rm, [&](int g) {
-                    xc[g] = vx.ld(c, b64.so[g]);
-                    bc[g] = vb.ld(c, b64.so[g]);
-                });
-                group_offdiag<double, G>(b64, rm, vx, c, off);
-                each_band<G>(rm, [&](int g) {
-                    const double acc = group_diag(b64, g, dgc, vl) * xc[g];
-                    const double rc = bc[g] - (acc + b64.vt[g] * off[g]);
-                    vr64.st(c, b64.so[g], rc);
-                    red[g] += rc * rc;
-                });
-            }
-            block_sum<BLOCK, G>(red, lds);
-            each_band<G>(rm, [&](int g) { s_nr[g] = sqrt(red[g]); });
-        }
-        if (threadIdx.x == 0)
-            each_band<G>(P, [&](int g) {
-                w.iters[b64.sys[g]] += total[g];
-                w.fails[b64.sys[g]] += fail[g] + ((s_nr[g] > cfg.tol * s_nb[g] && s_nr[g] > 1e-300) ? 1 : 0);
             });
         __syncthreads();
     }

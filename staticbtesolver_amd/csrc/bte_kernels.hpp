@@ -320,6 +320,7 @@ struct SolveCfg {
     double tol;
     int refine_steps;
     int reference_semantics;  // Jacobi path: reproduce bicgstab.hpp's first step and restarts
+    double tol32;             // mixed precision: relative residual target of each FP32 inner solve
 };
 
 template <typename R>
@@ -462,7 +463,8 @@ __device__ __forceinline__ void sweep(const SysView<R>& sv, const R* v, R* y) {
 //    with r_hat = 6.8 rhs + 0.6; stop when ||r|| / ||r0|| < tol on the recursive residual.
 //  - standard form (sweep path): p = 0, v = 0 so the first step is p = r; an early exit when
 //    ||s|| / ||r0|| < tol (with the exact sweep this ends after one sweep and one product).
-// Returns the iterations used (maxit + 1 if the limit was hit without convergence).
+// Returns the iterations used (maxit + 1 if the limit was hit without convergence, or if
+// stall_window > 0 and the residual norm set no new minimum for that many iterations).
 //
 // Five passes over the vectors per iteration, each fusing what needs no global reduction in
 // between: (1) p update, with the Jacobi preconditioner applied in the same pass; (2) v = A p^
@@ -471,11 +473,12 @@ __device__ __forceinline__ void sweep(const SysView<R>& sv, const R* v, R* y) {
 // element and the order of every per-thread sum are those of the unfused sequence.
 template <typename R, int BLOCK, bool SWEEP>
 __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R* p, R* v, R* ph, R* sh, R* t,
-                        int maxit, double tol, bool refsem, double* lds) {
-    // cells per group in the streaming passes and in the operator products (1024-wide work-groups
-    // get half: their 128-register budget would otherwise spill)
-    constexpr int US = (BLOCK >= 1024) ? 2 : 4;
-    constexpr int UM = (BLOCK >= 1024) ? 1 : 2;
+                        int maxit, double tol, bool refsem, double* lds, int stall_window = 0) {
+    // cells per group in the streaming passes and in the operator products: FP32 groups twice as
+    // many cells for the same bytes in flight; 1024-wide work-groups get half (their 128-register
+    // budget would otherwise spill)
+    constexpr int US = ((BLOCK >= 1024) ? 2 : 4) * (sizeof(R) == 4 ? 2 : 1);
+    constexpr int UM = ((BLOCK >= 1024) ? 1 : 2) * (sizeof(R) == 4 ? 2 : 1);
     const int N = sv.N;
     double red[2];
     // setup: x = 0, r = rhs, r_hat = r, p = r (reference) or 0, v = 0
@@ -503,6 +506,8 @@ __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R
     double rho_next = red[0];  // <r_hat, r> with r_hat = r
     int it = 0;
     bool converged = false;
+    double best = nrm0;
+    int best_it = 0;
     for (int i = 1; i <= maxit; i++) {
         it = i;
         if (nrm < 1e-50) {
@@ -672,6 +677,13 @@ __device__ int bicgstab(const SysView<R>& sv, const R* rhs, R* x, R* r, R* rh, R
         rho_next = red[1];
         if (nrm / nrm0 < tol) {
             converged = true;
+            break;
+        }
+        // stagnation: no new smallest residual for stall_window iterations
+        if (nrm < best) {
+            best = nrm;
+            best_it = i;
+        } else if (stall_window > 0 && i - best_it >= stall_window) {
             break;
         }
     }
@@ -1130,7 +1142,16 @@ __global__ __launch_bounds__(BLOCK) void k_solve64(Ops o, Work w, SolveCfg cfg) 
 }
 
 // Mixed precision: FP64 RHS and residual, FP32 BiCGSTAB on the scaled residual equation,
-// x += ||r|| * y per refinement step, until ||b - A x|| / ||b|| < tol (FP64, on the GPU).
+// x += ||r|| * y per refinement step, until ||b - A x|| / ||b|| < tol (FP64, on the GPU).  FP32
+// BiCGSTAB on a transport operator can stall far above its target (the Jacobi-preconditioned
+// operator is far from normal and FP32 rounding is amplified along long upwind chains): when an
+// inner solve sets no new residual minimum for kStall32 iterations or hits the limit, or the
+// refinement steps end short of tol, the work-group re-solves the system in FP64 at once (the
+// FP64 solve of k_solve64), overlapping the other work-groups' FP32 solves.  The FP32 Krylov
+// vectors of system s live in the first half of its FP64 slot, so that re-solve touches no other
+// system's vectors.
+constexpr int kStall32 = 100;
+
 template <int BLOCK, bool SWEEP>
 __global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg cfg) {
     __shared__ double lds[2 * (BLOCK / 64)];
@@ -1153,21 +1174,26 @@ __global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg c
         const SysView<float> sv32 = make_view<float>(o, b, d);
         const SysView<double> sv64 = make_view<double>(o, b, d);
         float* y = (float*)w.y + base;
-        float* r32 = (float*)w.r + base;
-        int total = 0, fail = 0;
+        float* r32 = (float*)((double*)w.r + base);
+        int total = 0;
+        bool stalled = false;
         for (int step = 0; step < cfg.refine_steps && nr > cfg.tol * nb && nr > 1e-300; step++) {
             // y solves A y = r64 / ||r64|| in FP32 (scaled into FP32 range)
             const double inv = 1.0 / nr;
             float* fr = (float*)w.f + base;  // scaled residual as the FP32 right-hand side
             for (int c = threadIdx.x; c < o.N; c += BLOCK) fr[c] = (float)(r64[c] * inv);
             __syncthreads();
-            // FP32 inner tolerance: as deep as FP32 allows; refinement supplies the rest
-            const int it = bicgstab<float, BLOCK, SWEEP>(sv32, fr, y, r32, (float*)w.rh + base, (float*)w.p + base,
-                                                         (float*)w.v + base, (float*)w.ph + base,
-                                                         (float*)w.sh + base, (float*)w.t + base, cfg.maxit,
-                                                         1e-6, false, lds);
+            // FP32 inner tolerance cfg.tol32 (short of FP32 rounding, where BiCGSTAB can stall);
+            // the FP64 refinement supplies the rest
+            const int it = bicgstab<float, BLOCK, SWEEP>(
+                sv32, fr, y, r32, (float*)((double*)w.rh + base), (float*)((double*)w.p + base),
+                (float*)((double*)w.v + base), (float*)((double*)w.ph + base), (float*)((double*)w.sh + base),
+                (float*)((double*)w.t + base), cfg.maxit, cfg.tol32, false, lds, kStall32);
             total += (it > cfg.maxit) ? cfg.maxit : it;
-            fail += (it > cfg.maxit) ? 1 : 0;
+            if (it > cfg.maxit) {  // stalled: y is not a correction to keep
+                stalled = true;
+                break;
+            }
             for (int c = threadIdx.x; c < o.N; c += BLOCK) x[c] += nr * (double)y[c];
             __syncthreads();
             // r64 = b - A x in FP64
@@ -1186,9 +1212,18 @@ __global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg c
             block_sum<BLOCK, 1>(red, lds);
             nr = sqrt(red[0]);
         }
+        int fail = 0;
+        if (stalled || (nr > cfg.tol * nb && nr > 1e-300)) {
+            const int it = bicgstab<double, BLOCK, SWEEP>(
+                sv64, rhs, x, (double*)w.r + base, (double*)w.rh + base, (double*)w.p + base, (double*)w.v + base,
+                (double*)w.ph + base, (double*)w.sh + base, (double*)w.t + base, cfg.maxit, cfg.tol,
+                cfg.reference_semantics != 0, lds);
+            total += (it > cfg.maxit) ? cfg.maxit : it;
+            fail = (it > cfg.maxit) ? 1 : 0;
+        }
         if (threadIdx.x == 0) {
             w.iters[s] += total;
-            w.fails[s] += fail + ((nr > cfg.tol * nb && nr > 1e-300) ? 1 : 0);
+            w.fails[s] += fail;
         }
         __syncthreads();
     }

@@ -305,6 +305,7 @@ void bte_default_opts(bte_opts* o) {
     o->accel = BTE_ACCEL_NONE;
     o->accel_memory = 20;
     o->band_group = 0;
+    o->inner_tol32 = 1e-5;
 }
 
 int bte_ctx_create(int device, bte_ctx** out) {
@@ -614,14 +615,16 @@ int ensure_work(bte_ctx* ctx, int precision) {
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
     const int S = ctx->B * ctx->D;
     if (ctx->work_elems == SN && ctx->work_prec == precision) return BTE_OK;
-    const size_t ks = (precision == BTE_PREC_MIXED) ? sizeof(float) : sizeof(double);
+    // Krylov vectors are FP64-sized in both precisions: the mixed form re-solves in FP64 a system
+    // whose FP32 inner solve stalls
+    const size_t ks = sizeof(double);
     HIPCHK(ctx->eeA.ensure(sizeof(double) * SN));
     HIPCHK(ctx->eeB.ensure(sizeof(double) * SN));
     HIPCHK(ctx->wb.ensure(sizeof(double) * SN));
     if (precision == BTE_PREC_MIXED) {
         HIPCHK(ctx->wr64.ensure(sizeof(double) * SN));
-        HIPCHK(ctx->wy.ensure(ks * SN));
-        HIPCHK(ctx->wf.ensure(ks * SN));
+        HIPCHK(ctx->wy.ensure(sizeof(float) * SN));
+        HIPCHK(ctx->wf.ensure(sizeof(float) * SN));
     }
     for (DBuf* b : {&ctx->wr, &ctx->wrh, &ctx->wp, &ctx->wv, &ctx->wph, &ctx->wsh, &ctx->wt}) HIPCHK(b->ensure(ks * SN));
     HIPCHK(ctx->iters.ensure(sizeof(int) * S));
@@ -700,15 +703,16 @@ struct Shape {
 // Bands per work-group a Krylov form uses when it is not autotuned: opts.band_group if it is a
 // supported group of the Jacobi path, else one system per work-group.
 int krylov_group(const bte_opts* o) {
-    if (o->precond != BTE_PC_JACOBI) return 1;
+    if (o->precond != BTE_PC_JACOBI || o->precision != BTE_PREC_FP64) return 1;
     return (o->band_group == 2 || o->band_group == 4) ? o->band_group : 1;
 }
 
-// Whether the band-batched kernels run G bands per work-group of `block` threads: the Jacobi path
-// only, G in {2, 4}, work-groups of at most 512 (wider ones would spill), and G bands of D*N FP64
-// elements below 4 GiB (the kernels address a group's vectors through 32-bit buffer offsets).
+// Whether the band-batched kernels run G bands per work-group of `block` threads: the FP64 Jacobi
+// path only (for the FP32 inner solves of the mixed form one system per work-group measured
+// faster), G in {2, 4}, work-groups of at most 512 (wider ones would spill), and G bands of D*N
+// FP64 elements below 4 GiB (the kernels address a group's vectors through 32-bit buffer offsets).
 bool group_ok(const bte_ctx* ctx, const bte_opts* o, int G, int block) {
-    return o->precond == BTE_PC_JACOBI && (G == 2 || G == 4) && block <= 512 &&
+    return o->precond == BTE_PC_JACOBI && o->precision == BTE_PREC_FP64 && (G == 2 || G == 4) && block <= 512 &&
            (double)G * ctx->D * ctx->N * sizeof(double) < 4294967295.0;
 }
 
@@ -749,7 +753,8 @@ void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, con
 template <int BLOCK>
 void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const double* ee_prev, double* ee_curr,
                     const double* dens) {
-    SolveCfg cfg{S, o->inner_maxit, o->inner_tol, std::max(1, o->refine_steps), o->precond == BTE_PC_JACOBI ? 1 : 0};
+    SolveCfg cfg{S, o->inner_maxit, o->inner_tol, std::max(1, o->refine_steps), o->precond == BTE_PC_JACOBI ? 1 : 0,
+                 o->inner_tol32 > 0 ? o->inner_tol32 : 1e-5};
     const Geo g = geo_of(ctx);
     const Quad q = quad_of(ctx);
     const Bnd bd = bnd_of(ctx);
@@ -814,18 +819,10 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     if constexpr (BLOCK <= 512) if (G > 1) {
         // band-batched: one work-group per (direction, group of G bands)
         const dim3 gg(ctx->D * ((ctx->B + G - 1) / G));
-        const bool mixed = o->precision == BTE_PREC_MIXED;
-        if (G == 2) {
-            if (mixed)
-                hipLaunchKernelGGL((k_solve_mixed_bands<BLOCK, 2>), gg, blk, 0, ctx->stream, op, w, cfg);
-            else
-                hipLaunchKernelGGL((k_solve64_bands<BLOCK, 2>), gg, blk, 0, ctx->stream, op, w, cfg);
-        } else {
-            if (mixed)
-                hipLaunchKernelGGL((k_solve_mixed_bands<BLOCK, 4>), gg, blk, 0, ctx->stream, op, w, cfg);
-            else
-                hipLaunchKernelGGL((k_solve64_bands<BLOCK, 4>), gg, blk, 0, ctx->stream, op, w, cfg);
-        }
+        if (G == 2)
+            hipLaunchKernelGGL((k_solve64_bands<BLOCK, 2>), gg, blk, 0, ctx->stream, op, w, cfg);
+        else
+            hipLaunchKernelGGL((k_solve64_bands<BLOCK, 4>), gg, blk, 0, ctx->stream, op, w, cfg);
         return;
     }
     if (o->precision == BTE_PREC_MIXED) {
