@@ -831,46 +831,40 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
         sweep_fast_system<BLOCK, XLDS>(g, q, bd, bands, o, w, s, xs, ee_prev, dens);
 }
 
-// Direct sweep with the recent intensities in an LDS ring and software-pipelined operand loads.
-// Upwind neighbours are addressed by their level-order position p (swp_pos); the value of position p
-// lives in ring slot p mod R.  The host enables this form only when every upwind neighbour of a
-// cell in level l satisfies p >= end(l) - R, so each read finds its slot not yet overwritten and no
-// intensity is ever read back from HBM; the output is written to HBM once per cell.  Operands of
-// level l+2 and the emission density of level l+1 are loaded while level l computes, so a level
-// costs an LDS gather, the arithmetic and one LDS barrier.  Levels wider than BLOCK finish their
-// extra cells unpipelined.  Same arithmetic, in the same order, as k_sweep_fast.
 // Right-hand sides of every system in sweep (level) order, rhs_lo[s*N + i] for cell perm_d[i]: the
 // emission term for interior cells, rhs_at for cells with an incoming boundary face -- the values
 // the level sweeps form on the fly, computed up front, fully parallel, so that the pipelined sweep
 // streams them with its other operands instead of gathering by cell inside the level chain.
-// One thread per (direction, level position) forms the values of every band, so the level-ordered
-// operands are read once per direction rather than once per system.
+// One thread per (system, level position): consecutive threads take consecutive positions of one
+// system, so the stores are coalesced; the level-ordered operands of a direction are re-read per
+// band from L2.
 __global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const double* __restrict__ ee_prev,
                              const double* __restrict__ dens, double* __restrict__ rhs_lo) {
-    const int N = o.N, D = o.D, B = S / D;
-    const long long total = (long long)D * N;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int d = (int)(t / N), i = (int)(t % N);
-        if (o.swp_bnd[t]) continue;  // cells with an incoming boundary face: k_rhs_levels_bnd
-        const int c = o.perm[t];
-        const double vw = o.swp_vw[t];
-        for (int b = 0; b < B; b++) rhs_lo[((size_t)b * D + d) * N + i] = dens[(size_t)b * N + c] * vw;
+    const int N = o.N, D = o.D;
+    for (int s = blockIdx.y; s < S; s += gridDim.y) {
+        const int b = s / D, d = s % D;
+        const size_t od = (size_t)d * N;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+            if (o.swp_bnd[od + i]) continue;  // cells with an incoming boundary face: k_rhs_levels_bnd
+            rhs_lo[(size_t)s * N + i] = dens[(size_t)b * N + o.perm[od + i]] * o.swp_vw[od + i];
+        }
     }
 }
 
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
-// their face loops run in work-groups of their own instead of stalling the interior waves.
+// their face loops run in work-groups of their own instead of stalling the interior waves; one
+// thread per (list entry, band).
 __global__ void k_rhs_levels_bnd(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const int* __restrict__ list,
                                  int n_list, const double* __restrict__ ee_prev, const double* __restrict__ dens,
                                  double* __restrict__ rhs_lo) {
     const int N = o.N, D = o.D, B = S / D;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_list; k += gridDim.x * blockDim.x) {
+    const long long total = (long long)n_list * B;
+    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
+         u += (long long)gridDim.x * blockDim.x) {
+        const int b = (int)(u / n_list), k = (int)(u % n_list);
         const int t = list[k];
         const int d = t / N, i = t % N;
-        const int c = o.perm[t];
-        for (int b = 0; b < B; b++)
-            rhs_lo[((size_t)b * D + d) * N + i] = rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens);
+        rhs_lo[((size_t)b * D + d) * N + i] = rhs_at(g, q, bd, bands, b, d, o.perm[t], ee_prev, dens);
     }
 }
 
@@ -903,6 +897,30 @@ __device__ __forceinline__ void sweep_item_load(int i, int N, const int* __restr
     it.dgc = sdg[j];
     it.vw = svw[j];
     it.rhs = rhs[j];
+}
+
+// Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order: positions p >= lim from
+// the LDS ring, older ones from HBM.  The HBM reads are rare, and each one's value is consumed (a
+// register move) inside its own branch: a load still pending where the branches merge makes the
+// compiler wait there for every outstanding load, the operand prefetches of later levels included
+// (a select between the two sources is worse: the compiler merges them into one flat load and
+// waits for all loads after every cell).
+__device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, const double* ring, int ring_mask,
+                                                  const double* xg) {
+    double v[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) v[k] = ring[it.p[k] & ring_mask];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (it.p[k] >= 0 && it.p[k] < lim) {
+            const double h = xg[it.col[k]];
+            asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
+        }
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (it.p[k] >= 0) acc += it.a[k] * v[k];
+    return acc;
 }
 
 // LDS-only work-group barrier: the ring is the only memory threads exchange inside the sweep, so
@@ -951,10 +969,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             return i < lps[l + 1] ? i : -1;
         };
         auto solve_cell = [&](const SweepItem& it, int end) {
-            double acc = 0;
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-                if (it.p[k] >= 0) acc += it.a[k] * (it.p[k] >= end - R ? ring[it.p[k] & ring_mask] : xg[it.col[k]]);
+            const double acc = ring_upwind_sum(it, end - R, ring, ring_mask, xg);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
             xg[it.c] = xv;
@@ -1033,10 +1048,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         return i < lps[l + 1] ? i : -1;
     };
     auto solve_cell = [&](const SweepItem& it, int end) {
-        double acc = 0;
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-            if (it.p[k] >= 0) acc += it.a[k] * (it.p[k] >= end - R ? ring[it.p[k] & ring_mask] : xg[it.col[k]]);
+        const double acc = ring_upwind_sum(it, end - R, ring, ring_mask, xg);
         const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
         ring[it.i & ring_mask] = xv;
         xg[it.c] = xv;

@@ -742,10 +742,12 @@ int ring_size(const bte_ctx* ctx, int* sync_every) {
 // Right-hand sides of all systems in level order (interior cells, then the boundary-cell list).
 void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out) {
     const int S = ctx->B * ctx->D;
-    hipLaunchKernelGGL(k_rhs_levels, dim3(grid_for((long long)ctx->D * ctx->N, 256)), dim3(256), 0, st, geo_of(ctx),
-                       quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx), S, ee_prev, dens, out);
+    const dim3 grid_lv((ctx->N + 255) / 256, std::min(S, 65535));
+    hipLaunchKernelGGL(k_rhs_levels, grid_lv, dim3(256), 0, st, geo_of(ctx), quad_of(ctx), bnd_of(ctx), bands_of(ctx),
+                       ops_of(ctx), S, ee_prev, dens, out);
     if (ctx->n_bnd_list > 0)
-        hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(grid_for(ctx->n_bnd_list, 64)), dim3(64), 0, st, geo_of(ctx),
+        hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(grid_for((long long)ctx->n_bnd_list * ctx->B, 64)), dim3(64), 0, st,
+                           geo_of(ctx),
                            quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx), S, ctx->bnd_list.as<int>(),
                            ctx->n_bnd_list, ee_prev, dens, out);
 }
