@@ -868,6 +868,17 @@ __global__ void k_rhs_levels_bnd(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int 
     }
 }
 
+// Global-memory (HBM) views of pointers.  A pointer the kernel reads from memory (the uneven
+// batch's per-problem descriptors) is generic to the compiler: loads through it become flat loads,
+// which it counts as LDS traffic too, so every LDS wait of the pipelined sweeps would also wait for
+// the operand prefetches.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem<T>* to_global(T* p) {
+    return (gmem<T>*)p;
+}
+
 struct SweepItem {
     int i, c;     // level-order position (-1: no cell for this thread) and cell
     int p[3];     // level-order positions of the upwind neighbours (-1: none)
@@ -880,11 +891,12 @@ struct SweepItem {
 // loads position 0 and is ignored), so the compiler's wait counting stays exact and a later use of
 // the item waits for its own loads only.
 template <int W>
-__device__ __forceinline__ void sweep_item_load(int i, int N, const int* __restrict__ perm,
-                                                const int* __restrict__ spos, const int* __restrict__ scol,
-                                                const double* __restrict__ sval, const double* __restrict__ sdg,
-                                                const double* __restrict__ svw, const double* __restrict__ rhs,
-                                                SweepItem& it) {
+__device__ __forceinline__ void sweep_item_load(int i, int N, const gmem<int>* __restrict__ perm,
+                                                const gmem<int>* __restrict__ spos, const gmem<int>* __restrict__ scol,
+                                                const gmem<double>* __restrict__ sval,
+                                                const gmem<double>* __restrict__ sdg,
+                                                const gmem<double>* __restrict__ svw,
+                                                const gmem<double>* __restrict__ rhs, SweepItem& it) {
     it.i = i;
     const int j = i < 0 ? 0 : i;
     it.c = perm[j];
@@ -906,7 +918,7 @@ __device__ __forceinline__ void sweep_item_load(int i, int N, const int* __restr
 // (a select between the two sources is worse: the compiler merges them into one flat load and
 // waits for all loads after every cell).
 __device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, const double* ring, int ring_mask,
-                                                  const double* xg) {
+                                                  const gmem<double>* xg) {
     double v[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) v[k] = ring[it.p[k] & ring_mask];
@@ -951,14 +963,14 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
     {
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
-        double* __restrict__ xg = w.x + (size_t)s * N;
-        const int* __restrict__ perm = o.perm + (size_t)d * N;
-        const int* __restrict__ spos = o.swp_pos + (size_t)d * W * N;
-        const int* __restrict__ scol = o.swp_col + (size_t)d * W * N;
-        const double* __restrict__ sval = o.swp_val + (size_t)d * W * N;
-        const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
-        const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
-        const double* __restrict__ rhs = w.b + (size_t)s * N;
+        gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
+        const gmem<int>* __restrict__ perm = to_global(o.perm + (size_t)d * N);
+        const gmem<int>* __restrict__ spos = to_global(o.swp_pos + (size_t)d * W * N);
+        const gmem<int>* __restrict__ scol = to_global(o.swp_col + (size_t)d * W * N);
+        const gmem<double>* __restrict__ sval = to_global(o.swp_val + (size_t)d * W * N);
+        const gmem<double>* __restrict__ sdg = to_global(o.swp_dg + (size_t)d * N);
+        const gmem<double>* __restrict__ svw = to_global(o.swp_vw + (size_t)d * N);
+        const gmem<double>* __restrict__ rhs = to_global(w.b + (size_t)s * N);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
         for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
@@ -1030,14 +1042,14 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     int* lps = (int*)(ring + R);
     const int b = s / o.D, d = s % o.D;
     const double vt = o.vt[b];
-    double* __restrict__ xg = w.x + (size_t)s * N;
-    const int* __restrict__ perm = o.perm + (size_t)d * N;
-    const int* __restrict__ spos = o.swp_pos + (size_t)d * W * N;
-    const int* __restrict__ scol = o.swp_col + (size_t)d * W * N;
-    const double* __restrict__ sval = o.swp_val + (size_t)d * W * N;
-    const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
-    const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
-    const double* __restrict__ rhs = w.b + (size_t)s * N;
+    gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
+    const gmem<int>* __restrict__ perm = to_global(o.perm + (size_t)d * N);
+    const gmem<int>* __restrict__ spos = to_global(o.swp_pos + (size_t)d * W * N);
+    const gmem<int>* __restrict__ scol = to_global(o.swp_col + (size_t)d * W * N);
+    const gmem<double>* __restrict__ sval = to_global(o.swp_val + (size_t)d * W * N);
+    const gmem<double>* __restrict__ sdg = to_global(o.swp_dg + (size_t)d * N);
+    const gmem<double>* __restrict__ svw = to_global(o.swp_vw + (size_t)d * N);
+    const gmem<double>* __restrict__ rhs = to_global(w.b + (size_t)s * N);
     const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
     const int L = o.lvl_cnt[d];
     for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
