@@ -21,29 +21,6 @@
 
 namespace bte {
 
-typedef unsigned int bte_u32x2 __attribute__((ext_vector_type(2)));
-
-// Raw buffer resource over [p, p + bytes) (stride 0; word 3 as for every gfx9 target).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, size_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(unsigned)bytes, 0x00020000);
-}
-
-template <typename T>
-__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-    else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-
-template <typename T>
-__device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    if constexpr (sizeof(T) == 8)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bte_u32x2, v), r, voff, soff, 0);
-    else
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
-}
-
 // One [B*D*N] vector seen from a band group: element c of band g.
 template <typename R>
 struct GVec {

@@ -879,6 +879,32 @@ __device__ __forceinline__ gmem<T>* to_global(T* p) {
     return (gmem<T>*)p;
 }
 
+// Raw buffer loads and stores: a buffer resource (four scalar registers) holds an array's base and
+// size, a scalar byte offset selects a slot (band, operator column), and one vector byte offset per
+// lane is shared by every array indexed alike -- in place of a 64-bit vector address per array.
+typedef unsigned int bte_u32x2 __attribute__((ext_vector_type(2)));
+
+// Raw buffer resource over [p, p + bytes) (stride 0; word 3 as for every gfx9 target).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(unsigned)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bte_u32x2, v), r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
 struct SweepItem {
     int i, c;     // level-order position (-1: no cell for this thread) and cell
     int p[3];     // level-order positions of the upwind neighbours (-1: none)
@@ -887,28 +913,45 @@ struct SweepItem {
     double dgc, vw, rhs;
 };
 
+// The level-ordered arrays of one system's sweep, as buffer resources.
+struct SweepSrc {
+    __amdgpu_buffer_rsrc_t perm, spos, scol, sval, sdg, svw, rhs;
+    unsigned n4, n8;  // byte distance between operator slots: N * 4, N * 8
+};
+
+template <int W>
+__device__ __forceinline__ SweepSrc sweep_src(const Ops& o, const Work& w, int s, int d) {
+    const size_t N = (size_t)o.N;
+    SweepSrc r;
+    r.perm = buffer_of(o.perm + d * N, N * 4);
+    r.spos = buffer_of(o.swp_pos + d * W * N, W * N * 4);
+    r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
+    r.sval = buffer_of(o.swp_val + d * W * N, W * N * 8);
+    r.sdg = buffer_of(o.swp_dg + d * N, N * 8);
+    r.svw = buffer_of(o.swp_vw + d * N, N * 8);
+    r.rhs = buffer_of(w.b + (size_t)s * N, N * 8);
+    r.n4 = (unsigned)(N * 4);
+    r.n8 = (unsigned)(N * 8);
+    return r;
+}
+
 // Straight-line loads (W, the incoming-slot count, is a compile-time constant and an absent item
 // loads position 0 and is ignored), so the compiler's wait counting stays exact and a later use of
-// the item waits for its own loads only.
+// the item waits for its own loads only.  All loads share the lane's two byte offsets.
 template <int W>
-__device__ __forceinline__ void sweep_item_load(int i, int N, const gmem<int>* __restrict__ perm,
-                                                const gmem<int>* __restrict__ spos, const gmem<int>* __restrict__ scol,
-                                                const gmem<double>* __restrict__ sval,
-                                                const gmem<double>* __restrict__ sdg,
-                                                const gmem<double>* __restrict__ svw,
-                                                const gmem<double>* __restrict__ rhs, SweepItem& it) {
+__device__ __forceinline__ void sweep_item_load(int i, const SweepSrc& src, SweepItem& it) {
     it.i = i;
-    const int j = i < 0 ? 0 : i;
-    it.c = perm[j];
+    const unsigned j4 = (unsigned)(i < 0 ? 0 : i) * 4u, j8 = j4 * 2u;
+    it.c = buf_ld<int>(src.perm, j4, 0);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        it.p[k] = (k < W) ? spos[(size_t)k * N + j] : -1;
-        it.col[k] = (k < W) ? scol[(size_t)k * N + j] : -1;
-        it.a[k] = (k < W) ? sval[(size_t)k * N + j] : 0.0;
+        it.p[k] = (k < W) ? buf_ld<int>(src.spos, j4, k * src.n4) : -1;
+        it.col[k] = (k < W) ? buf_ld<int>(src.scol, j4, k * src.n4) : -1;
+        it.a[k] = (k < W) ? buf_ld<double>(src.sval, j8, k * src.n8) : 0.0;
     }
-    it.dgc = sdg[j];
-    it.vw = svw[j];
-    it.rhs = rhs[j];
+    it.dgc = buf_ld<double>(src.sdg, j8, 0);
+    it.vw = buf_ld<double>(src.svw, j8, 0);
+    it.rhs = buf_ld<double>(src.rhs, j8, 0);
 }
 
 // Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order: positions p >= lim from
@@ -964,13 +1007,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
         gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
-        const gmem<int>* __restrict__ perm = to_global(o.perm + (size_t)d * N);
-        const gmem<int>* __restrict__ spos = to_global(o.swp_pos + (size_t)d * W * N);
-        const gmem<int>* __restrict__ scol = to_global(o.swp_col + (size_t)d * W * N);
-        const gmem<double>* __restrict__ sval = to_global(o.swp_val + (size_t)d * W * N);
-        const gmem<double>* __restrict__ sdg = to_global(o.swp_dg + (size_t)d * N);
-        const gmem<double>* __restrict__ svw = to_global(o.swp_vw + (size_t)d * N);
-        const gmem<double>* __restrict__ rhs = to_global(w.b + (size_t)s * N);
+        const SweepSrc src = sweep_src<W>(o, w, s, d);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
         for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
@@ -991,11 +1028,11 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         // (a copy of a register with a load in flight would wait for that load)
         auto level = [&](int l, SweepItem& cur, SweepItem& ahead) {
             const int beg = lps[l], end = lps[l + 1];
-            sweep_item_load<W>(first(l + 2), N, perm, spos, scol, sval, sdg, svw, rhs, ahead);
+            sweep_item_load<W>(first(l + 2), src, ahead);
             if (cur.i >= 0) solve_cell(cur, end);
             for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
                 SweepItem E;
-                sweep_item_load<W>(i, N, perm, spos, scol, sval, sdg, svw, rhs, E);
+                sweep_item_load<W>(i, src, E);
                 solve_cell(E, end);
             }
             if ((l + 1) % K == 0)
@@ -1004,8 +1041,8 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
                 lds_barrier();
         };
         SweepItem X0, X1, X2;
-        sweep_item_load<W>(first(0), N, perm, spos, scol, sval, sdg, svw, rhs, X0);
-        sweep_item_load<W>(first(1), N, perm, spos, scol, sval, sdg, svw, rhs, X1);
+        sweep_item_load<W>(first(0), src, X0);
+        sweep_item_load<W>(first(1), src, X1);
         X2.i = -1;
         for (int l = 0; l < L; l += 3) {
             level(l, X0, X2);
@@ -1043,13 +1080,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     const int b = s / o.D, d = s % o.D;
     const double vt = o.vt[b];
     gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
-    const gmem<int>* __restrict__ perm = to_global(o.perm + (size_t)d * N);
-    const gmem<int>* __restrict__ spos = to_global(o.swp_pos + (size_t)d * W * N);
-    const gmem<int>* __restrict__ scol = to_global(o.swp_col + (size_t)d * W * N);
-    const gmem<double>* __restrict__ sval = to_global(o.swp_val + (size_t)d * W * N);
-    const gmem<double>* __restrict__ sdg = to_global(o.swp_dg + (size_t)d * N);
-    const gmem<double>* __restrict__ svw = to_global(o.swp_vw + (size_t)d * N);
-    const gmem<double>* __restrict__ rhs = to_global(w.b + (size_t)s * N);
+    const SweepSrc src = sweep_src<W>(o, w, s, d);
     const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
     const int L = o.lvl_cnt[d];
     for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
@@ -1069,13 +1100,13 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         const int beg = lps[l], end = lps[l + 1];
 #pragma unroll
         for (int k = 0; k < IPL; k++)
-            sweep_item_load<W>(first(l + 2, k), N, perm, spos, scol, sval, sdg, svw, rhs, ahead[k]);
+            sweep_item_load<W>(first(l + 2, k), src, ahead[k]);
 #pragma unroll
         for (int k = 0; k < IPL; k++)
             if (cur[k].i >= 0) solve_cell(cur[k], end);
         for (int i = beg + 64 * IPL + lane; i < end; i += 64) {
             SweepItem E;
-            sweep_item_load<W>(i, N, perm, spos, scol, sval, sdg, svw, rhs, E);
+            sweep_item_load<W>(i, src, E);
             solve_cell(E, end);
         }
         if ((l + 1) % K == 0)
@@ -1086,8 +1117,8 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     SweepItem X0[IPL], X1[IPL], X2[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
-        sweep_item_load<W>(first(0, k), N, perm, spos, scol, sval, sdg, svw, rhs, X0[k]);
-        sweep_item_load<W>(first(1, k), N, perm, spos, scol, sval, sdg, svw, rhs, X1[k]);
+        sweep_item_load<W>(first(0, k), src, X0[k]);
+        sweep_item_load<W>(first(1, k), src, X1[k]);
         X2[k].i = -1;
     }
     for (int l = 0; l < L; l += 3) {
