@@ -960,17 +960,21 @@ __device__ __forceinline__ void sweep_item_load(int i, const SweepSrc& src, Swee
 // compiler wait there for every outstanding load, the operand prefetches of later levels included
 // (a select between the two sources is worse: the compiler merges them into one flat load and
 // waits for all loads after every cell).
+// FB = false: the host found every upwind position within the ring, so there is no HBM path.
+template <bool FB>
 __device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, const double* ring, int ring_mask,
                                                   const gmem<double>* xg) {
     double v[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) v[k] = ring[it.p[k] & ring_mask];
+    if constexpr (FB) {
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-        if (it.p[k] >= 0 && it.p[k] < lim) {
-            const double h = xg[it.col[k]];
-            asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
-        }
+        for (int k = 0; k < 3; k++)
+            if (it.p[k] >= 0 && it.p[k] < lim) {
+                const double h = xg[it.col[k]];
+                asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
+            }
+    }
     double acc = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++)
@@ -995,8 +999,9 @@ __host__ __device__ inline size_t ring_lds_bytes(int ring, int max_levels) {
 // every o.ring_sync levels the work-group synchronises with its HBM writes complete, and the host
 // sizes R to cover o.ring_sync consecutive levels, so such a value was written before the last of
 // those synchronisations.  In between, levels synchronise on the LDS only, leaving the prefetches in
-// flight.
-template <int BLOCK, int W>
+// flight.  FB = false when the host found every upwind position within R of its level's end: no
+// HBM read-back path and no periodic full synchronisation at all.
+template <int BLOCK, int W, bool FB>
 __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
     const int N = o.N;
     const int tid = threadIdx.x;
@@ -1018,7 +1023,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             return i < lps[l + 1] ? i : -1;
         };
         auto solve_cell = [&](const SweepItem& it, int end) {
-            const double acc = ring_upwind_sum(it, end - R, ring, ring_mask, xg);
+            const double acc = ring_upwind_sum<FB>(it, end - R, ring, ring_mask, xg);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
             xg[it.c] = xv;
@@ -1035,7 +1040,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
                 sweep_item_load<W>(i, src, E);
                 solve_cell(E, end);
             }
-            if ((l + 1) % K == 0)
+            if (FB && (l + 1) % K == 0)
                 __syncthreads();  // HBM writes of the levels so far complete and visible
             else
                 lds_barrier();
@@ -1055,11 +1060,11 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
 }
 
 // W = incoming slots per cell (F - 1: 1 in 1D, 2 in 2D, 3 in 3D)
-template <int BLOCK, int W>
+template <int BLOCK, int W, bool FB>
 __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
     for (int s = blockIdx.x; s < S; s += gridDim.x)
-        sweep_ring_system<BLOCK, W>(o, w, s, ring, ring_mask);
+        sweep_ring_system<BLOCK, W, FB>(o, w, s, ring, ring_mask);
 }
 
 // Single-wave form of the pipelined sweep (variant 5): one 64-lane wave per system, IPL cells of a
@@ -1070,7 +1075,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int 
 // extra cells unpipelined.  Same arithmetic, in the same order, as the other sweeps.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-template <int W, int IPL>
+template <int W, int IPL, bool FB>
 __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
     const int N = o.N;
     const int lane = threadIdx.x;
@@ -1091,7 +1096,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         return i < lps[l + 1] ? i : -1;
     };
     auto solve_cell = [&](const SweepItem& it, int end) {
-        const double acc = ring_upwind_sum(it, end - R, ring, ring_mask, xg);
+        const double acc = ring_upwind_sum<FB>(it, end - R, ring, ring_mask, xg);
         const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
         ring[it.i & ring_mask] = xv;
         xg[it.c] = xv;
@@ -1109,7 +1114,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
             sweep_item_load<W>(i, src, E);
             solve_cell(E, end);
         }
-        if ((l + 1) % K == 0)
+        if (FB && (l + 1) % K == 0)
             __syncthreads();  // HBM writes so far complete
         else
             wave_lds_sync();
@@ -1130,10 +1135,10 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     if (lane == 0) w.iters[s] += 1;
 }
 
-template <int W>
+template <int W, bool FB>
 __global__ __launch_bounds__(64) void k_sweep_wave(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
-    for (int s = blockIdx.x; s < S; s += gridDim.x) sweep_wave_system<W, 2>(o, w, s, ring, ring_mask);
+    for (int s = blockIdx.x; s < S; s += gridDim.x) sweep_wave_system<W, 2, FB>(o, w, s, ring, ring_mask);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1161,9 +1166,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_batch(const SweepDesc* __restri
         const SweepDesc& D = desc[it.x];
         if constexpr (RING)
             switch (D.o.W) {
-                case 1: sweep_ring_system<BLOCK, 1>(D.o, D.w, it.y, lds_x, ring_mask); break;
-                case 2: sweep_ring_system<BLOCK, 2>(D.o, D.w, it.y, lds_x, ring_mask); break;
-                default: sweep_ring_system<BLOCK, 3>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                case 1: sweep_ring_system<BLOCK, 1, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                case 2: sweep_ring_system<BLOCK, 2, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                default: sweep_ring_system<BLOCK, 3, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
             }
         else
             sweep_fast_system<BLOCK, false>(D.g, D.q, D.bd, D.bands, D.o, D.w, it.y, lds_x, D.ee_prev, D.dens);

@@ -49,7 +49,8 @@ struct bte_ctx {
     DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
     DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd, swp_pos;
     int max_levels = 0, lagged = 0;
-    int ring_span[4] = {0, 0, 0, 0};  // max positions spanned by 8, 4, 2, 1 consecutive sweep levels
+    int ring_span[7] = {};             // max positions spanned by 64, 32, ..., 2, 1 consecutive sweep levels
+    int max_back = 0;                  // max over cells of (end of its level) - (upwind position)
     std::vector<int> h_lvl_cnt;  // sweep levels per direction
     DBuf bnd_list;               // level-order entries d*N+i of cells with an incoming boundary face
     int n_bnd_list = 0;
@@ -526,6 +527,7 @@ int prepare(bte_ctx* ctx, bool timing) {
         ctx->max_levels = 0;
         ctx->lagged = 0;
         for (int& r : ctx->ring_span) r = 0;
+        ctx->max_back = 0;
         for (int d = 0; d < D; d++) {
             off[d] = (int)lp_all.size();
             cnt[d] = (int)lptr[d].size() - 1;
@@ -541,12 +543,14 @@ int prepare(bte_ctx* ctx, bool timing) {
                 for (int i = lptr[d][l]; i < lptr[d][l + 1]; i++)
                     for (int k = 0; k < W; k++) {
                         const int j = scol[d][(size_t)k * N + i];
-                        if (j >= 0) spos_all[((size_t)d * W + k) * N + i] = inv[j];
+                        if (j < 0) continue;
+                        spos_all[((size_t)d * W + k) * N + i] = inv[j];
+                        ctx->max_back = std::max(ctx->max_back, lptr[d][l + 1] - inv[j]);
                     }
             // positions spanned by K consecutive levels (the reach the ring needs when the sweep makes
-            // HBM writes visible every K levels), K = 8, 4, 2, 1
-            for (int q = 0; q < 4; q++) {
-                const int K = 8 >> q;
+            // HBM writes visible every K levels), K = 64, 32, ..., 2, 1
+            for (int q = 0; q < 7; q++) {
+                const int K = 64 >> q;
                 for (int l = 0; l + 1 < (int)lptr[d].size(); l++)
                     ctx->ring_span[q] = std::max(ctx->ring_span[q], lptr[d][l + 1] - lptr[d][std::max(0, l - K + 1)]);
             }
@@ -725,15 +729,16 @@ bool lds_fits(const bte_ctx* ctx) { return (size_t)ctx->N * sizeof(double) + 102
 // LDS ring of the pipelined sweep (variant 4): 8192 entries (64 KB; the fewer values are older than
 // the ring, the fewer reads go to HBM), or the smallest power of two >= 1024 that still fits the LDS
 // with the level pointers; it must cover the positions of K consecutive levels, K the largest of
-// {8, 4, 2, 1} that it covers; *sync_every = K.  0 when even one level is wider than the ring.
+// {64, 32, ..., 2, 1} that it covers (the fewer full synchronisations, the longer the prefetches
+// stay in flight); *sync_every = K.  0 when even one level is wider than the ring.
 int ring_size(const bte_ctx* ctx, int* sync_every) {
     if (ctx->lagged != 0) return 0;
     int r = 8192;
     while (r > 1024 && ring_lds_bytes(r, ctx->max_levels) > (size_t)ctx->max_lds) r >>= 1;
     if (ring_lds_bytes(r, ctx->max_levels) > (size_t)ctx->max_lds) return 0;
-    for (int q = 0; q < 4; q++)
+    for (int q = 0; q < 7; q++)
         if (ctx->ring_span[q] > 0 && ctx->ring_span[q] <= r) {
-            if (sync_every) *sync_every = 8 >> q;
+            if (sync_every) *sync_every = 64 >> q;
             return r;
         }
     return 0;
@@ -775,12 +780,14 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 hipLaunchKernelGGL(kernel, grid, dim3(64), lds, ctx->stream, op, w, S, ring - 1);
             };
+            // without reads older than the ring the sweep needs no HBM read-back path
+            const bool fb = ctx->max_back > ring;
             if (ctx->W == 1)
-                go(k_sweep_wave<1>);
+                fb ? go(k_sweep_wave<1, true>) : go(k_sweep_wave<1, false>);
             else if (ctx->W == 2)
-                go(k_sweep_wave<2>);
+                fb ? go(k_sweep_wave<2, true>) : go(k_sweep_wave<2, false>);
             else
-                go(k_sweep_wave<3>);
+                fb ? go(k_sweep_wave<3, true>) : go(k_sweep_wave<3, false>);
         } else if (variant == 4 && ring > 0) {
             ctx->variant_used = 4;
             // right-hand sides in level order into w.b, then the pipelined sweep
@@ -790,12 +797,13 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 hipLaunchKernelGGL(kernel, grid, blk, lds, ctx->stream, op, w, S, ring - 1);
             };
+            const bool fb = ctx->max_back > ring;
             if (ctx->W == 1)
-                go(k_sweep_ring<BLOCK, 1>);
+                fb ? go(k_sweep_ring<BLOCK, 1, true>) : go(k_sweep_ring<BLOCK, 1, false>);
             else if (ctx->W == 2)
-                go(k_sweep_ring<BLOCK, 2>);
+                fb ? go(k_sweep_ring<BLOCK, 2, true>) : go(k_sweep_ring<BLOCK, 2, false>);
             else
-                go(k_sweep_ring<BLOCK, 3>);
+                fb ? go(k_sweep_ring<BLOCK, 3, true>) : go(k_sweep_ring<BLOCK, 3, false>);
         } else if (variant == 3 && lds_fits(ctx)) {
             ctx->variant_used = 3;
             const size_t lds = (size_t)ctx->N * sizeof(double);
