@@ -63,7 +63,9 @@ struct bte_ctx {
     int work_prec = -1;
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
     double* h_margin_pinned = nullptr;
-    double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients
+    double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients, then [129] their dot products
+    int aa_slot = 0;            // history column of the step between aa_launch and aa_finish
+    bool aa_skip_now = false;   // that step follows an accepted correction
     // graphs
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_key = -1;
@@ -323,7 +325,7 @@ int bte_ctx_create(int device, bte_ctx** out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_margin_pinned, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_gamma, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_gamma, 256 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
         delete ctx;
         return BTE_ERR_HIP;
     }
@@ -1076,10 +1078,14 @@ static int load_state(bte_ctx* ctx) {
 
 // One Anderson step after an outer iteration produced g = G(x) in ctx->T: forms the next iterate
 // x in aa_x and the emission density of x for the following iteration.
-static int aa_step(bte_ctx* ctx) {
+// First half of an Anderson step: the state gather, the difference columns and, for a mixing
+// step, the dot products, copied to pinned host memory; nothing waits.  Returns in *finish whether
+// aa_finish must follow (after the stream has been synchronised).
+static int aa_launch(bte_ctx* ctx, bool* finish) {
     const int N = ctx->N, m = ctx->aa_m, S = ctx->B * ctx->D, nb = ctx->aa_nb;
     const int Z = (int)ctx->aa_Z;
     const int gb = grid_for(Z, 256);
+    *finish = false;
     ctx->aa_steps++;
     // g = G(x): the new temperature field and reflecting-boundary intensities
     hipLaunchKernelGGL(k_state_gather, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb, ctx->aa_bcells.as<int>(),
@@ -1116,44 +1122,64 @@ static int aa_step(bte_ctx* ctx) {
         // plain step: x_1 = G(x_0)
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
         ctx->aa_first = false;
-    } else {
-        if (!skip) ctx->aa_order.push_back(slot);
-        const int nq = 2 * m + 1, nblk = 64;
-        hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk, m + 1), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
-                           skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
-                           ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
-        hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
-                           ctx->aa_part.as<double>(), ctx->aa_dots.as<double>());
-        std::vector<double> dots(nq);
-        HIPCHK(hipMemcpyAsync(dots.data(), ctx->aa_dots.p, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (!skip)
-            for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
-        const double ff = dots[2 * m];
-        ctx->aa_last_ff = ff;
-        // divergence guard: restart the history when the residual grew far beyond the best seen
-        if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
-        // mixing coefficients go to the device from pinned memory: the copy runs in stream order and
-        // the buffer is rewritten only after the next step's synchronising read of its dots
-        double* gamma = ctx->h_gamma;
-        for (int j = 0; j < m; j++) gamma[j] = 0.0;
-        if (ff > 1e6 * ctx->aa_best) {
-            ctx->aa_order.clear();
-        } else {
-            const int n = (int)ctx->aa_order.size();
-            std::vector<double> H((size_t)n * n), b(n), y;
-            for (int i = 0; i < n; i++) {
-                b[i] = dots[m + ctx->aa_order[i]];
-                for (int j = 0; j < n; j++) H[(size_t)i * n + j] = ctx->aa_H[(size_t)ctx->aa_order[i] * m + ctx->aa_order[j]];
-            }
-            small_solve(H, b, n, y);
-            for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
-        }
-        HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, Z, m, ctx->aa_gamma.as<double>(), g,
-                           ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(), ctx->aa_x.as<double>());
+        return load_state(ctx);
     }
+    if (!skip) ctx->aa_order.push_back(slot);
+    const int nq = 2 * m + 1, nblk = 64;
+    hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk, m + 1), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
+                       skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
+                       ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
+    hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
+                       ctx->aa_part.as<double>(), ctx->aa_dots.as<double>());
+    HIPCHK(hipMemcpyAsync(ctx->h_gamma + 64, ctx->aa_dots.p, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->aa_slot = slot;
+    ctx->aa_skip_now = skip;
+    *finish = true;
+    return BTE_OK;
+}
+
+// Second half, once the dot products have arrived: the m x m least squares on the host, the mixing
+// coefficients to the device (from pinned memory: the copy runs in stream order, and the buffer is
+// rewritten only after the next step's synchronising read of its dots), the update and the state.
+static int aa_finish(bte_ctx* ctx) {
+    const int m = ctx->aa_m;
+    const int Z = (int)ctx->aa_Z;
+    const int gb = grid_for(Z, 256);
+    const int slot = ctx->aa_slot;
+    const double* dots = ctx->h_gamma + 64;
+    if (!ctx->aa_skip_now)
+        for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
+    const double ff = dots[2 * m];
+    ctx->aa_last_ff = ff;
+    // divergence guard: restart the history when the residual grew far beyond the best seen
+    if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
+    double* gamma = ctx->h_gamma;
+    for (int j = 0; j < m; j++) gamma[j] = 0.0;
+    if (ff > 1e6 * ctx->aa_best) {
+        ctx->aa_order.clear();
+    } else {
+        const int n = (int)ctx->aa_order.size();
+        std::vector<double> H((size_t)n * n), b(n), y;
+        for (int i = 0; i < n; i++) {
+            b[i] = dots[m + ctx->aa_order[i]];
+            for (int j = 0; j < n; j++) H[(size_t)i * n + j] = ctx->aa_H[(size_t)ctx->aa_order[i] * m + ctx->aa_order[j]];
+        }
+        small_solve(H, b, n, y);
+        for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, Z, m, ctx->aa_gamma.as<double>(),
+                       ctx->aa_g.as<double>(), ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(),
+                       ctx->aa_x.as<double>());
     return load_state(ctx);
+}
+
+static int aa_step(bte_ctx* ctx) {
+    bool finish = false;
+    int rc = aa_launch(ctx, &finish);
+    if (rc || !finish) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return aa_finish(ctx);
 }
 
 // ---- learned correction ------------------------------------------------------------------------
@@ -1512,10 +1538,13 @@ static int run_iteration(bte_ctx* ctx, double& m) {
 // Bookkeeping after an outer iteration of ctx produced margin m (ee_final, run_iter already
 // advanced): judge a pending learned correction (a rejected one is undone and the iteration does not
 // count), record the margin, test convergence, then mix the state and propose the next correction.
-static int after_iteration(bte_ctx* ctx, double m, bool& counted) {
+// The parts before and after the state mixing are separate (after_judge, after_mix) so that the
+// uneven batch can issue the mixing steps of all its problems with one synchronisation.
+static int after_judge(bte_ctx* ctx, double m, bool& counted, bool& mix) {
     const bte_opts& o = ctx->run_opts;
     int rc;
     counted = false;
+    mix = false;
     if (ctx->learn_pending) {
         ctx->learn_pending = false;
         bool ok = false;
@@ -1548,12 +1577,26 @@ static int after_iteration(bte_ctx* ctx, double m, bool& counted) {
         ctx->run_converged = true;
         if (o.verbose) std::printf("\n");
     } else if (o.accel == BTE_ACCEL_ANDERSON || o.learned) {
-        if ((rc = aa_step(ctx))) return rc;
-        // corrections at state steps 1, 1 + every, 1 + 2 every, ... until two in a row fail
-        if (o.learned && o.learned_every > 0 && ctx->learn_streak < 2 && (ctx->aa_steps - 1) % o.learned_every == 0)
-            if ((rc = learn_propose(ctx))) return rc;
+        mix = true;
     }
     return BTE_OK;
+}
+
+// After the mixing step: corrections at state steps 1, 1 + every, 1 + 2 every, ... until two in a
+// row fail.
+static int after_mix(bte_ctx* ctx) {
+    const bte_opts& o = ctx->run_opts;
+    if (o.learned && o.learned_every > 0 && ctx->learn_streak < 2 && (ctx->aa_steps - 1) % o.learned_every == 0)
+        return learn_propose(ctx);
+    return BTE_OK;
+}
+
+static int after_iteration(bte_ctx* ctx, double m, bool& counted) {
+    bool mix = false;
+    int rc = after_judge(ctx, m, counted, mix);
+    if (rc || !mix) return rc;
+    if ((rc = aa_step(ctx))) return rc;
+    return after_mix(ctx);
 }
 
 int bte_step(bte_ctx* ctx, int n_iter, int* n_done, int* converged) {
@@ -2027,16 +2070,37 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
         int rc = batch_iteration(b, act, m);
         if (rc) return rc;
         std::vector<int> next;
+        // the bookkeeping of every problem, then the mixing steps of all of them with one
+        // synchronisation for their dot products (every member runs on the batch stream)
+        std::vector<int> fin;
         for (size_t a = 0; a < act.size(); a++) {
             bte_ctx* ctx = b->ctx[act[a]];
             const int par = ctx->run_iter & 1;
             ctx->ee_final = par ? ctx->eeA.as<double>() : ctx->eeB.as<double>();
             ctx->run_iter++;
-            bool counted = false;
-            if ((rc = after_iteration(ctx, m[a], counted))) {
+            bool counted = false, mix = false, finish = false;
+            rc = after_judge(ctx, m[a], counted, mix);
+            if (!rc && mix) rc = aa_launch(ctx, &finish);
+            if (rc) {
                 b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
                 return rc;
             }
+            if (finish) fin.push_back((int)a);
+            if (mix && !finish && (rc = after_mix(ctx))) {
+                b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
+                return rc;
+            }
+        }
+        if (!fin.empty()) BCHK(hipStreamSynchronize(b->stream));
+        for (int a : fin) {
+            bte_ctx* ctx = b->ctx[act[a]];
+            if ((rc = aa_finish(ctx)) || (rc = after_mix(ctx))) {
+                b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
+                return rc;
+            }
+        }
+        for (size_t a = 0; a < act.size(); a++) {
+            bte_ctx* ctx = b->ctx[act[a]];
             if (!ctx->run_converged && (int)ctx->run_margins.size() < o.max_iter) next.push_back(act[a]);
         }
         act.swap(next);
