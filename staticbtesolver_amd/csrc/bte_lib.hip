@@ -1880,6 +1880,7 @@ struct bte_batch {
     std::string err;
     std::vector<bte_ctx*> ctx;
     DBuf desc, items, mptr, mout;
+    std::vector<int> items_act;  // the active problems b->items and b->mptr were built for
     double* h_m = nullptr;  // pinned [K] margins
     int ring = 0, block = 256, n_items = 0, max_levels = 0;
     double seconds = 0;
@@ -1973,7 +1974,7 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
                                geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, buf[par], ctx->ediff.as<double>());
         if (b->ring > 0)  // the pipelined sweep reads its right-hand sides in level order from w.b
             enqueue_rhs_levels(ctx, b->stream, buf[par], ctx->dens.as<double>(), ctx->wb.as<double>());
-        SweepDesc& d = hd[a];
+        SweepDesc& d = hd[a];  // per iteration: the intensity buffers alternate
         d.g = geo_of(ctx);
         d.q = quad_of(ctx);
         d.bd = bnd_of(ctx);
@@ -1982,22 +1983,30 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         d.w = work_of(ctx, buf[1 - par]);
         d.ee_prev = buf[par];
         d.dens = ctx->dens.as<double>();
-        const int S = ctx->B * ctx->D;
-        for (int s = 0; s < S; s++) order.push_back({ctx->h_lvl_cnt[s % ctx->D], make_int2((int)a, s)});
         mp[a] = ctx->margin.as<double>();
     }
-    // longest level chains first, so the last work-groups to start are the short ones
-    std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-    std::vector<int2> items(order.size());
-    for (size_t t = 0; t < order.size(); t++) items[t] = order[t].second;
-    b->n_items = (int)items.size();
+    // the work items change only when a problem leaves the batch
+    std::vector<int2> items;
+    if (act != b->items_act) {
+        for (size_t a = 0; a < act.size(); a++) {
+            const bte_ctx* ctx = b->ctx[act[a]];
+            const int S = ctx->B * ctx->D;
+            for (int s = 0; s < S; s++) order.push_back({ctx->h_lvl_cnt[s % ctx->D], make_int2((int)a, s)});
+        }
+        // longest level chains first, so the last work-groups to start are the short ones
+        std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        items.resize(order.size());
+        for (size_t t = 0; t < order.size(); t++) items[t] = order[t].second;
+        b->n_items = (int)items.size();
+        BCHK(b->items.ensure(sizeof(int2) * items.size()));
+        BCHK(b->mptr.ensure(sizeof(double*) * mp.size()));
+        BCHK(hipMemcpyAsync(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, b->stream));
+        BCHK(hipMemcpyAsync(b->mptr.p, mp.data(), sizeof(double*) * mp.size(), hipMemcpyHostToDevice, b->stream));
+        b->items_act = act;
+    }
     BCHK(b->desc.ensure(sizeof(SweepDesc) * hd.size()));
-    BCHK(b->items.ensure(sizeof(int2) * items.size()));
-    BCHK(b->mptr.ensure(sizeof(double*) * mp.size()));
     BCHK(b->mout.ensure(sizeof(double) * mp.size()));
     BCHK(hipMemcpyAsync(b->desc.p, hd.data(), sizeof(SweepDesc) * hd.size(), hipMemcpyHostToDevice, b->stream));
-    BCHK(hipMemcpyAsync(b->items.p, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, b->stream));
-    BCHK(hipMemcpyAsync(b->mptr.p, mp.data(), sizeof(double*) * mp.size(), hipMemcpyHostToDevice, b->stream));
     switch (b->block) {
         case 64: launch_batch_b<64>(b, b->ring); break;
         case 128: launch_batch_b<128>(b, b->ring); break;
@@ -2065,6 +2074,7 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
     auto t0 = std::chrono::steady_clock::now();
     std::vector<int> act;
     for (int k = 0; k < K; k++) act.push_back(k);
+    b->items_act.clear();  // the members may have changed since the last call
     std::vector<double> m;
     while (!act.empty()) {
         int rc = batch_iteration(b, act, m);
