@@ -69,8 +69,9 @@ typedef struct bte_opts {
                              3 level-ordered operands with the intensity vector held in LDS
                              (needs n_cell * 8 bytes of LDS), 4 pipelined (operands loaded two
                              levels ahead, recent intensities in an LDS ring), 5 as 4 with one
-                             wave per system (no work-group barriers); a form the mesh cannot
-                             use falls back to 2; default 0 */
+                             wave per system (no work-group barriers), 6 / 7 as 4 with 2 / 4 bands
+                             of a direction per work-group sharing each operator load; a form the
+                             mesh cannot use falls back to 2; default 0 */
     int accel;            /* outer iteration: BTE_ACCEL_NONE (the reference's source iteration) or
                              BTE_ACCEL_ANDERSON (Anderson mixing of the temperature field over the
                              last accel_memory iterates); default BTE_ACCEL_NONE */
@@ -117,7 +118,7 @@ typedef struct bte_stats {
     int max_levels;       /* deepest sweep schedule over directions */
     int lagged_edges;     /* upwind edges dropped from the sweep to break cycles */
     int block;            /* work-group size used */
-    int variant;          /* direct-sweep variant used (1..5), 0 for the Krylov paths */
+    int variant;          /* direct-sweep variant used (1..7), 0 for the Krylov paths */
     int band_group;       /* Krylov paths: bands per work-group used (opts.band_group); 0 direct */
 } bte_stats;
 

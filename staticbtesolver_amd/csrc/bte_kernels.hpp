@@ -938,7 +938,7 @@ __device__ __forceinline__ SweepSrc sweep_src(const Ops& o, const Work& w, int s
 // Straight-line loads (W, the incoming-slot count, is a compile-time constant and an absent item
 // loads position 0 and is ignored), so the compiler's wait counting stays exact and a later use of
 // the item waits for its own loads only.  All loads share the lane's two byte offsets.
-template <int W>
+template <int W, bool RHS = true>
 __device__ __forceinline__ void sweep_item_load(int i, const SweepSrc& src, SweepItem& it) {
     it.i = i;
     const unsigned j4 = (unsigned)(i < 0 ? 0 : i) * 4u, j8 = j4 * 2u;
@@ -951,7 +951,7 @@ __device__ __forceinline__ void sweep_item_load(int i, const SweepSrc& src, Swee
     }
     it.dgc = buf_ld<double>(src.sdg, j8, 0);
     it.vw = buf_ld<double>(src.svw, j8, 0);
-    it.rhs = buf_ld<double>(src.rhs, j8, 0);
+    if constexpr (RHS) it.rhs = buf_ld<double>(src.rhs, j8, 0);
 }
 
 // Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order: positions p >= lim from
@@ -1065,6 +1065,97 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int 
     extern __shared__ double ring[];
     for (int s = blockIdx.x; s < S; s += gridDim.x)
         sweep_ring_system<BLOCK, W, FB>(o, w, s, ring, ring_mask);
+}
+
+// Band-batched pipelined sweep (variants 6 and 7): one work-group sweeps G bands of one direction.
+// The bands share the level schedule and the stored operator, so each cell's operator item (cell,
+// upwind positions and cells, coefficients, diagonal face sum, volume weight) is loaded once for
+// all G bands, together with the G right-hand sides; each band keeps its own LDS ring of R values
+// (the rings and then the level pointers fill the dynamic LDS).  Per band the arithmetic, and its
+// order, are those of sweep_ring_system.  A band beyond the last one (a short last group) is
+// skipped.
+template <int BLOCK, int W, int G, bool FB>
+__device__ __forceinline__ void sweep_ring_bands(const Ops& o, const Work& w, int B, int d, int b0, double* ring,
+                                                 int ring_mask) {
+    const int N = o.N;
+    const int tid = threadIdx.x;
+    const int R = ring_mask + 1;
+    const int K = o.ring_sync;
+    int* lps = (int*)(ring + G * R);
+    const int nb = min(G, B - b0);  // bands present in this group
+    double vt[G];
+    gmem<double>* xg[G];
+    __amdgpu_buffer_rsrc_t rhsr[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int s = (b0 + (g < nb ? g : 0)) * o.D + d;
+        vt[g] = o.vt[b0 + (g < nb ? g : 0)];
+        xg[g] = to_global(w.x + (size_t)s * N);
+        rhsr[g] = buffer_of(w.b + (size_t)s * N, (size_t)N * 8);
+    }
+    const SweepSrc src = sweep_src<W>(o, w, b0 * o.D + d, d);
+    const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
+    const int L = o.lvl_cnt[d];
+    for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
+    __syncthreads();
+    auto first = [&](int l) {  // this thread's first-pass position in level l, or -1
+        if (l >= L) return -1;
+        const int i = lps[l] + tid;
+        return i < lps[l + 1] ? i : -1;
+    };
+    auto load = [&](int i, SweepItem& it, double (&rhs)[G]) {
+        sweep_item_load<W, false>(i, src, it);
+        const unsigned j8 = (unsigned)(i < 0 ? 0 : i) * 8u;
+#pragma unroll
+        for (int g = 0; g < G; g++) rhs[g] = buf_ld<double>(rhsr[g], j8, 0);
+    };
+    auto solve_cell = [&](const SweepItem& it, const double (&rhs)[G], int end) {
+#pragma unroll
+        for (int g = 0; g < G; g++)
+            if (g < nb) {
+                double* rg = ring + g * R;
+                const double acc = ring_upwind_sum<FB>(it, end - R, rg, ring_mask, xg[g]);
+                const double xv = (rhs[g] - vt[g] * acc) / (vt[g] * it.dgc + it.vw);
+                rg[it.i & ring_mask] = xv;
+                xg[g][it.c] = xv;
+            }
+    };
+    // as in sweep_ring_system: three register sets rotate by unrolling, never by copies
+    auto level = [&](int l, SweepItem& cur, double (&rcur)[G], SweepItem& ahead, double (&rahead)[G]) {
+        const int beg = lps[l], end = lps[l + 1];
+        load(first(l + 2), ahead, rahead);
+        if (cur.i >= 0) solve_cell(cur, rcur, end);
+        for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
+            SweepItem E;
+            double rE[G];
+            load(i, E, rE);
+            solve_cell(E, rE, end);
+        }
+        if (FB && (l + 1) % K == 0)
+            __syncthreads();  // HBM writes of the levels so far complete and visible
+        else
+            lds_barrier();
+    };
+    SweepItem X0, X1, X2;
+    double r0[G], r1[G], r2[G];
+    load(first(0), X0, r0);
+    load(first(1), X1, r1);
+    X2.i = -1;
+    for (int l = 0; l < L; l += 3) {
+        level(l, X0, r0, X2, r2);
+        if (l + 1 < L) level(l + 1, X1, r1, X0, r0);
+        if (l + 2 < L) level(l + 2, X2, r2, X1, r1);
+    }
+    __syncthreads();  // the rings and level pointers are reused by the next group of this work-group
+    if (tid < nb) w.iters[(b0 + tid) * o.D + d] += 1;
+}
+
+template <int BLOCK, int W, int G, bool FB>
+__global__ __launch_bounds__(BLOCK) void k_sweep_ring_bands(Ops o, Work w, int B, int ring_mask) {
+    extern __shared__ double ring[];
+    const int ngroups = o.D * ((B + G - 1) / G);
+    for (int gid = blockIdx.x; gid < ngroups; gid += gridDim.x)
+        sweep_ring_bands<BLOCK, W, G, FB>(o, w, B, gid % o.D, (gid / o.D) * G, ring, ring_mask);
 }
 
 // Single-wave form of the pipelined sweep (variant 5): one 64-lane wave per system, IPL cells of a

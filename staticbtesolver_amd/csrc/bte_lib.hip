@@ -647,7 +647,7 @@ int ensure_work(bte_ctx* ctx, int precision) {
     return BTE_OK;
 }
 
-int ring_size(const bte_ctx* ctx, int* sync_every = nullptr);
+int ring_size(const bte_ctx* ctx, int* sync_every = nullptr, int G = 1);
 
 Ops ops_of(bte_ctx* ctx) {
     Ops o{};
@@ -733,11 +733,13 @@ bool lds_fits(const bte_ctx* ctx) { return (size_t)ctx->N * sizeof(double) + 102
 // with the level pointers; it must cover the positions of K consecutive levels, K the largest of
 // {64, 32, ..., 2, 1} that it covers (the fewer full synchronisations, the longer the prefetches
 // stay in flight); *sync_every = K.  0 when even one level is wider than the ring.
-int ring_size(const bte_ctx* ctx, int* sync_every) {
+// With G bands per work-group (variants 6, 7) each band has a ring of 8192 / G entries (the same
+// LDS in total), at least 1024.
+int ring_size(const bte_ctx* ctx, int* sync_every, int G) {
     if (ctx->lagged != 0) return 0;
-    int r = 8192;
-    while (r > 1024 && ring_lds_bytes(r, ctx->max_levels) > (size_t)ctx->max_lds) r >>= 1;
-    if (ring_lds_bytes(r, ctx->max_levels) > (size_t)ctx->max_lds) return 0;
+    int r = 8192 / G;
+    while (r > 1024 && ring_lds_bytes(G * r, ctx->max_levels) > (size_t)ctx->max_lds) r >>= 1;
+    if (r < 1024 || ring_lds_bytes(G * r, ctx->max_levels) > (size_t)ctx->max_lds) return 0;
     for (int q = 0; q < 7; q++)
         if (ctx->ring_span[q] > 0 && ctx->ring_span[q] <= r) {
             if (sync_every) *sync_every = 64 >> q;
@@ -790,6 +792,36 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                 fb ? go(k_sweep_wave<2, true>) : go(k_sweep_wave<2, false>);
             else
                 fb ? go(k_sweep_wave<3, true>) : go(k_sweep_wave<3, false>);
+        } else if ((variant == 6 || variant == 7) && ctx->B >= (variant == 6 ? 2 : 4) &&
+                   ring_size(ctx, nullptr, variant == 6 ? 2 : 4) > 0) {
+            // G bands of a direction per work-group, each with its own ring of 8192 / G entries
+            const int G = variant == 6 ? 2 : 4;
+            int sync_g = 1;
+            const int ring_g = ring_size(ctx, &sync_g, G);
+            ctx->variant_used = variant;
+            enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b);
+            Ops opg = op;
+            opg.ring_sync = sync_g;
+            const size_t lds = ring_lds_bytes(G * ring_g, ctx->max_levels);
+            const dim3 gg(ctx->D * ((ctx->B + G - 1) / G));
+            auto go = [&](auto kernel) {
+                (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                hipLaunchKernelGGL(kernel, gg, blk, lds, ctx->stream, opg, w, ctx->B, ring_g - 1);
+            };
+            const bool fb = ctx->max_back > ring_g;
+            auto by_w = [&](auto gc) {
+                constexpr int GG = decltype(gc)::value;
+                if (ctx->W == 1)
+                    fb ? go(k_sweep_ring_bands<BLOCK, 1, GG, true>) : go(k_sweep_ring_bands<BLOCK, 1, GG, false>);
+                else if (ctx->W == 2)
+                    fb ? go(k_sweep_ring_bands<BLOCK, 2, GG, true>) : go(k_sweep_ring_bands<BLOCK, 2, GG, false>);
+                else
+                    fb ? go(k_sweep_ring_bands<BLOCK, 3, GG, true>) : go(k_sweep_ring_bands<BLOCK, 3, GG, false>);
+            };
+            if (G == 2)
+                by_w(Unroll<2>{});
+            else
+                by_w(Unroll<4>{});
         } else if (variant == 4 && ring > 0) {
             ctx->variant_used = 4;
             // right-hand sides in level order into w.b, then the pipelined sweep
@@ -907,6 +939,8 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
             if (lds_fits(ctx)) cand.push_back({blk, 3});
             if (ring_size(ctx) > 0) cand.push_back({blk, 4});
             if (blk == 64 && ring_size(ctx) > 0) cand.push_back({64, 5});  // one wave per system
+            if (blk >= 128 && ctx->B >= 2 && ring_size(ctx, nullptr, 2) > 0) cand.push_back({blk, 6});
+            if (blk >= 128 && ctx->B >= 4 && ring_size(ctx, nullptr, 4) > 0) cand.push_back({blk, 7});
         } else if (blk >= 128) {
             for (int G : {1, 2, 4}) {
                 const bool allowed = (o->band_group == 0) ? (G == 1 || (G <= ctx->B && group_ok(ctx, o, G, blk)))
@@ -1446,6 +1480,8 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     ctx->block_used = shape.block;
     // the sweep variant that actually runs (a requested form the mesh cannot use falls back to 2)
     ctx->variant_used = !direct_path(ctx, &o)                              ? 0
+                        : (shape.variant == 6 && ctx->B >= 2 && ring_size(ctx, nullptr, 2) > 0) ? 6
+                        : (shape.variant == 7 && ctx->B >= 4 && ring_size(ctx, nullptr, 4) > 0) ? 7
                         : (shape.variant == 5 && ring_size(ctx) > 0)      ? 5
                         : (shape.variant == 4 && ring_size(ctx) > 0)      ? 4
                         : (shape.variant == 3 && lds_fits(ctx))           ? 3

@@ -123,7 +123,7 @@ def test_sweep_variants_and_block_sizes_agree(name):
     say("variants", name)
     c = capi.Case(**case_kwargs(name))
     base = c.solve(sweep_variant=1, block=256, stop_margin=1e-4)
-    for variant in (2, 3, 4, 5):
+    for variant in (2, 3, 4, 5, 6, 7):
         for block in (64, 128, 256, 512):
             r = c.solve(sweep_variant=variant, block=block, stop_margin=1e-4)
             st = c.stats()
@@ -133,14 +133,16 @@ def test_sweep_variants_and_block_sizes_agree(name):
             np.testing.assert_allclose(r["flux"], base["flux"], rtol=1e-9, atol=1e-12 * np.abs(base["flux"]).max())
     r = c.solve(stop_margin=1e-4)  # autotuned
     st = c.stats()
-    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3, 4, 5)
+    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3, 4, 5, 6, 7)
     np.testing.assert_allclose(r["T"], base["T"], rtol=1e-13)
 
 
 @pytest.mark.parametrize("variant,block,n_cells", [(4, 64, 20000), (4, 256, 20000), (5, 64, 20000),
-                                                   (4, 256, 50000), (5, 64, 50000)])
+                                                   (4, 256, 50000), (5, 64, 50000), (6, 128, 20000),
+                                                   (6, 256, 50000), (7, 256, 20000), (7, 512, 50000)])
 def test_ring_sweep_matches_level_sweep(tmp_path, variant, block, n_cells):
-    """The pipelined LDS-ring sweeps (variant 4: work-group per system; 5: one wave per system)
+    """The pipelined LDS-ring sweeps (variant 4: work-group per system; 5: one wave per system;
+    6 / 7: 2 / 4 bands of a direction per work-group, 10 bands so the last group of 4 is short)
     perform the level-ordered sweep's arithmetic in the same order: identical iterates, including
     levels wider than the pipelined lanes (finished unpipelined) and, at 50k cells, upwind values
     older than the ring (read back from HBM)."""
