@@ -179,6 +179,27 @@ def spmv_bytes(case, w=8):
     return float(per_dir.sum() * case.N_band)
 
 
+def alg_bytes(case, w, direct, per_sys=1.0, jacobi=False):
+    """Algorithmic bytes of one inner-solve launch (all systems), and the per-system canonical count.
+
+    The stored operator of a direction serves all its bands (the band enters as the scalar
+    v_b tau_b), so a product of all systems needs each direction's operator once:
+    B_shared = sum_d [nnz_d (w+4) + 4 (n+1)] + sum_(b,d) 2 w n.  An exact sweep moves one product's
+    worth (operator, right-hand side in, intensities out): B_shared.  A BiCGSTAB iteration: 2 B_shared
+    + 21 w n per system (the fused vector passes' lower bound) + the Jacobi preconditioner's per-band
+    diagonal reads 2 (w + 8) n, times the mean iterations per system.  The canonical count charges
+    every system its own operator (B_spmv per system)."""
+    n, S = case.N_cell, case.N_band * case.N_dir
+    canon = spmv_bytes(case, w)  # sum over systems of B_spmv
+    vec = 2.0 * w * n * S
+    op_dir = canon / case.N_band - 2.0 * w * n * case.N_dir  # every direction's operator once
+    shared = op_dir + vec
+    if direct:
+        return shared, canon
+    jac = 2.0 * (w + 8) * n * S if jacobi else 0.0
+    return ((2.0 * shared + 21.0 * w * n * S + jac) * per_sys, (2.0 * canon + 21.0 * w * n * S) * per_sys)
+
+
 def cpu_baseline(kw, outer_iters, budget_s=15.0):
     """PETSc-default restatement (GMRES(30) + ILU(0), rtol 1e-9) timed on a bounded sample: the
     first outer iteration's systems, taken in order until budget_s of CPU time, then extrapolated
@@ -201,17 +222,10 @@ def cpu_baseline(kw, outer_iters, budget_s=15.0):
 
 def roofline_only(case, form, k, prec):
     """K outer iterations with per-kernel hipEvents; roofline of the inner solve (all kernels of
-    the solve step: right-hand sides, inner solves and, in the mixed form, FP64 re-solves).
-
-    Algorithmic bytes per launch.  The stored operator of a direction is shared by its bands (the
-    band enters as the scalar v_b tau_b), so a product of all systems needs the operator of each
-    direction once: B_shared = sum_d [nnz_d (w+4) + 4 (n+1)] + sum_(b,d) 2 w n.  Per BiCGSTAB
-    iteration of all systems: 2 B_shared + 21 w n per system (the fused vector passes' lower
-    bound) + the per-band diagonal reads of the Jacobi preconditioner, 2 (w + 8) n per system;
-    times the mean iterations per system (FP64 re-solve iterations of the mixed form counted at
-    w = 4: a lower bound).  For an exact sweep: 2 B_spmv per system.  achieved_canonical charges
-    every system its own operator (2 B_spmv + 21 w n per iteration); with the operator shared
-    through the caches it can exceed the HBM traffic actually moved (rocprofv3 counters)."""
+    the solve step: right-hand sides, inner solves and, in the mixed form, FP64 re-solves), with
+    the algorithmic bytes of alg_bytes (FP64 re-solve iterations of the mixed form counted at
+    w = 4: a lower bound).  achieved_canonical charges every system its own operator; with the
+    operator shared through the caches it can exceed the HBM traffic actually moved."""
     from staticbtesolver_amd import capi
 
     f = dict(form, max_iter=k, stop_margin=0.0, timing=1)
@@ -224,16 +238,7 @@ def roofline_only(case, form, k, prec):
     w = 8 if prec == capi.PREC_FP64 else 4
     direct = st["variant"] > 0
     G = max(1, st.get("band_group", 1))
-    n = case.N_cell
-    canon = spmv_bytes(case, w)           # sum over systems of B_spmv
-    vec = 2.0 * w * n * S                 # the products' x and y of every system
-    op_dir = canon / case.N_band - 2.0 * w * n * case.N_dir  # the operator of every direction once
-    if direct:
-        b_alg = b_canon = 2.0 * canon
-    else:
-        jac = 2.0 * (w + 8) * n * S if form.get("precond") == capi.PC_JACOBI else 0.0
-        b_alg = (2.0 * (op_dir + vec) + 21.0 * w * n * S + jac) * per_sys
-        b_canon = (2.0 * canon + 21.0 * w * n * S) * per_sys
+    b_alg, b_canon = alg_bytes(case, w, direct, per_sys, form.get("precond") == capi.PC_JACOBI)
     achieved = b_alg / (solve_ms * 1e-3) / 1e9
     return {"roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel_ms": round(solve_ms, 4),
@@ -382,7 +387,7 @@ def main():
     # per launch: one exact sweep per system (FP64 direct form) or the refinement loop's products
     per_sys = rt["inner_iters"] / max(1, st["calls"]["solve"]) / (case.N_band * case.N_dir)
     w = 8 if prec == capi.PREC_FP64 else 4
-    b_alg = 2.0 * spmv_bytes(case, w) * max(1.0, per_sys)
+    b_alg, b_canon = alg_bytes(case, w, st["variant"] > 0, per_sys, form.get("precond") == capi.PC_JACOBI)
     achieved = b_alg / (solve_ms * 1e-3) / 1e9
     cmp = dict(form=args.form, precision=args.precision, target_rel_err=args.tol,
                time_to_target_s=round(tta, 5), iterations_to_target=last["tta_iters"],
@@ -431,6 +436,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": prof["bytes"] if prof else None,
                      "kernel": "inner solve", "kernel_ms": round(solve_ms, 5), "alg_bytes_per_launch": b_alg,
+                     "alg_bytes_formula": "B_shared: each direction's operator once + 2 w n per system",
+                     "achieved_canonical": round(b_canon / (solve_ms * 1e-3) / 1e9, 2),
                      "traffic_source": prof},
         "cpu_baseline": cpu,
     }
