@@ -294,7 +294,14 @@ struct Ops {
     const double* swp_vw;          // [D*N] w_d V_c in level order
     const unsigned char* swp_bnd;  // [D*N] 1 if the cell has an incoming boundary face
     const int* swp_pos;   // [(d*W+k)*N + i] level-order position of the upwind neighbour (-1: none)
-    int ring_sync;        // pipelined sweep: make HBM writes visible to the work-group every ring_sync levels
+    int ring_sync;        // band-batched pipelined sweeps: HBM writes made visible every ring_sync levels
+    // work-group-per-system pipelined sweep (ring of ring_r values): lvl_sync[lvl_off[d] + l] = 1 when
+    // the sweep of direction d must make its HBM writes visible after level l (a later level reads,
+    // from HBM, a value older than the ring written at or before l); dir_fb[d] = 1 when direction d
+    // has such reads at all
+    const unsigned char* lvl_sync;
+    const int* dir_fb;
+    int ring_r;
 };
 
 struct Work {
@@ -968,12 +975,19 @@ __device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, 
 #pragma unroll
     for (int k = 0; k < 3; k++) v[k] = ring[it.p[k] & ring_mask];
     if constexpr (FB) {
+        // one wave-uniform test: the HBM path is rare and its per-slot branches cost more than the
+        // arithmetic when they are taken around on every cell
+        bool old = false;
 #pragma unroll
-        for (int k = 0; k < 3; k++)
-            if (it.p[k] >= 0 && it.p[k] < lim) {
-                const double h = xg[it.col[k]];
-                asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
-            }
+        for (int k = 0; k < 3; k++) old |= it.p[k] >= 0 && it.p[k] < lim;
+        if (__ballot(old)) {
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (it.p[k] >= 0 && it.p[k] < lim) {
+                    const double h = xg[it.col[k]];
+                    asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
+                }
+        }
     }
     double acc = 0;
 #pragma unroll
@@ -994,19 +1008,18 @@ __host__ __device__ inline size_t ring_lds_bytes(int ring, int max_levels) {
 // One system of the pipelined sweep; needs rhs_lo (k_rhs_levels) in w.b.  ring holds the LDS ring
 // of R = ring_mask + 1 values followed by room for the direction's level pointers.  Per thread every
 // operand of a level -- including its right-hand side -- is a coalesced load in level order, issued
-// two levels ahead; the level pointers are read from LDS.  An upwind value at position
-// p >= end(l) - R is read from the ring (its slot is not yet reused).  An older one is read from HBM:
-// every o.ring_sync levels the work-group synchronises with its HBM writes complete, and the host
-// sizes R to cover o.ring_sync consecutive levels, so such a value was written before the last of
-// those synchronisations.  In between, levels synchronise on the LDS only, leaving the prefetches in
-// flight.  FB = false when the host found every upwind position within R of its level's end: no
-// HBM read-back path and no periodic full synchronisation at all.
-template <int BLOCK, int W, bool FB>
+// PD levels ahead.  An upwind value at position p >= end(l) - R is read from the ring (its slot is
+// not yet reused).  An older one is read from HBM: the host flags (o.lvl_sync) the levels after
+// which the work-group synchronises with its HBM writes complete, chosen so that one such
+// synchronisation lies between the write of every value read this way and its read.  All other
+// levels synchronise on the LDS only, leaving the prefetches in flight.  FB = false: no direction
+// of the launch has such reads (no HBM read-back path, no full synchronisation); with FB = true a
+// direction without them (o.dir_fb[d] = 0) still runs the FB = false code.
+template <int BLOCK, int W, bool FB, int PD = 3>
 __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
     const int N = o.N;
     const int tid = threadIdx.x;
     const int R = ring_mask + 1;
-    const int K = o.ring_sync;
     int* lps = (int*)(ring + R);
     {
         const int b = s / o.D, d = s % o.D;
@@ -1015,12 +1028,34 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         const SweepSrc src = sweep_src<W>(o, w, s, d);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
-        for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
+        // level pointers, with the synchronisation flag of level t in bit 31 of entry t
+        if constexpr (FB) {
+            const unsigned char* __restrict__ fl = o.lvl_sync + o.lvl_off[d];
+            for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t] | (t < L && fl[t] ? (int)0x80000000 : 0);
+        } else {
+            for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
+        }
         __syncthreads();
+        // Level pointers lps[wb .. wb + 63] are held one per lane and read with readlane (a register
+        // read, no LDS round trip on the level's dependency chain), their flags as one 64-bit mask;
+        // the window moves before a level would need a pointer beyond it (level l reads l .. l+PD+1).
+        const int lane = tid & 63;
+        int wb = 0, win = 0;
+        unsigned long long syncs = 0;
+        auto load_window = [&](int from) {
+            wb = from;
+            const int v = lps[min(from + lane, L)];
+            win = v & 0x7fffffff;
+            if constexpr (FB) syncs = __ballot(v < 0);
+        };
+        load_window(0);
+        // the flags hold for rings of at least o.ring_r values; a smaller ring synchronises every level
+        const bool every = R < o.ring_r;
+        auto lp_at = [&](int l) { return __builtin_amdgcn_readlane(win, l - wb); };
         auto first = [&](int l) {  // this thread's first-pass position in level l, or -1
             if (l >= L) return -1;
-            const int i = lps[l] + tid;
-            return i < lps[l + 1] ? i : -1;
+            const int i = lp_at(l) + tid;
+            return i < lp_at(l + 1) ? i : -1;
         };
         auto solve_cell = [&](const SweepItem& it, int end) {
             const double acc = ring_upwind_sum<FB>(it, end - R, ring, ring_mask, xg);
@@ -1028,43 +1063,58 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             ring[it.i & ring_mask] = xv;
             xg[it.c] = xv;
         };
-        // level l computes from the item loaded two levels earlier and loads the item of level l+2
-        // into the register set it just freed; the three sets rotate by unrolling, never by copies
+        // level l computes from the item loaded PD levels earlier and loads the item of level l+PD
+        // into the register set it just freed; the PD + 1 sets rotate by unrolling, never by copies
         // (a copy of a register with a load in flight would wait for that load)
         auto level = [&](int l, SweepItem& cur, SweepItem& ahead) {
-            const int beg = lps[l], end = lps[l + 1];
-            sweep_item_load<W>(first(l + 2), src, ahead);
+            if (l + PD + 1 - wb > 63) load_window(l);
+            const int beg = lp_at(l), end = lp_at(l + 1);
+            sweep_item_load<W>(first(l + PD), src, ahead);
             if (cur.i >= 0) solve_cell(cur, end);
             for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
                 SweepItem E;
                 sweep_item_load<W>(i, src, E);
                 solve_cell(E, end);
             }
-            if (FB && (l + 1) % K == 0)
+            if (FB && (every || ((syncs >> (l - wb)) & 1ull)))
                 __syncthreads();  // HBM writes of the levels so far complete and visible
             else
                 lds_barrier();
         };
-        SweepItem X0, X1, X2;
-        sweep_item_load<W>(first(0), src, X0);
-        sweep_item_load<W>(first(1), src, X1);
-        X2.i = -1;
-        for (int l = 0; l < L; l += 3) {
-            level(l, X0, X2);
-            if (l + 1 < L) level(l + 1, X1, X0);
-            if (l + 2 < L) level(l + 2, X2, X1);
+        SweepItem X[PD + 1];
+#pragma unroll
+        for (int k = 0; k < PD; k++) sweep_item_load<W>(first(k), src, X[k]);
+        X[PD].i = -1;
+        // Steady state: every copy of the unrolled body runs, so on every path into a level the
+        // compiler sees its item loaded PD levels earlier and waits for those loads only (a copy
+        // that may be skipped inside the loop would make it wait for the last level's loads too).
+        int l = 0;
+        for (; l + PD + 1 <= L; l += PD + 1) {
+#pragma unroll
+            for (int k = 0; k <= PD; k++) level(l + k, X[k], X[(k + PD) % (PD + 1)]);
         }
+#pragma unroll
+        for (int k = 0; k < PD; k++)
+            if (l + k < L) level(l + k, X[k], X[(k + PD) % (PD + 1)]);
         __syncthreads();  // the ring and level pointers are reused by the next system of this work-group
         if (tid == 0) w.iters[s] += 1;
     }
 }
 
+// FB = true: the HBM read-back path only for a direction that has such reads
+template <int BLOCK, int W, bool FB, int PD = 3>
+__device__ __forceinline__ void sweep_ring_dispatch(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
+    if (FB && (o.dir_fb[s % o.D] || ring_mask + 1 < o.ring_r))
+        sweep_ring_system<BLOCK, W, true, PD>(o, w, s, ring, ring_mask);
+    else
+        sweep_ring_system<BLOCK, W, false, PD>(o, w, s, ring, ring_mask);
+}
+
 // W = incoming slots per cell (F - 1: 1 in 1D, 2 in 2D, 3 in 3D)
-template <int BLOCK, int W, bool FB>
+template <int BLOCK, int W, bool FB, int PD = 3>
 __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
-    for (int s = blockIdx.x; s < S; s += gridDim.x)
-        sweep_ring_system<BLOCK, W, FB>(o, w, s, ring, ring_mask);
+    for (int s = blockIdx.x; s < S; s += gridDim.x) sweep_ring_dispatch<BLOCK, W, FB, PD>(o, w, s, ring, ring_mask);
 }
 
 // Band-batched pipelined sweep (variants 6 and 7): one work-group sweeps G bands of one direction.
@@ -1161,8 +1211,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_ring_bands(Ops o, Work w, int B
 // Single-wave form of the pipelined sweep (variant 5): one 64-lane wave per system, IPL cells of a
 // level per lane, all loaded two levels ahead.  A wave's LDS instructions execute in order, so the
 // ring needs no work-group barrier between levels: waiting for the wave's own LDS writes is enough.
-// Values older than the ring come from HBM under the same rule as sweep_ring_system (every
-// o.ring_sync levels the wave waits for its HBM writes).  Levels wider than 64 * IPL finish their
+// Values older than the ring come from HBM under the same rule as sweep_ring_system (after the
+// levels the host flags the wave waits for its HBM writes).  Levels wider than 64 * IPL finish their
 // extra cells unpipelined.  Same arithmetic, in the same order, as the other sweeps.
 __device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
@@ -1171,7 +1221,6 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     const int N = o.N;
     const int lane = threadIdx.x;
     const int R = ring_mask + 1;
-    const int K = o.ring_sync;
     int* lps = (int*)(ring + R);
     const int b = s / o.D, d = s % o.D;
     const double vt = o.vt[b];
@@ -1179,12 +1228,29 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     const SweepSrc src = sweep_src<W>(o, w, s, d);
     const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
     const int L = o.lvl_cnt[d];
-    for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
+    // level pointers and synchronisation flags as in sweep_ring_system
+    if constexpr (FB) {
+        const unsigned char* __restrict__ fl = o.lvl_sync + o.lvl_off[d];
+        for (int t = lane; t <= L; t += 64) lps[t] = lp[t] | (t < L && fl[t] ? (int)0x80000000 : 0);
+    } else {
+        for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
+    }
     __syncthreads();
+    int wb = 0, win = 0;
+    unsigned long long syncs = 0;
+    auto load_window = [&](int from) {
+        wb = from;
+        const int v = lps[min(from + lane, L)];
+        win = v & 0x7fffffff;
+        if constexpr (FB) syncs = __ballot(v < 0);
+    };
+    load_window(0);
+    const bool every = R < o.ring_r;
+    auto lp_at = [&](int l) { return __builtin_amdgcn_readlane(win, l - wb); };
     auto first = [&](int l, int k) {
         if (l >= L) return -1;
-        const int i = lps[l] + lane + 64 * k;
-        return i < lps[l + 1] ? i : -1;
+        const int i = lp_at(l) + lane + 64 * k;
+        return i < lp_at(l + 1) ? i : -1;
     };
     auto solve_cell = [&](const SweepItem& it, int end) {
         const double acc = ring_upwind_sum<FB>(it, end - R, ring, ring_mask, xg);
@@ -1193,7 +1259,8 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         xg[it.c] = xv;
     };
     auto level = [&](int l, SweepItem (&cur)[IPL], SweepItem (&ahead)[IPL]) {
-        const int beg = lps[l], end = lps[l + 1];
+        if (l + 3 - wb > 63) load_window(l);
+        const int beg = lp_at(l), end = lp_at(l + 1);
 #pragma unroll
         for (int k = 0; k < IPL; k++)
             sweep_item_load<W>(first(l + 2, k), src, ahead[k]);
@@ -1205,7 +1272,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
             sweep_item_load<W>(i, src, E);
             solve_cell(E, end);
         }
-        if (FB && (l + 1) % K == 0)
+        if (FB && (every || ((syncs >> (l - wb)) & 1ull)))
             __syncthreads();  // HBM writes so far complete
         else
             wave_lds_sync();
@@ -1217,11 +1284,15 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         sweep_item_load<W>(first(1, k), src, X1[k]);
         X2[k].i = -1;
     }
-    for (int l = 0; l < L; l += 3) {
+    // unconditional steady-state body, tail peeled (as in sweep_ring_system)
+    int l = 0;
+    for (; l + 3 <= L; l += 3) {
         level(l, X0, X2);
-        if (l + 1 < L) level(l + 1, X1, X0);
-        if (l + 2 < L) level(l + 2, X2, X1);
+        level(l + 1, X1, X0);
+        level(l + 2, X2, X1);
     }
+    if (l < L) level(l, X0, X2);
+    if (l + 1 < L) level(l + 1, X1, X0);
     __syncthreads();
     if (lane == 0) w.iters[s] += 1;
 }
@@ -1229,7 +1300,12 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
 template <int W, bool FB>
 __global__ __launch_bounds__(64) void k_sweep_wave(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
-    for (int s = blockIdx.x; s < S; s += gridDim.x) sweep_wave_system<W, 2, FB>(o, w, s, ring, ring_mask);
+    for (int s = blockIdx.x; s < S; s += gridDim.x) {
+        if (FB && (o.dir_fb[s % o.D] || ring_mask + 1 < o.ring_r))
+            sweep_wave_system<W, 2, true>(o, w, s, ring, ring_mask);
+        else
+            sweep_wave_system<W, 2, false>(o, w, s, ring, ring_mask);
+    }
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1257,9 +1333,9 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_batch(const SweepDesc* __restri
         const SweepDesc& D = desc[it.x];
         if constexpr (RING)
             switch (D.o.W) {
-                case 1: sweep_ring_system<BLOCK, 1, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
-                case 2: sweep_ring_system<BLOCK, 2, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
-                default: sweep_ring_system<BLOCK, 3, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                case 1: sweep_ring_dispatch<BLOCK, 1, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                case 2: sweep_ring_dispatch<BLOCK, 2, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
+                default: sweep_ring_dispatch<BLOCK, 3, true>(D.o, D.w, it.y, lds_x, ring_mask); break;
             }
         else
             sweep_fast_system<BLOCK, false>(D.g, D.q, D.bd, D.bands, D.o, D.w, it.y, lds_x, D.ee_prev, D.dens);

@@ -48,6 +48,8 @@ struct bte_ctx {
     int W = 0;
     DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
     DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd, swp_pos;
+    DBuf lvl_sync, dir_fb;             // pipelined sweep: synchronisation levels, directions with HBM reads
+    int ring_r = 0;                    // ring size those were computed for (ring_size(ctx), 0: none)
     int max_levels = 0, lagged = 0;
     int ring_span[7] = {};             // max positions spanned by 64, 32, ..., 2, 1 consecutive sweep levels
     int max_back = 0;                  // max over cells of (end of its level) - (upwind position)
@@ -440,6 +442,9 @@ int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const
 
 namespace {
 
+// LDS ring size of the pipelined sweeps (G bands per work-group), 0 when they cannot run
+int ring_size(const bte_ctx* ctx, int* sync_every = nullptr, int G = 1);
+
 // One-time preparation: boundary tables, assembly, sweep schedule, FP32 copies.
 int prepare(bte_ctx* ctx, bool timing) {
     if (!ctx->have_geo || !ctx->have_quad || !ctx->have_bands || !ctx->have_bcs)
@@ -557,6 +562,45 @@ int prepare(bte_ctx* ctx, bool timing) {
                     ctx->ring_span[q] = std::max(ctx->ring_span[q], lptr[d][l + 1] - lptr[d][std::max(0, l - K + 1)]);
             }
         }
+        // Synchronisation levels of the work-group-per-system pipelined sweep with its ring of
+        // ring_size(ctx) values: an upwind value older than the ring is read from HBM, so its write
+        // must complete before a full work-group synchronisation that precedes the read, i.e. at
+        // the end of some level in [level written, level read - 1].  Greedy interval stabbing
+        // (intervals by their last level) flags the fewest level ends that cover every such read.
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
+            ctx->max_lds = lds;
+        ctx->ring_r = ring_size(ctx);
+        std::vector<unsigned char> sync_all(lp_all.size(), 0);
+        std::vector<int> fb_dir(D, 0);
+        if (ctx->ring_r > 0) {
+            std::vector<int> lv_of(N);
+            std::vector<std::pair<int, int>> iv;  // (last, first) level end that may synchronise
+            for (int d = 0; d < D; d++) {
+                const std::vector<int>& P = lptr[d];
+                const int L = (int)P.size() - 1;
+                for (int l = 0; l < L; l++)
+                    for (int i = P[l]; i < P[l + 1]; i++) lv_of[i] = l;
+                iv.clear();
+                for (int l = 0; l < L; l++)
+                    for (int i = P[l]; i < P[l + 1]; i++)
+                        for (int k = 0; k < W; k++) {
+                            const int p = spos_all[((size_t)d * W + k) * N + i];
+                            if (p >= 0 && p < P[l + 1] - ctx->ring_r) iv.push_back({l - 1, lv_of[p]});
+                        }
+                if (iv.empty()) continue;
+                fb_dir[d] = 1;
+                std::sort(iv.begin(), iv.end());
+                int last = -1;
+                for (const auto& e : iv)
+                    if (last < e.second) {
+                        last = e.first;
+                        sync_all[off[d] + last] = 1;
+                    }
+            }
+        }
+        HIPCHK(upload(ctx->lvl_sync, sync_all, ctx->stream));
+        HIPCHK(upload(ctx->dir_fb, fb_dir, ctx->stream));
         ctx->h_lvl_cnt = cnt;
         HIPCHK(upload(ctx->lvl_off, off, ctx->stream));
         HIPCHK(upload(ctx->lvl_cnt, cnt, ctx->stream));
@@ -584,9 +628,6 @@ int prepare(bte_ctx* ctx, bool timing) {
             if (hb[t]) blist.push_back((int)t);
         ctx->n_bnd_list = (int)blist.size();
         HIPCHK(upload(ctx->bnd_list, blist, ctx->stream));
-        int lds = 0;
-        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
-            ctx->max_lds = lds;
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     ctx->have_sweep = true;
@@ -647,7 +688,6 @@ int ensure_work(bte_ctx* ctx, int precision) {
     return BTE_OK;
 }
 
-int ring_size(const bte_ctx* ctx, int* sync_every = nullptr, int G = 1);
 
 Ops ops_of(bte_ctx* ctx) {
     Ops o{};
@@ -674,6 +714,9 @@ Ops ops_of(bte_ctx* ctx) {
     o.swp_vw = ctx->swp_vw.as<double>();
     o.swp_bnd = ctx->swp_bnd.as<unsigned char>();
     o.swp_pos = ctx->swp_pos.as<int>();
+    o.lvl_sync = ctx->lvl_sync.as<unsigned char>();
+    o.dir_fb = ctx->dir_fb.as<int>();
+    o.ring_r = ctx->ring_r;
     o.ring_sync = 1;
     (void)ring_size(ctx, &o.ring_sync);
     return o;
