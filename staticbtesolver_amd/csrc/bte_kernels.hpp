@@ -295,10 +295,10 @@ struct Ops {
     const unsigned char* swp_bnd;  // [D*N] 1 if the cell has an incoming boundary face
     const int* swp_pos;   // [(d*W+k)*N + i] level-order position of the upwind neighbour (-1: none)
     int ring_sync;        // band-batched pipelined sweeps: HBM writes made visible every ring_sync levels
-    // work-group-per-system pipelined sweep (ring of ring_r values): lvl_sync[lvl_off[d] + l] = 1 when
-    // the sweep of direction d must make its HBM writes visible after level l (a later level reads,
-    // from HBM, a value older than the ring written at or before l); dir_fb[d] = 1 when direction d
-    // has such reads at all
+    // pipelined sweeps with a ring of ring_r values: lvl_sync[lvl_off[d] + l] bit 1 when the sweep
+    // of direction d must make its HBM writes visible after level l (a later level reads, from HBM, a
+    // value older than the ring written at or before l), bit 0 when level l reads such values;
+    // dir_fb[d] = 1 when direction d has such reads at all
     const unsigned char* lvl_sync;
     const int* dir_fb;
     int ring_r;
@@ -360,6 +360,10 @@ __device__ __forceinline__ R diag_of(const SysView<R>& sv, R dgc, int c) {
 template <int K>
 struct Unroll {
     static constexpr int value = K;
+};
+template <bool B>
+struct BoolC {
+    static constexpr bool value = B;
 };
 
 template <int BLOCK, int U, class F>
@@ -1028,40 +1032,54 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         const SweepSrc src = sweep_src<W>(o, w, s, d);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
-        // level pointers, with the synchronisation flag of level t in bit 31 of entry t
+        // level pointers with the level's flags (o.lvl_sync) in bits 31 (synchronise after it) and
+        // 30 (it reads values older than the ring)
         if constexpr (FB) {
             const unsigned char* __restrict__ fl = o.lvl_sync + o.lvl_off[d];
-            for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t] | (t < L && fl[t] ? (int)0x80000000 : 0);
+            for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t] | (t < L ? (int)fl[t] << 30 : 0);
         } else {
             for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
         }
         __syncthreads();
         // Level pointers lps[wb .. wb + 63] are held one per lane and read with readlane (a register
-        // read, no LDS round trip on the level's dependency chain), their flags as one 64-bit mask;
+        // read, no LDS round trip on the level's dependency chain), each flag as one 64-bit mask;
         // the window moves before a level would need a pointer beyond it (level l reads l .. l+PD+1).
         const int lane = tid & 63;
         int wb = 0, win = 0;
-        unsigned long long syncs = 0;
+        unsigned long long syncs = 0, olds = 0;
+        // the flags hold for rings of at least o.ring_r values; a smaller ring treats every level as
+        // flagged
+        const bool every = R < o.ring_r;
         auto load_window = [&](int from) {
             wb = from;
             const int v = lps[min(from + lane, L)];
-            win = v & 0x7fffffff;
-            if constexpr (FB) syncs = __ballot(v < 0);
+            win = v & 0x3fffffff;
+            if constexpr (FB) {
+                syncs = every ? ~0ull : __ballot(v & 0x80000000);
+                olds = every ? ~0ull : __ballot(v & 0x40000000);
+            }
         };
         load_window(0);
-        // the flags hold for rings of at least o.ring_r values; a smaller ring synchronises every level
-        const bool every = R < o.ring_r;
         auto lp_at = [&](int l) { return __builtin_amdgcn_readlane(win, l - wb); };
         auto first = [&](int l) {  // this thread's first-pass position in level l, or -1
             if (l >= L) return -1;
             const int i = lp_at(l) + tid;
             return i < lp_at(l + 1) ? i : -1;
         };
-        auto solve_cell = [&](const SweepItem& it, int end) {
-            const double acc = ring_upwind_sum<FB>(it, end - R, ring, ring_mask, xg);
+        // HB: the level may read values older than the ring (HBM read-back path compiled in)
+        auto solve_cell = [&](const SweepItem& it, int end, auto hb) {
+            const double acc = ring_upwind_sum<decltype(hb)::value>(it, end - R, ring, ring_mask, xg);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
             xg[it.c] = xv;
+        };
+        auto solve_level = [&](const SweepItem& cur, int beg, int end, auto hb) {
+            if (cur.i >= 0) solve_cell(cur, end, hb);
+            for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
+                SweepItem E;
+                sweep_item_load<W>(i, src, E);
+                solve_cell(E, end, hb);
+            }
         };
         // level l computes from the item loaded PD levels earlier and loads the item of level l+PD
         // into the register set it just freed; the PD + 1 sets rotate by unrolling, never by copies
@@ -1070,13 +1088,11 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             if (l + PD + 1 - wb > 63) load_window(l);
             const int beg = lp_at(l), end = lp_at(l + 1);
             sweep_item_load<W>(first(l + PD), src, ahead);
-            if (cur.i >= 0) solve_cell(cur, end);
-            for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
-                SweepItem E;
-                sweep_item_load<W>(i, src, E);
-                solve_cell(E, end);
-            }
-            if (FB && (every || ((syncs >> (l - wb)) & 1ull)))
+            if (FB && ((olds >> (l - wb)) & 1ull))
+                solve_level(cur, beg, end, BoolC<true>{});
+            else
+                solve_level(cur, beg, end, BoolC<false>{});
+            if (FB && ((syncs >> (l - wb)) & 1ull))
                 __syncthreads();  // HBM writes of the levels so far complete and visible
             else
                 lds_barrier();
@@ -1231,7 +1247,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     // level pointers and synchronisation flags as in sweep_ring_system
     if constexpr (FB) {
         const unsigned char* __restrict__ fl = o.lvl_sync + o.lvl_off[d];
-        for (int t = lane; t <= L; t += 64) lps[t] = lp[t] | (t < L && fl[t] ? (int)0x80000000 : 0);
+        for (int t = lane; t <= L; t += 64) lps[t] = lp[t] | (t < L ? (int)fl[t] << 30 : 0);
     } else {
         for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
     }
@@ -1241,8 +1257,8 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     auto load_window = [&](int from) {
         wb = from;
         const int v = lps[min(from + lane, L)];
-        win = v & 0x7fffffff;
-        if constexpr (FB) syncs = __ballot(v < 0);
+        win = v & 0x3fffffff;
+        if constexpr (FB) syncs = __ballot(v & 0x80000000);
     };
     load_window(0);
     const bool every = R < o.ring_r;

@@ -562,10 +562,10 @@ int prepare(bte_ctx* ctx, bool timing) {
                     ctx->ring_span[q] = std::max(ctx->ring_span[q], lptr[d][l + 1] - lptr[d][std::max(0, l - K + 1)]);
             }
         }
-        // Synchronisation levels of the work-group-per-system pipelined sweep with its ring of
-        // ring_size(ctx) values: an upwind value older than the ring is read from HBM, so its write
-        // must complete before a full work-group synchronisation that precedes the read, i.e. at
-        // the end of some level in [level written, level read - 1].  Greedy interval stabbing
+        // Level flags of the pipelined sweeps with a ring of ring_size(ctx) values: an upwind value
+        // older than the ring is read from HBM (flag 1 on the reading level), so its write must
+        // complete before a full work-group synchronisation that precedes the read, i.e. at the end
+        // of some level in [level written, level read - 1] (flag 2).  Greedy interval stabbing
         // (intervals by their last level) flags the fewest level ends that cover every such read.
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
@@ -586,7 +586,10 @@ int prepare(bte_ctx* ctx, bool timing) {
                     for (int i = P[l]; i < P[l + 1]; i++)
                         for (int k = 0; k < W; k++) {
                             const int p = spos_all[((size_t)d * W + k) * N + i];
-                            if (p >= 0 && p < P[l + 1] - ctx->ring_r) iv.push_back({l - 1, lv_of[p]});
+                            if (p >= 0 && p < P[l + 1] - ctx->ring_r) {
+                                iv.push_back({l - 1, lv_of[p]});
+                                sync_all[off[d] + l] |= 1;
+                            }
                         }
                 if (iv.empty()) continue;
                 fb_dir[d] = 1;
@@ -595,7 +598,7 @@ int prepare(bte_ctx* ctx, bool timing) {
                 for (const auto& e : iv)
                     if (last < e.second) {
                         last = e.first;
-                        sync_all[off[d] + last] = 1;
+                        sync_all[off[d] + last] |= 2;
                     }
             }
         }
