@@ -104,6 +104,32 @@ __global__ void k_gather_sweep(int N, int D, int W, const int* __restrict__ perm
     }
 }
 
+// Packed level-order operand records of the pipelined sweeps, one 16-byte and one 32-byte record
+// per (direction, position) so that a cell's operands are three 16-byte loads: pki = {cell, upwind
+// positions (absent slot: INT_MAX, which is never older than a ring and whose ring slot is read
+// with coefficient 0)}, pkd = {coefficients (absent: 0), then the diagonal face sum and the volume
+// weight as far as they fit: W = 1 {a0, dg, vw, 0}, W = 2 {a0, a1, dg, vw}, W = 3 {a0, a1, a2, dg}
+// with the volume weight left in swp_vw}.
+__global__ void k_pack_sweep(int N, int D, int W, const int* __restrict__ perm, const int* __restrict__ spos,
+                             const double* __restrict__ sval, const double* __restrict__ sdg,
+                             const double* __restrict__ svw, int* __restrict__ pki, double* __restrict__ pkd) {
+    const long long total = (long long)D * N;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int d = (int)(t / N), i = (int)(t % N);
+        double a[3] = {0, 0, 0};
+        pki[t * 4] = perm[t];
+        for (int k = 0; k < 3; k++) {
+            const int q = k < W ? spos[((size_t)d * W + k) * N + i] : -1;
+            pki[t * 4 + 1 + k] = q < 0 ? 0x7fffffff : q;
+            if (q >= 0) a[k] = sval[((size_t)d * W + k) * N + i];
+        }
+        const double f[4] = {W == 1 ? a[0] : a[0], W == 1 ? sdg[t] : a[1], W == 1 ? svw[t] : (W == 2 ? sdg[t] : a[2]),
+                             W == 1 ? 0.0 : (W == 2 ? svw[t] : sdg[t])};
+        for (int k = 0; k < 4; k++) pkd[t * 4 + k] = f[k];
+    }
+}
+
 // Per-direction, level-ordered volume weights w_d V_c and a flag for cells with an incoming
 // boundary face in direction d (the only cells whose right-hand side needs the face loop).
 __global__ void k_sweep_aux(Geo g, Quad q, const int* __restrict__ perm, const double* __restrict__ w_eff,
@@ -302,6 +328,9 @@ struct Ops {
     const unsigned char* lvl_sync;
     const int* dir_fb;
     int ring_r;
+    // packed level-order records of the pipelined sweeps (k_pack_sweep), 4 entries per (d, i)
+    const int* swp_pki;
+    const double* swp_pkd;
 };
 
 struct Work {
@@ -1000,6 +1029,100 @@ __device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, 
     return acc;
 }
 
+// Operands of one cell of the pipelined sweeps from the packed records (k_pack_sweep).
+struct PkItem {
+    int i, c;     // level-order position (-1: no cell for this thread) and cell
+    int p[3];     // level-order positions of the upwind neighbours (absent: INT_MAX)
+    double a[3];  // their coefficients (absent: 0)
+    double dgc, vw, rhs;
+};
+
+struct PkSrc {
+    __amdgpu_buffer_rsrc_t pki, pkd, scol, svw, rhs;
+    unsigned n4;  // byte distance between the upwind-cell slots of swp_col: N * 4
+};
+
+template <int W>
+__device__ __forceinline__ PkSrc pk_src(const Ops& o, const Work& w, int s, int d) {
+    const size_t N = (size_t)o.N;
+    PkSrc r;
+    r.pki = buffer_of(o.swp_pki + d * N * 4, N * 16);
+    r.pkd = buffer_of(o.swp_pkd + d * N * 4, N * 32);
+    r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
+    r.svw = buffer_of(o.swp_vw + d * N, N * 8);
+    r.rhs = buffer_of(w.b + (size_t)s * N, N * 8);
+    r.n4 = (unsigned)(N * 4);
+    return r;
+}
+
+typedef unsigned int bte_u32x4 __attribute__((ext_vector_type(4)));
+typedef double bte_f64x2 __attribute__((ext_vector_type(2)));
+
+// Three 16-byte loads (four with W = 3) and the right-hand side, all at the lane's position; an
+// absent item (i < 0) loads position 0 and is ignored.
+template <int W>
+__device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it) {
+    it.i = i;
+    const unsigned j = (unsigned)(i < 0 ? 0 : i);
+    const bte_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(src.pki, j * 16u, 0, 0);
+    const bte_f64x2 d0 = __builtin_bit_cast(bte_f64x2, __builtin_amdgcn_raw_buffer_load_b128(src.pkd, j * 32u, 0, 0));
+    const bte_f64x2 d1 =
+        __builtin_bit_cast(bte_f64x2, __builtin_amdgcn_raw_buffer_load_b128(src.pkd, j * 32u + 16u, 0, 0));
+    it.c = (int)q[0];
+    it.p[0] = (int)q[1];
+    it.p[1] = (int)q[2];
+    it.p[2] = (int)q[3];
+    if constexpr (W == 1) {
+        it.a[0] = d0[0];
+        it.a[1] = it.a[2] = 0.0;
+        it.dgc = d0[1];
+        it.vw = d1[0];
+    } else if constexpr (W == 2) {
+        it.a[0] = d0[0];
+        it.a[1] = d0[1];
+        it.a[2] = 0.0;
+        it.dgc = d1[0];
+        it.vw = d1[1];
+    } else {
+        it.a[0] = d0[0];
+        it.a[1] = d0[1];
+        it.a[2] = d1[0];
+        it.dgc = d1[1];
+        it.vw = buf_ld<double>(src.svw, j * 8u, 0);
+    }
+    it.rhs = buf_ld<double>(src.rhs, j * 8u, 0);
+}
+
+// sum_k a_k x(p_k) in slot order over the W slots, as ring_upwind_sum: an absent slot reads the
+// ring's last slot (finite: cleared before the sweep, then sweep values) times 0, which leaves the
+// sum unchanged.  HB: the level has reads older than the ring, resolved from HBM after one
+// wave-uniform test; the upwind cell of such a read is loaded only then.
+template <int W, bool HB>
+__device__ __forceinline__ double pk_upwind_sum(const PkItem& it, int lim, const double* ring, int ring_mask,
+                                                const gmem<double>* xg, const PkSrc& src) {
+    double v[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) v[k] = ring[it.p[k] & ring_mask];
+    if constexpr (HB) {
+        bool old = false;
+#pragma unroll
+        for (int k = 0; k < W; k++) old |= it.p[k] < lim;
+        if (__ballot(old)) {
+#pragma unroll
+            for (int k = 0; k < W; k++)
+                if (it.p[k] < lim) {
+                    const int c = buf_ld<int>(src.scol, (unsigned)it.i * 4u, k * src.n4);
+                    const double h = xg[c];
+                    asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
+                }
+        }
+    }
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) acc += it.a[k] * v[k];
+    return acc;
+}
+
 // LDS-only work-group barrier: the ring is the only memory threads exchange inside the sweep, so
 // outstanding HBM loads (the prefetches) stay in flight across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -1029,7 +1152,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
         gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
-        const SweepSrc src = sweep_src<W>(o, w, s, d);
+        const PkSrc src = pk_src<W>(o, w, s, d);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
         // level pointers with the level's flags (o.lvl_sync) in bits 31 (synchronise after it) and
@@ -1040,6 +1163,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         } else {
             for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
         }
+        if (tid == 0) ring[ring_mask] = 0.0;  // read, times 0, for absent upwind slots
         __syncthreads();
         // Level pointers lps[wb .. wb + 63] are held one per lane and read with readlane (a register
         // read, no LDS round trip on the level's dependency chain), each flag as one 64-bit mask;
@@ -1067,27 +1191,27 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             return i < lp_at(l + 1) ? i : -1;
         };
         // HB: the level may read values older than the ring (HBM read-back path compiled in)
-        auto solve_cell = [&](const SweepItem& it, int end, auto hb) {
-            const double acc = ring_upwind_sum<decltype(hb)::value>(it, end - R, ring, ring_mask, xg);
+        auto solve_cell = [&](const PkItem& it, int end, auto hb) {
+            const double acc = pk_upwind_sum<W, decltype(hb)::value>(it, end - R, ring, ring_mask, xg, src);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
             xg[it.c] = xv;
         };
-        auto solve_level = [&](const SweepItem& cur, int beg, int end, auto hb) {
+        auto solve_level = [&](const PkItem& cur, int beg, int end, auto hb) {
             if (cur.i >= 0) solve_cell(cur, end, hb);
             for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
-                SweepItem E;
-                sweep_item_load<W>(i, src, E);
+                PkItem E;
+                pk_item_load<W>(i, src, E);
                 solve_cell(E, end, hb);
             }
         };
         // level l computes from the item loaded PD levels earlier and loads the item of level l+PD
         // into the register set it just freed; the PD + 1 sets rotate by unrolling, never by copies
         // (a copy of a register with a load in flight would wait for that load)
-        auto level = [&](int l, SweepItem& cur, SweepItem& ahead) {
+        auto level = [&](int l, PkItem& cur, PkItem& ahead) {
             if (l + PD + 1 - wb > 63) load_window(l);
             const int beg = lp_at(l), end = lp_at(l + 1);
-            sweep_item_load<W>(first(l + PD), src, ahead);
+            pk_item_load<W>(first(l + PD), src, ahead);
             if (FB && ((olds >> (l - wb)) & 1ull))
                 solve_level(cur, beg, end, BoolC<true>{});
             else
@@ -1097,9 +1221,9 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             else
                 lds_barrier();
         };
-        SweepItem X[PD + 1];
+        PkItem X[PD + 1];
 #pragma unroll
-        for (int k = 0; k < PD; k++) sweep_item_load<W>(first(k), src, X[k]);
+        for (int k = 0; k < PD; k++) pk_item_load<W>(first(k), src, X[k]);
         X[PD].i = -1;
         // Steady state: every copy of the unrolled body runs, so on every path into a level the
         // compiler sees its item loaded PD levels earlier and waits for those loads only (a copy
@@ -1241,7 +1365,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     const int b = s / o.D, d = s % o.D;
     const double vt = o.vt[b];
     gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
-    const SweepSrc src = sweep_src<W>(o, w, s, d);
+    const PkSrc src = pk_src<W>(o, w, s, d);
     const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
     const int L = o.lvl_cnt[d];
     // level pointers and synchronisation flags as in sweep_ring_system
@@ -1251,6 +1375,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     } else {
         for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
     }
+    if (lane == 0) ring[ring_mask] = 0.0;  // read, times 0, for absent upwind slots
     __syncthreads();
     int wb = 0, win = 0;
     unsigned long long syncs = 0;
@@ -1268,24 +1393,24 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         const int i = lp_at(l) + lane + 64 * k;
         return i < lp_at(l + 1) ? i : -1;
     };
-    auto solve_cell = [&](const SweepItem& it, int end) {
-        const double acc = ring_upwind_sum<FB>(it, end - R, ring, ring_mask, xg);
+    auto solve_cell = [&](const PkItem& it, int end) {
+        const double acc = pk_upwind_sum<W, FB>(it, end - R, ring, ring_mask, xg, src);
         const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
         ring[it.i & ring_mask] = xv;
         xg[it.c] = xv;
     };
-    auto level = [&](int l, SweepItem (&cur)[IPL], SweepItem (&ahead)[IPL]) {
+    auto level = [&](int l, PkItem (&cur)[IPL], PkItem (&ahead)[IPL]) {
         if (l + 3 - wb > 63) load_window(l);
         const int beg = lp_at(l), end = lp_at(l + 1);
 #pragma unroll
         for (int k = 0; k < IPL; k++)
-            sweep_item_load<W>(first(l + 2, k), src, ahead[k]);
+            pk_item_load<W>(first(l + 2, k), src, ahead[k]);
 #pragma unroll
         for (int k = 0; k < IPL; k++)
             if (cur[k].i >= 0) solve_cell(cur[k], end);
         for (int i = beg + 64 * IPL + lane; i < end; i += 64) {
-            SweepItem E;
-            sweep_item_load<W>(i, src, E);
+            PkItem E;
+            pk_item_load<W>(i, src, E);
             solve_cell(E, end);
         }
         if (FB && (every || ((syncs >> (l - wb)) & 1ull)))
@@ -1293,11 +1418,11 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         else
             wave_lds_sync();
     };
-    SweepItem X0[IPL], X1[IPL], X2[IPL];
+    PkItem X0[IPL], X1[IPL], X2[IPL];
 #pragma unroll
     for (int k = 0; k < IPL; k++) {
-        sweep_item_load<W>(first(0, k), src, X0[k]);
-        sweep_item_load<W>(first(1, k), src, X1[k]);
+        pk_item_load<W>(first(0, k), src, X0[k]);
+        pk_item_load<W>(first(1, k), src, X1[k]);
         X2[k].i = -1;
     }
     // unconditional steady-state body, tail peeled (as in sweep_ring_system)

@@ -49,6 +49,7 @@ struct bte_ctx {
     DBuf dg, ell_col, ell_val, overflow, dg32, ell_val32;
     DBuf lvl_off, lvl_cnt, lvl_ptr, perm, swp_col, swp_val, swp_dg, swp_val32, swp_dg32, swp_vw, swp_bnd, swp_pos;
     DBuf lvl_sync, dir_fb;             // pipelined sweep: synchronisation levels, directions with HBM reads
+    DBuf swp_pki, swp_pkd;             // pipelined sweep: packed level-order operand records
     int ring_r = 0;                    // ring size those were computed for (ring_size(ctx), 0: none)
     int max_levels = 0, lagged = 0;
     int ring_span[7] = {};             // max positions spanned by 64, 32, ..., 2, 1 consecutive sweep levels
@@ -622,6 +623,13 @@ int prepare(bte_ctx* ctx, bool timing) {
                            quad_of(ctx), ctx->perm.as<int>(), ctx->w_eff.as<double>(), ctx->swp_vw.as<double>(),
                            ctx->swp_bnd.as<unsigned char>());
         HIPCHK(hipGetLastError());
+        HIPCHK(ctx->swp_pki.ensure(sizeof(int) * 4 * (size_t)D * N));
+        HIPCHK(ctx->swp_pkd.ensure(sizeof(double) * 4 * (size_t)D * N));
+        hipLaunchKernelGGL(k_pack_sweep, dim3(grid_for((long long)D * N, 256)), dim3(256), 0, ctx->stream, N, D, W,
+                           ctx->perm.as<int>(), ctx->swp_pos.as<int>(), ctx->swp_val.as<double>(),
+                           ctx->swp_dg.as<double>(), ctx->swp_vw.as<double>(), ctx->swp_pki.as<int>(),
+                           ctx->swp_pkd.as<double>());
+        HIPCHK(hipGetLastError());
         // compact list of the level-order entries with an incoming boundary face
         std::vector<unsigned char> hb((size_t)D * N);
         HIPCHK(hipMemcpyAsync(hb.data(), ctx->swp_bnd.p, hb.size(), hipMemcpyDeviceToHost, ctx->stream));
@@ -720,6 +728,8 @@ Ops ops_of(bte_ctx* ctx) {
     o.lvl_sync = ctx->lvl_sync.as<unsigned char>();
     o.dir_fb = ctx->dir_fb.as<int>();
     o.ring_r = ctx->ring_r;
+    o.swp_pki = ctx->swp_pki.as<int>();
+    o.swp_pkd = ctx->swp_pkd.as<double>();
     o.ring_sync = 1;
     (void)ring_size(ctx, &o.ring_sync);
     return o;
