@@ -50,12 +50,21 @@ def summary(prof_dir, title, command, copy_mib=1024):
     tk = top_kernels(prof_dir)
     for n, calls, tot, avg, pct in tk[:10]:
         lines.append(f"| `{short(n)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |")
-    dom = short(tk[0][0]).split("::")[-1].split("<")[0]
-    lines += ["", f"Dominant kernel: `{dom}`, average {tk[0][3]:.2f} us per dispatch.", ""]
-    res = {"dominant": dom, "avg_us": tk[0][3]}
+    # the kernel reported: the busiest one, or (PROF_KERNEL=<name part>) the instantiation of that
+    # kernel launched most often -- the one the autotuner kept, not the candidates it timed
+    row = tk[0]
+    want = os.environ.get("PROF_KERNEL")
+    if want:
+        cands = [r for r in tk if want in r[0]]
+        if cands:
+            row = max(cands, key=lambda r: r[1])
+    full = short(row[0])
+    dom = full.split("::")[-1].split("<")[0]
+    lines += ["", f"Dominant kernel: `{full}`, average {row[3]:.2f} us per dispatch.", ""]
+    res = {"dominant": dom, "instance": full, "avg_us": row[3]}
     try:
-        f_k, nf = counter_per_dispatch(os.path.join(prof_dir, "pmc_FETCH_SIZE"), "FETCH_SIZE", dom)
-        w_k, nw = counter_per_dispatch(os.path.join(prof_dir, "pmc_WRITE_SIZE"), "WRITE_SIZE", dom)
+        f_k, nf = counter_per_dispatch(os.path.join(prof_dir, "pmc_FETCH_SIZE"), "FETCH_SIZE", full)
+        w_k, nw = counter_per_dispatch(os.path.join(prof_dir, "pmc_WRITE_SIZE"), "WRITE_SIZE", full)
         cf, _ = counter_per_dispatch(os.path.join(prof_dir, "pmc_copy_FETCH_SIZE"), "FETCH_SIZE", "k_copy")
         cw, _ = counter_per_dispatch(os.path.join(prof_dir, "pmc_copy_WRITE_SIZE"), "WRITE_SIZE", "k_copy")
         exact_kib = copy_mib * 1024.0

@@ -3,7 +3,8 @@
 # (FETCH_SIZE, WRITE_SIZE) in separate passes, each also for the calibration copy kernel, then
 # writes summary.md / summary.json (prof_summary.py) next to the raw results.
 # Usage (from the repository root, on a GPU box): bash staticbtesolver_amd/profile.sh [bench args]
-# (results under $PROF_OUT, default gpurun_out/prof)
+# (results under $PROF_OUT, default gpurun_out/prof; PROF_KERNEL=<name part> reports the most
+# launched instantiation of that kernel instead of the busiest kernel)
 set -euo pipefail
 ROOT=$(cd "$(dirname "$0")/.." && pwd)
 OUT=${PROF_OUT:-$ROOT/gpurun_out/prof}
