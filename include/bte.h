@@ -67,7 +67,7 @@ typedef struct bte_opts {
                              formed on the fly instead of BiCGSTAB (same solution); default 1 */
     int sweep_variant;    /* direct sweep kernel: 0 autotuned, 1 basic, 2 level-ordered operands,
                              3 level-ordered operands with the intensity vector held in LDS
-                             (needs n_cell * 8 bytes of LDS), 4 pipelined (operands loaded two
+                             (needs n_cell * 8 bytes of LDS), 4 pipelined (operands loaded three
                              levels ahead, recent intensities in an LDS ring), 5 as 4 with one
                              wave per system (no work-group barriers), 6 / 7 as 4 with 2 / 4 bands
                              of a direction per work-group sharing each operator load; a form the
