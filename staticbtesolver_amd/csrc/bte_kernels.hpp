@@ -106,8 +106,9 @@ __global__ void k_gather_sweep(int N, int D, int W, const int* __restrict__ perm
 
 // Packed level-order operand records of the pipelined sweeps, one 16-byte and one 32-byte record
 // per (direction, position) so that a cell's operands are three 16-byte loads: pki = {cell, upwind
-// positions (absent slot: INT_MAX, which is never older than a ring and whose ring slot is read
-// with coefficient 0)}, pkd = {coefficients (absent: 0), then the diagonal face sum and the volume
+// positions times 8, i.e. byte offsets of doubles (absent slot: INT_MAX, which is never older than
+// a ring and whose ring slot, the last, is read with coefficient 0)}, pkd = {coefficients (absent:
+// 0), then the diagonal face sum and the volume
 // weight as far as they fit: W = 1 {a0, dg, vw, 0}, W = 2 {a0, a1, dg, vw}, W = 3 {a0, a1, a2, dg}
 // with the volume weight left in swp_vw}.
 __global__ void k_pack_sweep(int N, int D, int W, const int* __restrict__ perm, const int* __restrict__ spos,
@@ -121,7 +122,7 @@ __global__ void k_pack_sweep(int N, int D, int W, const int* __restrict__ perm, 
         pki[t * 4] = perm[t];
         for (int k = 0; k < 3; k++) {
             const int q = k < W ? spos[((size_t)d * W + k) * N + i] : -1;
-            pki[t * 4 + 1 + k] = q < 0 ? 0x7fffffff : q;
+            pki[t * 4 + 1 + k] = q < 0 ? 0x7fffffff : q * 8;
             if (q >= 0) a[k] = sval[((size_t)d * W + k) * N + i];
         }
         const double f[4] = {W == 1 ? a[0] : a[0], W == 1 ? sdg[t] : a[1], W == 1 ? svw[t] : (W == 2 ? sdg[t] : a[2]),
@@ -1032,13 +1033,13 @@ __device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, 
 // Operands of one cell of the pipelined sweeps from the packed records (k_pack_sweep).
 struct PkItem {
     int i, c;     // level-order position (-1: no cell for this thread) and cell
-    int p[3];     // level-order positions of the upwind neighbours (absent: INT_MAX)
+    int p[3];     // 8 x level-order positions of the upwind neighbours (absent: INT_MAX)
     double a[3];  // their coefficients (absent: 0)
     double dgc, vw, rhs;
 };
 
 struct PkSrc {
-    __amdgpu_buffer_rsrc_t pki, pkd, scol, svw, rhs;
+    __amdgpu_buffer_rsrc_t pki, pkd, scol, svw, rhs, x;
     unsigned n4;  // byte distance between the upwind-cell slots of swp_col: N * 4
 };
 
@@ -1051,6 +1052,7 @@ __device__ __forceinline__ PkSrc pk_src(const Ops& o, const Work& w, int s, int 
     r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
     r.svw = buffer_of(o.swp_vw + d * N, N * 8);
     r.rhs = buffer_of(w.b + (size_t)s * N, N * 8);
+    r.x = buffer_of(w.x + (size_t)s * N, N * 8);
     r.n4 = (unsigned)(N * 4);
     return r;
 }
@@ -1097,20 +1099,21 @@ __device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it
 // ring's last slot (finite: cleared before the sweep, then sweep values) times 0, which leaves the
 // sum unchanged.  HB: the level has reads older than the ring, resolved from HBM after one
 // wave-uniform test; the upwind cell of such a read is loaded only then.
+// Positions are byte offsets (8 p): the ring slot is p8 & (8 R - 8), and lim8 = 8 (end - R).
 template <int W, bool HB>
-__device__ __forceinline__ double pk_upwind_sum(const PkItem& it, int lim, const double* ring, int ring_mask,
+__device__ __forceinline__ double pk_upwind_sum(const PkItem& it, int lim8, const double* ring, int mask8,
                                                 const gmem<double>* xg, const PkSrc& src) {
     double v[W];
 #pragma unroll
-    for (int k = 0; k < W; k++) v[k] = ring[it.p[k] & ring_mask];
+    for (int k = 0; k < W; k++) v[k] = *(const double*)((const char*)ring + (it.p[k] & mask8));
     if constexpr (HB) {
         bool old = false;
 #pragma unroll
-        for (int k = 0; k < W; k++) old |= it.p[k] < lim;
+        for (int k = 0; k < W; k++) old |= it.p[k] < lim8;
         if (__ballot(old)) {
 #pragma unroll
             for (int k = 0; k < W; k++)
-                if (it.p[k] < lim) {
+                if (it.p[k] < lim8) {
                     const int c = buf_ld<int>(src.scol, (unsigned)it.i * 4u, k * src.n4);
                     const double h = xg[c];
                     asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
@@ -1192,10 +1195,11 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         };
         // HB: the level may read values older than the ring (HBM read-back path compiled in)
         auto solve_cell = [&](const PkItem& it, int end, auto hb) {
-            const double acc = pk_upwind_sum<W, decltype(hb)::value>(it, end - R, ring, ring_mask, xg, src);
+            const double acc =
+                pk_upwind_sum<W, decltype(hb)::value>(it, (end - R) * 8, ring, ring_mask * 8, xg, src);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
-            xg[it.c] = xv;
+            buf_st<double>(xv, src.x, (unsigned)it.c * 8u, 0);
         };
         auto solve_level = [&](const PkItem& cur, int beg, int end, auto hb) {
             if (cur.i >= 0) solve_cell(cur, end, hb);
@@ -1394,7 +1398,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
         return i < lp_at(l + 1) ? i : -1;
     };
     auto solve_cell = [&](const PkItem& it, int end) {
-        const double acc = pk_upwind_sum<W, FB>(it, end - R, ring, ring_mask, xg, src);
+        const double acc = pk_upwind_sum<W, FB>(it, (end - R) * 8, ring, ring_mask * 8, xg, src);
         const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
         ring[it.i & ring_mask] = xv;
         xg[it.c] = xv;
