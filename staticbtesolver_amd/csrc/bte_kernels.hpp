@@ -946,90 +946,6 @@ __device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, unsigned v
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 
-struct SweepItem {
-    int i, c;     // level-order position (-1: no cell for this thread) and cell
-    int p[3];     // level-order positions of the upwind neighbours (-1: none)
-    int col[3];   // their cells (for values older than the ring)
-    double a[3];  // their coefficients
-    double dgc, vw, rhs;
-};
-
-// The level-ordered arrays of one system's sweep, as buffer resources.
-struct SweepSrc {
-    __amdgpu_buffer_rsrc_t perm, spos, scol, sval, sdg, svw, rhs;
-    unsigned n4, n8;  // byte distance between operator slots: N * 4, N * 8
-};
-
-template <int W>
-__device__ __forceinline__ SweepSrc sweep_src(const Ops& o, const Work& w, int s, int d) {
-    const size_t N = (size_t)o.N;
-    SweepSrc r;
-    r.perm = buffer_of(o.perm + d * N, N * 4);
-    r.spos = buffer_of(o.swp_pos + d * W * N, W * N * 4);
-    r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
-    r.sval = buffer_of(o.swp_val + d * W * N, W * N * 8);
-    r.sdg = buffer_of(o.swp_dg + d * N, N * 8);
-    r.svw = buffer_of(o.swp_vw + d * N, N * 8);
-    r.rhs = buffer_of(w.b + (size_t)s * N, N * 8);
-    r.n4 = (unsigned)(N * 4);
-    r.n8 = (unsigned)(N * 8);
-    return r;
-}
-
-// Straight-line loads (W, the incoming-slot count, is a compile-time constant and an absent item
-// loads position 0 and is ignored), so the compiler's wait counting stays exact and a later use of
-// the item waits for its own loads only.  All loads share the lane's two byte offsets.
-template <int W, bool RHS = true>
-__device__ __forceinline__ void sweep_item_load(int i, const SweepSrc& src, SweepItem& it) {
-    it.i = i;
-    const unsigned j4 = (unsigned)(i < 0 ? 0 : i) * 4u, j8 = j4 * 2u;
-    it.c = buf_ld<int>(src.perm, j4, 0);
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        it.p[k] = (k < W) ? buf_ld<int>(src.spos, j4, k * src.n4) : -1;
-        it.col[k] = (k < W) ? buf_ld<int>(src.scol, j4, k * src.n4) : -1;
-        it.a[k] = (k < W) ? buf_ld<double>(src.sval, j8, k * src.n8) : 0.0;
-    }
-    it.dgc = buf_ld<double>(src.sdg, j8, 0);
-    it.vw = buf_ld<double>(src.svw, j8, 0);
-    if constexpr (RHS) it.rhs = buf_ld<double>(src.rhs, j8, 0);
-}
-
-// Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order: positions p >= lim from
-// the LDS ring, older ones from HBM.  The HBM reads are rare, and each one's value is consumed (a
-// register move) inside its own branch: a load still pending where the branches merge makes the
-// compiler wait there for every outstanding load, the operand prefetches of later levels included
-// (a select between the two sources is worse: the compiler merges them into one flat load and
-// waits for all loads after every cell).
-// FB = false: the host found every upwind position within the ring, so there is no HBM path.
-template <bool FB>
-__device__ __forceinline__ double ring_upwind_sum(const SweepItem& it, int lim, const double* ring, int ring_mask,
-                                                  const gmem<double>* xg) {
-    double v[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) v[k] = ring[it.p[k] & ring_mask];
-    if constexpr (FB) {
-        // one wave-uniform test: the HBM path is rare and its per-slot branches cost more than the
-        // arithmetic when they are taken around on every cell
-        bool old = false;
-#pragma unroll
-        for (int k = 0; k < 3; k++) old |= it.p[k] >= 0 && it.p[k] < lim;
-        if (__ballot(old)) {
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-                if (it.p[k] >= 0 && it.p[k] < lim) {
-                    const double h = xg[it.col[k]];
-                    asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
-                }
-        }
-    }
-    double acc = 0;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-        if (it.p[k] >= 0) acc += it.a[k] * v[k];
-    return acc;
-}
-
 // Operands of one cell of the pipelined sweeps from the packed records (k_pack_sweep).
 struct PkItem {
     int i, c;     // level-order position (-1: no cell for this thread) and cell
@@ -1062,7 +978,7 @@ typedef double bte_f64x2 __attribute__((ext_vector_type(2)));
 
 // Three 16-byte loads (four with W = 3) and the right-hand side, all at the lane's position; an
 // absent item (i < 0) loads position 0 and is ignored.
-template <int W>
+template <int W, bool RHS = true>
 __device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it) {
     it.i = i;
     const unsigned j = (unsigned)(i < 0 ? 0 : i);
@@ -1092,13 +1008,18 @@ __device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it
         it.dgc = d1[1];
         it.vw = buf_ld<double>(src.svw, j * 8u, 0);
     }
-    it.rhs = buf_ld<double>(src.rhs, j * 8u, 0);
+    if constexpr (RHS) it.rhs = buf_ld<double>(src.rhs, j * 8u, 0);
 }
 
-// sum_k a_k x(p_k) in slot order over the W slots, as ring_upwind_sum: an absent slot reads the
-// ring's last slot (finite: cleared before the sweep, then sweep values) times 0, which leaves the
-// sum unchanged.  HB: the level has reads older than the ring, resolved from HBM after one
-// wave-uniform test; the upwind cell of such a read is loaded only then.
+// Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order over the W slots (the
+// order of the level sweep): positions p >= end - R from the LDS ring, older ones from HBM.  An
+// absent slot reads the ring's last slot (finite: cleared before the sweep, then sweep values)
+// times 0, which leaves the sum unchanged.  HB: the level has reads older than the ring, resolved
+// after one wave-uniform test; the upwind cell of such a read is loaded only then, and its value
+// is consumed (a register move) inside its own branch: a load still pending where the branches
+// merge makes the compiler wait there for every outstanding load, the operand prefetches of later
+// levels included (a select between the two sources is worse: the compiler merges them into one
+// flat load and waits for all loads after every cell).
 // Positions are byte offsets (8 p): the ring slot is p8 & (8 R - 8), and lim8 = 8 (end - R).
 template <int W, bool HB>
 __device__ __forceinline__ double pk_upwind_sum(const PkItem& it, int lim8, const double* ring, int mask8,
@@ -1287,40 +1208,41 @@ __device__ __forceinline__ void sweep_ring_bands(const Ops& o, const Work& w, in
         xg[g] = to_global(w.x + (size_t)s * N);
         rhsr[g] = buffer_of(w.b + (size_t)s * N, (size_t)N * 8);
     }
-    const SweepSrc src = sweep_src<W>(o, w, b0 * o.D + d, d);
+    const PkSrc src = pk_src<W>(o, w, b0 * o.D + d, d);
     const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
     const int L = o.lvl_cnt[d];
     for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
+    if (tid < G) ring[tid * R + ring_mask] = 0.0;  // read, times 0, for absent upwind slots
     __syncthreads();
     auto first = [&](int l) {  // this thread's first-pass position in level l, or -1
         if (l >= L) return -1;
         const int i = lps[l] + tid;
         return i < lps[l + 1] ? i : -1;
     };
-    auto load = [&](int i, SweepItem& it, double (&rhs)[G]) {
-        sweep_item_load<W, false>(i, src, it);
+    auto load = [&](int i, PkItem& it, double (&rhs)[G]) {
+        pk_item_load<W, false>(i, src, it);
         const unsigned j8 = (unsigned)(i < 0 ? 0 : i) * 8u;
 #pragma unroll
         for (int g = 0; g < G; g++) rhs[g] = buf_ld<double>(rhsr[g], j8, 0);
     };
-    auto solve_cell = [&](const SweepItem& it, const double (&rhs)[G], int end) {
+    auto solve_cell = [&](const PkItem& it, const double (&rhs)[G], int end) {
 #pragma unroll
         for (int g = 0; g < G; g++)
             if (g < nb) {
                 double* rg = ring + g * R;
-                const double acc = ring_upwind_sum<FB>(it, end - R, rg, ring_mask, xg[g]);
+                const double acc = pk_upwind_sum<W, FB>(it, (end - R) * 8, rg, ring_mask * 8, xg[g], src);
                 const double xv = (rhs[g] - vt[g] * acc) / (vt[g] * it.dgc + it.vw);
                 rg[it.i & ring_mask] = xv;
                 xg[g][it.c] = xv;
             }
     };
     // as in sweep_ring_system: three register sets rotate by unrolling, never by copies
-    auto level = [&](int l, SweepItem& cur, double (&rcur)[G], SweepItem& ahead, double (&rahead)[G]) {
+    auto level = [&](int l, PkItem& cur, double (&rcur)[G], PkItem& ahead, double (&rahead)[G]) {
         const int beg = lps[l], end = lps[l + 1];
         load(first(l + 2), ahead, rahead);
         if (cur.i >= 0) solve_cell(cur, rcur, end);
         for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
-            SweepItem E;
+            PkItem E;
             double rE[G];
             load(i, E, rE);
             solve_cell(E, rE, end);
@@ -1330,7 +1252,7 @@ __device__ __forceinline__ void sweep_ring_bands(const Ops& o, const Work& w, in
         else
             lds_barrier();
     };
-    SweepItem X0, X1, X2;
+    PkItem X0, X1, X2;
     double r0[G], r1[G], r2[G];
     load(first(0), X0, r0);
     load(first(1), X1, r1);
