@@ -105,12 +105,13 @@ __global__ void k_gather_sweep(int N, int D, int W, const int* __restrict__ perm
 }
 
 // Packed level-order operand records of the pipelined sweeps, one 16-byte and one 32-byte record
-// per (direction, position) so that a cell's operands are three 16-byte loads: pki = {cell, upwind
-// positions times 8, i.e. byte offsets of doubles (absent slot: INT_MAX, which is never older than
-// a ring and whose ring slot, the last, is read with coefficient 0)}, pkd = {coefficients (absent:
-// 0), then the diagonal face sum and the volume
-// weight as far as they fit: W = 1 {a0, dg, vw, 0}, W = 2 {a0, a1, dg, vw}, W = 3 {a0, a1, a2, dg}
-// with the volume weight left in swp_vw}.
+// per (direction, position), so that a cell's operands are three 16-byte loads:
+//   pki = {cell, upwind positions times 8 (byte offsets of doubles; an absent slot has INT_MAX,
+//          which is never older than a ring and whose ring slot, the last, is read with
+//          coefficient 0)},
+//   pkd = {coefficients (absent: 0), then the diagonal face sum and the volume weight as far as
+//          they fit: W = 1 {a0, dg, vw, 0}, W = 2 {a0, a1, dg, vw}, W = 3 {a0, a1, a2, dg}, the
+//          volume weight then staying in swp_vw}.
 __global__ void k_pack_sweep(int N, int D, int W, const int* __restrict__ perm, const int* __restrict__ spos,
                              const double* __restrict__ sval, const double* __restrict__ sdg,
                              const double* __restrict__ svw, int* __restrict__ pki, double* __restrict__ pkd) {
@@ -125,9 +126,21 @@ __global__ void k_pack_sweep(int N, int D, int W, const int* __restrict__ perm, 
             pki[t * 4 + 1 + k] = q < 0 ? 0x7fffffff : q * 8;
             if (q >= 0) a[k] = sval[((size_t)d * W + k) * N + i];
         }
-        const double f[4] = {W == 1 ? a[0] : a[0], W == 1 ? sdg[t] : a[1], W == 1 ? svw[t] : (W == 2 ? sdg[t] : a[2]),
-                             W == 1 ? 0.0 : (W == 2 ? svw[t] : sdg[t])};
-        for (int k = 0; k < 4; k++) pkd[t * 4 + k] = f[k];
+        double* f = pkd + t * 4;
+        f[0] = a[0];
+        if (W == 1) {
+            f[1] = sdg[t];
+            f[2] = svw[t];
+            f[3] = 0.0;
+        } else if (W == 2) {
+            f[1] = a[1];
+            f[2] = sdg[t];
+            f[3] = svw[t];
+        } else {
+            f[1] = a[1];
+            f[2] = a[2];
+            f[3] = sdg[t];
+        }
     }
 }
 

@@ -699,7 +699,6 @@ int ensure_work(bte_ctx* ctx, int precision) {
     return BTE_OK;
 }
 
-
 Ops ops_of(bte_ctx* ctx) {
     Ops o{};
     o.N = ctx->N;
