@@ -196,10 +196,12 @@ struct Bands {
 
 // Diffuse (type 2) re-emitted intensity of each boundary face, StaticBTESolver.cpp:180-204.  It
 // does not depend on the incoming direction, so it is formed once per (band, face) and iteration.
-__global__ void k_diffuse(Geo g, Quad q, Bnd bd, int B, const double* __restrict__ ee_prev,
-                          double* __restrict__ ediff) {
+// Body over work-group blk of nblk (blockDim.x threads each); k_diffuse runs it over the grid.
+__device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const Bnd& bd, int B,
+                                              const double* __restrict__ ee_prev, double* __restrict__ ediff, int blk,
+                                              int nblk) {
     const int total = B * bd.n_bf;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    for (int t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
         const int b = t / bd.n_bf, bf = t % bd.n_bf;
         const int c = bd.bf_cell[bf], f = bd.bf_face[bf];
         const int nb = g.nbr[(size_t)c * g.F + f];
@@ -230,6 +232,11 @@ __global__ void k_diffuse(Geo g, Quad q, Bnd bd, int B, const double* __restrict
         }
         ediff[t] = (g.dim != 1) ? es / tw : es / kPi;
     }
+}
+
+__global__ void k_diffuse(Geo g, Quad q, Bnd bd, int B, const double* __restrict__ ee_prev,
+                          double* __restrict__ ediff) {
+    diffuse_range(g, q, bd, B, ee_prev, ediff, blockIdx.x, gridDim.x);
 }
 
 // Right-hand side of system (b, d) at cell c, StaticBTESolver.cpp:162-269.
@@ -892,34 +899,45 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
 // One thread per (system, level position): consecutive threads take consecutive positions of one
 // system, so the stores are coalesced; the level-ordered operands of a direction are re-read per
 // band from L2.
-__global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const double* __restrict__ ee_prev,
-                             const double* __restrict__ dens, double* __restrict__ rhs_lo) {
+__device__ __forceinline__ void rhs_levels_range(const Ops& o, int S, const double* __restrict__ dens,
+                                                 double* __restrict__ rhs_lo, int bx, int nbx, int by, int nby) {
     const int N = o.N, D = o.D;
-    for (int s = blockIdx.y; s < S; s += gridDim.y) {
+    for (int s = by; s < S; s += nby) {
         const int b = s / D, d = s % D;
         const size_t od = (size_t)d * N;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        for (int i = bx * blockDim.x + threadIdx.x; i < N; i += nbx * blockDim.x) {
             if (o.swp_bnd[od + i]) continue;  // cells with an incoming boundary face: k_rhs_levels_bnd
             rhs_lo[(size_t)s * N + i] = dens[(size_t)b * N + o.perm[od + i]] * o.swp_vw[od + i];
         }
     }
 }
 
+__global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const double* __restrict__ ee_prev,
+                             const double* __restrict__ dens, double* __restrict__ rhs_lo) {
+    rhs_levels_range(o, S, dens, rhs_lo, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+}
+
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
 // their face loops run in work-groups of their own instead of stalling the interior waves; one
 // thread per (list entry, band).
-__global__ void k_rhs_levels_bnd(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const int* __restrict__ list,
-                                 int n_list, const double* __restrict__ ee_prev, const double* __restrict__ dens,
-                                 double* __restrict__ rhs_lo) {
+__device__ __forceinline__ void rhs_bnd_range(const Geo& g, const Quad& q, const Bnd& bd, const Bands& bands,
+                                              const Ops& o, int S, const int* __restrict__ list, int n_list,
+                                              const double* __restrict__ ee_prev, const double* __restrict__ dens,
+                                              double* __restrict__ rhs_lo, int blk, int nblk) {
     const int N = o.N, D = o.D, B = S / D;
     const long long total = (long long)n_list * B;
-    for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
-         u += (long long)gridDim.x * blockDim.x) {
+    for (long long u = blk * (long long)blockDim.x + threadIdx.x; u < total; u += (long long)nblk * blockDim.x) {
         const int b = (int)(u / n_list), k = (int)(u % n_list);
         const int t = list[k];
         const int d = t / N, i = t % N;
         rhs_lo[((size_t)b * D + d) * N + i] = rhs_at(g, q, bd, bands, b, d, o.perm[t], ee_prev, dens);
     }
+}
+
+__global__ void k_rhs_levels_bnd(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const int* __restrict__ list,
+                                 int n_list, const double* __restrict__ ee_prev, const double* __restrict__ dens,
+                                 double* __restrict__ rhs_lo) {
+    rhs_bnd_range(g, q, bd, bands, o, S, list, n_list, ee_prev, dens, rhs_lo, blockIdx.x, gridDim.x);
 }
 
 // Global-memory (HBM) views of pointers.  A pointer the kernel reads from memory (the uneven
@@ -1578,12 +1596,12 @@ __global__ __launch_bounds__(BLOCK) void k_temperature(int N, int D, Bands bands
 // threads than a cell loop, so enough loads are in flight to stream both intensity arrays at HBM
 // rate -- fused with the margin partial sums (:311-321).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, const double* __restrict__ omega,
-                                                const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
-                                                double* __restrict__ Tband, double* __restrict__ partial) {
+__device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, const double* __restrict__ omega,
+                                            const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
+                                            double* __restrict__ Tband, double* __restrict__ partial, int blk) {
     __shared__ double lds[BLOCK / 64];
     double mp[1] = {0};
-    const long long t = blockIdx.x * (long long)BLOCK + threadIdx.x;
+    const long long t = blk * (long long)BLOCK + threadIdx.x;
     if (t < (long long)bands.B * N) {
         const int b = (int)(t / N), c = (int)(t % N);
         const size_t base = (size_t)b * D * N + c;
@@ -1598,13 +1616,20 @@ __global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, cons
         Tband[t] = e0 / bands.C[b] + bands.T_ref;
     }
     block_sum<BLOCK, 1>(mp, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = mp[0];
+    if (threadIdx.x == 0) partial[blk] = mp[0];
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, const double* __restrict__ omega,
+                                                const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
+                                                double* __restrict__ Tband, double* __restrict__ partial) {
+    tband_block<BLOCK>(N, D, bands, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x);
 }
 
 // Equilibrium temperature and emission density per cell (StaticBTESolver.cpp:106-120).
-__global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, double* __restrict__ T,
-                        double* __restrict__ dens) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void tcell_range(int N, const Bands& bands, const double* __restrict__ Tband,
+                                            double* __restrict__ T, double* __restrict__ dens, int blk, int nblk) {
+    for (int c = blk * blockDim.x + threadIdx.x; c < N; c += nblk * blockDim.x) {
         double rrn = 0, rnn = 0;
         for (int b = 0; b < bands.B; b++) {
             rrn += bands.Lr[b] * Tband[(size_t)b * N + c];
@@ -1614,6 +1639,11 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
         T[c] = tc;
         for (int b = 0; b < bands.B; b++) dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
     }
+}
+
+__global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, double* __restrict__ T,
+                        double* __restrict__ dens) {
+    tcell_range(N, bands, Tband, T, dens, blockIdx.x, gridDim.x);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1786,12 +1816,103 @@ __global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, 
 }
 
 // Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).
-__global__ void k_margin(int nparts, const double* __restrict__ partial, double scale, double* __restrict__ out) {
+__device__ __forceinline__ void margin_block(int nparts, const double* __restrict__ partial, double scale,
+                                             double* __restrict__ out) {
     __shared__ double lds[4];
     double s[1] = {0};
     for (int i = threadIdx.x; i < nparts; i += 256) s[0] += partial[i];
     block_sum<256, 1>(s, lds);
     if (threadIdx.x == 0) out[0] = s[0] * scale;
+}
+
+__global__ void k_margin(int nparts, const double* __restrict__ partial, double scale, double* __restrict__ out) {
+    margin_block(nparts, partial, scale, out);
+}
+
+// ----------------------------------------------------------------------------------------------
+// Uneven batch: the per-problem steps of an outer iteration around the batched sweep, one launch
+// per step for all problems.  Work-group t belongs to the problem p with off[p] <= t < off[p + 1]
+// and runs that problem's body as work-group t - off[p] of off[p + 1] - off[p], with the same
+// partition (and so the same reduction order) as the single-problem launch.
+// ----------------------------------------------------------------------------------------------
+struct IterDesc {
+    Geo g;
+    Quad q;
+    Bnd bd;
+    Bands bands;
+    Ops o;
+    const double* ee_prev;  // previous iterate
+    const double* ee_curr;  // the sweep's new iterate
+    double* ediff;          // diffuse re-emission, written by the diffuse step
+    double* rhs;            // level-order right-hand sides
+    const int* bnd_list;    // level-order entries with an incoming boundary face
+    int n_bnd_list;
+    double* Tband;
+    double* T;
+    double* dens;
+    double* partial;
+    double* margin;
+    const double* omega;
+    int nblk_tband;         // work-groups of the band-temperature step (its margin partials)
+    double scale;           // 1 / (B N)
+};
+
+struct MultiIdx {
+    int p, blk, nblk;
+};
+
+__device__ __forceinline__ MultiIdx multi_index(const int* __restrict__ off, int n) {
+    const int t = blockIdx.x;
+    int lo = 0, hi = n;  // the last p with off[p] <= t
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return MultiIdx{lo, t - off[lo], off[lo + 1] - off[lo]};
+}
+
+__global__ void k_diffuse_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    const IterDesc& D = desc[m.p];
+    diffuse_range(D.g, D.q, D.bd, D.bands.B, D.ee_prev, D.ediff, m.blk, m.nblk);
+}
+
+// one row of positions per work-group: local work-group k is system k / nx, column k % nx
+__global__ void k_rhs_levels_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    const IterDesc& D = desc[m.p];
+    const int nx = (D.o.N + blockDim.x - 1) / blockDim.x;
+    const int S = D.bands.B * D.o.D;
+    rhs_levels_range(D.o, S, D.dens, D.rhs, m.blk % nx, nx, m.blk / nx, S);
+}
+
+__global__ void k_rhs_bnd_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    const IterDesc& D = desc[m.p];
+    rhs_bnd_range(D.g, D.q, D.bd, D.bands, D.o, D.bands.B * D.o.D, D.bnd_list, D.n_bnd_list, D.ee_prev, D.dens,
+                  D.rhs, m.blk, m.nblk);
+}
+
+__global__ __launch_bounds__(256) void k_tband_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off,
+                                                     int n) {
+    const MultiIdx m = multi_index(off, n);
+    const IterDesc& D = desc[m.p];
+    tband_block<256>(D.o.N, D.o.D, D.bands, D.omega, D.ee_curr, D.ee_prev, D.Tband, D.partial, m.blk);
+}
+
+__global__ void k_tcell_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    const IterDesc& D = desc[m.p];
+    tcell_range(D.o.N, D.bands, D.Tband, D.T, D.dens, m.blk, m.nblk);
+}
+
+// one work-group per problem
+__global__ __launch_bounds__(256) void k_margin_multi(const IterDesc* __restrict__ desc) {
+    const IterDesc& D = desc[blockIdx.x];
+    margin_block(D.nblk_tband, D.partial, D.scale, D.margin);
 }
 
 // Heat flux per (BC, band), StaticBTESolver.cpp:323-370.  One work-group per pair.

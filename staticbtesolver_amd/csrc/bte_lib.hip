@@ -1971,6 +1971,8 @@ struct bte_batch {
     std::string err;
     std::vector<bte_ctx*> ctx;
     DBuf desc, items, mptr, mout;
+    DBuf idesc;                  // per iteration: IterDesc of each active problem, then step offsets
+    std::vector<unsigned char> h_idesc;
     std::vector<int> items_act;  // the active problems b->items and b->mptr were built for
     double* h_m = nullptr;  // pinned [K] margins
     int ring = 0, block = 256, n_items = 0, max_levels = 0;
@@ -2053,18 +2055,49 @@ extern "C" {
 // One outer iteration of every active problem: boundary data, one batched sweep launch over all
 // their systems, temperature update and margins; returns the margins (host) of the active problems.
 static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vector<double>& m) {
+    const int P = (int)act.size();
     std::vector<SweepDesc> hd(act.size());
     std::vector<std::pair<int, int2>> order;  // (levels, item)
     std::vector<const double*> mp(act.size());
+    // the per-problem steps around the sweep, one launch per step for all problems: descriptors,
+    // then the work-group offsets of the diffuse, interior rhs, boundary rhs, band temperature and
+    // cell temperature steps (P + 1 each), uploaded in one copy
+    enum { ST_DIFF, ST_RHS, ST_BND, ST_TBAND, ST_TCELL, ST_N };
+    b->h_idesc.resize(sizeof(IterDesc) * P + sizeof(int) * ST_N * (P + 1));
+    IterDesc* id = (IterDesc*)b->h_idesc.data();
+    int* offs = (int*)(b->h_idesc.data() + sizeof(IterDesc) * P);
+    for (int k = 0; k < ST_N; k++) offs[k * (P + 1)] = 0;
     for (size_t a = 0; a < act.size(); a++) {
         bte_ctx* ctx = b->ctx[act[a]];
         const int par = ctx->run_iter & 1;
         double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
-        if (ctx->has_diffuse)
-            hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, b->stream,
-                               geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, buf[par], ctx->ediff.as<double>());
-        if (b->ring > 0)  // the pipelined sweep reads its right-hand sides in level order from w.b
-            enqueue_rhs_levels(ctx, b->stream, buf[par], ctx->dens.as<double>(), ctx->wb.as<double>());
+        IterDesc& e = id[a];
+        e.g = geo_of(ctx);
+        e.q = quad_of(ctx);
+        e.bd = bnd_of(ctx);
+        e.bands = bands_of(ctx);
+        e.o = ops_of(ctx);
+        e.ee_prev = buf[par];
+        e.ee_curr = buf[1 - par];
+        e.ediff = ctx->ediff.as<double>();
+        e.rhs = ctx->wb.as<double>();
+        e.bnd_list = ctx->bnd_list.as<int>();
+        e.n_bnd_list = ctx->n_bnd_list;
+        e.Tband = ctx->Tband.as<double>();
+        e.T = ctx->T.as<double>();
+        e.dens = ctx->dens.as<double>();
+        e.partial = ctx->partial.as<double>();
+        e.margin = ctx->margin.as<double>();
+        e.omega = ctx->omega.as<double>();
+        e.nblk_tband = grid_for((long long)ctx->B * ctx->N, 256);
+        e.scale = 1.0 / ctx->B / ctx->N;
+        // work-groups per step, as the single-problem launches partition it
+        const int nb[ST_N] = {ctx->has_diffuse ? grid_for((long long)ctx->B * ctx->n_bf, 256) : 0,
+                              b->ring > 0 ? ctx->B * ctx->D * ((ctx->N + 255) / 256) : 0,
+                              (b->ring > 0 && ctx->n_bnd_list > 0)
+                                  ? grid_for((long long)ctx->n_bnd_list * ctx->B, 64) : 0,
+                              e.nblk_tband, grid_for(ctx->N, 256)};
+        for (int k = 0; k < ST_N; k++) offs[k * (P + 1) + a + 1] = offs[k * (P + 1) + a] + nb[k];
         SweepDesc& d = hd[a];  // per iteration: the intensity buffers alternate
         d.g = geo_of(ctx);
         d.q = quad_of(ctx);
@@ -2095,6 +2128,19 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         BCHK(hipMemcpyAsync(b->mptr.p, mp.data(), sizeof(double*) * mp.size(), hipMemcpyHostToDevice, b->stream));
         b->items_act = act;
     }
+    BCHK(b->idesc.ensure(b->h_idesc.size()));
+    BCHK(hipMemcpyAsync(b->idesc.p, b->h_idesc.data(), b->h_idesc.size(), hipMemcpyHostToDevice, b->stream));
+    const IterDesc* did = b->idesc.as<IterDesc>();
+    const int* doff = (const int*)((const char*)b->idesc.p + sizeof(IterDesc) * P);
+    auto multi = [&](auto kernel, int step, int block) {
+        const int total = offs[step * (P + 1) + P];
+        if (total > 0)
+            hipLaunchKernelGGL(kernel, dim3(total), dim3(block), 0, b->stream, did, doff + step * (P + 1), P);
+    };
+    // boundary data, then the right-hand sides in level order (interior, then boundary cells)
+    multi(k_diffuse_multi, ST_DIFF, 256);
+    multi(k_rhs_levels_multi, ST_RHS, 256);
+    multi(k_rhs_bnd_multi, ST_BND, 64);
     BCHK(b->desc.ensure(sizeof(SweepDesc) * hd.size()));
     BCHK(b->mout.ensure(sizeof(double) * mp.size()));
     BCHK(hipMemcpyAsync(b->desc.p, hd.data(), sizeof(SweepDesc) * hd.size(), hipMemcpyHostToDevice, b->stream));
@@ -2104,19 +2150,10 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         case 512: launch_batch_b<512>(b, b->ring); break;
         default: launch_batch_b<256>(b, b->ring); break;
     }
-    for (size_t a = 0; a < act.size(); a++) {
-        bte_ctx* ctx = b->ctx[act[a]];
-        const Bands bn = bands_of(ctx);
-        const int par = ctx->run_iter & 1;
-        double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
-        const int nblk = grid_for((long long)ctx->B * ctx->N, 256);
-        hipLaunchKernelGGL(k_tband<256>, dim3(nblk), dim3(256), 0, b->stream, ctx->N, ctx->D, bn, ctx->omega.as<double>(),
-                           buf[1 - par], buf[par], ctx->Tband.as<double>(), ctx->partial.as<double>());
-        hipLaunchKernelGGL(k_tcell, dim3(grid_for(ctx->N, 256)), dim3(256), 0, b->stream, ctx->N, bn,
-                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>());
-        hipLaunchKernelGGL(k_margin, dim3(1), dim3(256), 0, b->stream, nblk, ctx->partial.as<double>(),
-                           1.0 / ctx->B / ctx->N, ctx->margin.as<double>());
-    }
+    // band and cell temperatures, emission densities and margins
+    multi(k_tband_multi, ST_TBAND, 256);
+    multi(k_tcell_multi, ST_TCELL, 256);
+    hipLaunchKernelGGL(k_margin_multi, dim3(P), dim3(256), 0, b->stream, did);
     hipLaunchKernelGGL(k_gather_scalars, dim3(1), dim3(256), 0, b->stream, (int)mp.size(),
                        b->mptr.as<const double*>(), b->mout.as<double>());
     BCHK(hipGetLastError());
