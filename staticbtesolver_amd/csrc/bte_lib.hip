@@ -2209,7 +2209,14 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
         if (rc) return rc;
         std::vector<int> next;
         // the bookkeeping of every problem, then the mixing steps of all of them with one
-        // synchronisation for their dot products (every member runs on the batch stream)
+        // synchronisation for their dot products.  These small per-problem kernels run on each
+        // problem's own stream, so that those of different problems overlap; batch_iteration left
+        // the batch stream idle, and the device is synchronised before it is used again.  Not with
+        // attached networks: problems may share one, and with it its work buffers.
+        bool own = true;
+        for (int k : act) own = own && b->ctx[k]->gnn == nullptr;
+        if (own)
+            for (int k : act) b->ctx[k]->stream = streams.saved[k];
         std::vector<int> fin;
         for (size_t a = 0; a < act.size(); a++) {
             bte_ctx* ctx = b->ctx[act[a]];
@@ -2229,13 +2236,17 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
                 return rc;
             }
         }
-        if (!fin.empty()) BCHK(hipStreamSynchronize(b->stream));
+        if (!fin.empty()) BCHK(own ? hipDeviceSynchronize() : hipStreamSynchronize(b->stream));
         for (int a : fin) {
             bte_ctx* ctx = b->ctx[act[a]];
             if ((rc = aa_finish(ctx)) || (rc = after_mix(ctx))) {
                 b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
                 return rc;
             }
+        }
+        if (own) {
+            BCHK(hipDeviceSynchronize());
+            for (int k : act) b->ctx[k]->stream = b->stream;
         }
         for (size_t a = 0; a < act.size(); a++) {
             bte_ctx* ctx = b->ctx[act[a]];
