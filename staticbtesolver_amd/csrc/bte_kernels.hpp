@@ -1809,6 +1809,29 @@ __global__ void k_aa_update(int N, int ncols, const double* __restrict__ gamma, 
 }
 
 // Emission density of a given temperature field (StaticBTESolver.cpp:115-119).
+// k_aa_update, then the state's loads of k_set_density and k_state_scatter from the same value, in
+// one pass over the Z = N + S nb state entries (the same arithmetic as the three kernels).
+__global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, const double* __restrict__ gamma,
+                                  const double* __restrict__ g, const double* __restrict__ dX,
+                                  const double* __restrict__ dF, double* __restrict__ x, Bands bands,
+                                  double* __restrict__ dens, const int* __restrict__ bcells,
+                                  const double* __restrict__ sc, double* __restrict__ ee) {
+    const long long Z = N + (long long)S * nb;
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < Z; c += (long long)gridDim.x * blockDim.x) {
+        double v = g[c];
+        for (int j = 0; j < ncols; j++) v -= gamma[j] * (dX[(size_t)j * Z + c] + dF[(size_t)j * Z + c]);
+        x[c] = v;
+        if (c < N) {
+            for (int b = 0; b < bands.B; b++)
+                dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (v - bands.T_ref);
+        } else {
+            const long long u = c - N;
+            const int s = (int)(u / nb), j = (int)(u % nb);
+            ee[(size_t)s * N + bcells[j]] = v / sc[s / D];
+        }
+    }
+}
+
 __global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x)
         for (int b = 0; b < bands.B; b++)

@@ -1257,10 +1257,13 @@ static int aa_finish(bte_ctx* ctx) {
         for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
     }
     HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_aa_update, dim3(gb), dim3(256), 0, ctx->stream, Z, m, ctx->aa_gamma.as<double>(),
-                       ctx->aa_g.as<double>(), ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>(),
-                       ctx->aa_x.as<double>());
-    return load_state(ctx);
+    // the update and the state's density and boundary intensities in one pass (load_state's work)
+    hipLaunchKernelGGL(k_aa_update_state, dim3(gb), dim3(256), 0, ctx->stream, ctx->N, ctx->D, ctx->B * ctx->D,
+                       ctx->aa_nb, m, ctx->aa_gamma.as<double>(), ctx->aa_g.as<double>(), ctx->aa_dX.as<double>(),
+                       ctx->aa_dF.as<double>(), ctx->aa_x.as<double>(), bands_of(ctx), ctx->dens.as<double>(),
+                       ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final);
+    HIPCHK(hipGetLastError());
+    return BTE_OK;
 }
 
 static int aa_step(bte_ctx* ctx) {
