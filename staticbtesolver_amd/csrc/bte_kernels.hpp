@@ -1649,22 +1649,8 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 // ----------------------------------------------------------------------------------------------
 // Anderson mixing of the outer iteration (fixed-point map T -> G(T) of one source iteration).
 // History columns dX_j = x_{j+1} - x_j and dF_j = f_{j+1} - f_j (f = G(x) - x) live in ring
-// buffers [m][N]; the m x m least-squares system is solved on the host.
+// buffers [m][Z]; the m x m least-squares system is solved on the host.
 // ----------------------------------------------------------------------------------------------
-__global__ void k_aa_diff(int N, int first, const double* __restrict__ g, const double* __restrict__ x,
-                          double* __restrict__ x_prev, double* __restrict__ f_prev, double* __restrict__ f_cur,
-                          double* __restrict__ dX, double* __restrict__ dF) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
-        const double f = g[c] - x[c];
-        if (!first) {
-            dX[c] = x[c] - x_prev[c];
-            dF[c] = f - f_prev[c];
-        }
-        x_prev[c] = x[c];
-        f_prev[c] = f;
-        f_cur[c] = f;
-    }
-}
 
 // partial[q * nblk + blk] for q < ncols: dF_q . dF_new, q in [ncols, 2 ncols): dF_{q-ncols} . f,
 // q = 2 ncols: f . f
@@ -1711,6 +1697,39 @@ __global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ parti
 // reflecting (diffuse or specular) boundary face -- the only parts of the previous iterate the next
 // right-hand sides read.  Intensities are carried as temperature equivalents e * Omega / C_b.
 // z = [T (N) ; e_bnd (S x nb)].
+// g = G(x) gathered as k_state_gather does, and in the same pass f = g - x into f_cur and f_prev,
+// x into x_prev and, unless first, the history columns dX = x - x_prev and dF = f - f_prev (from
+// the previous step's values), over the Z = N + S nb state entries.
+__global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __restrict__ bcells,
+                                    const double* __restrict__ sc, const double* __restrict__ T,
+                                    const double* __restrict__ ee, double* __restrict__ z, int first,
+                                    const double* __restrict__ x, double* __restrict__ x_prev,
+                                    double* __restrict__ f_prev, double* __restrict__ f_cur, double* __restrict__ dX,
+                                    double* __restrict__ dF) {
+    const long long total = N + (long long)S * nb;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        double gv;
+        if (t < N) {
+            gv = T[t];
+        } else {
+            const long long u = t - N;
+            const int s = (int)(u / nb), j = (int)(u % nb);
+            gv = ee[(size_t)s * N + bcells[j]] * sc[s / D];
+        }
+        z[t] = gv;
+        const double xv = x[t];
+        const double f = gv - xv;
+        if (!first) {
+            dX[t] = xv - x_prev[t];
+            dF[t] = f - f_prev[t];
+        }
+        x_prev[t] = xv;
+        f_prev[t] = f;
+        f_cur[t] = f;
+    }
+}
+
 __global__ void k_state_gather(int N, int D, int S, int nb, const int* __restrict__ bcells, const double* __restrict__ sc,
                                const double* __restrict__ T, const double* __restrict__ ee, double* __restrict__ z) {
     const long long total = N + (long long)S * nb;
@@ -1798,19 +1817,11 @@ __global__ void k_fill(int n, double v, double* __restrict__ x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
 }
 
-// x = g - sum_j gamma_j (dX_j + dF_j)
-__global__ void k_aa_update(int N, int ncols, const double* __restrict__ gamma, const double* __restrict__ g,
-                            const double* __restrict__ dX, const double* __restrict__ dF, double* __restrict__ x) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
-        double v = g[c];
-        for (int j = 0; j < ncols; j++) v -= gamma[j] * (dX[(size_t)j * N + c] + dF[(size_t)j * N + c]);
-        x[c] = v;
-    }
-}
 
 // Emission density of a given temperature field (StaticBTESolver.cpp:115-119).
-// k_aa_update, then the state's loads of k_set_density and k_state_scatter from the same value, in
-// one pass over the Z = N + S nb state entries (the same arithmetic as the three kernels).
+// The mixing update x = g - sum_j gamma_j (dX_j + dF_j) over the Z = N + S nb state entries, and
+// in the same pass the state's loads for the next iteration from the new x: the emission density
+// of its temperature part (as k_set_density) and its boundary intensities (as k_state_scatter).
 __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, const double* __restrict__ gamma,
                                   const double* __restrict__ g, const double* __restrict__ dX,
                                   const double* __restrict__ dF, double* __restrict__ x, Bands bands,

@@ -1176,15 +1176,18 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     const int gb = grid_for(Z, 256);
     *finish = false;
     ctx->aa_steps++;
-    // g = G(x): the new temperature field and reflecting-boundary intensities
-    hipLaunchKernelGGL(k_state_gather, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb, ctx->aa_bcells.as<int>(),
-                       ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final, ctx->aa_g.as<double>());
+    // g = G(x): the new temperature field and reflecting-boundary intensities, gathered in the same
+    // pass as the differences f = g - x (and, for a history step, the columns dX and dF)
+    auto gather_diff = [&](int first, double* dX, double* dF) {
+        hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
+                           ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
+                           ctx->aa_g.as<double>(), first, ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(),
+                           ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), dX, dF);
+    };
     if (ctx->aa_plain) {
         // plain iteration: x_{k+1} = G(x_k); the iteration already left T, density and the boundary
         // intensities of x_{k+1} in place
-        hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, Z, 1, ctx->aa_g.as<double>(),
-                           ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(),
-                           ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>());
+        gather_diff(1, ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>());
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, ctx->aa_g.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
         HIPCHK(hipGetLastError());
         return BTE_OK;
@@ -1203,10 +1206,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
         }
     }
     const double* g = ctx->aa_g.as<double>();
-    hipLaunchKernelGGL(k_aa_diff, dim3(gb), dim3(256), 0, ctx->stream, Z, (ctx->aa_first || skip) ? 1 : 0, g,
-                       ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(),
-                       ctx->aa_f.as<double>(), ctx->aa_dX.as<double>() + (size_t)slot * Z,
-                       ctx->aa_dF.as<double>() + (size_t)slot * Z);
+    gather_diff((ctx->aa_first || skip) ? 1 : 0, ctx->aa_dX.as<double>() + (size_t)slot * Z,
+                ctx->aa_dF.as<double>() + (size_t)slot * Z);
     if (ctx->aa_first) {
         // plain step: x_1 = G(x_0)
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
