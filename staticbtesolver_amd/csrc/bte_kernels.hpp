@@ -1665,6 +1665,8 @@ __global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const doubl
     double s[2] = {0, 0};
     if (j < ncols) {
         const double* a = dF + (size_t)j * N;
+        // unrolled so that several chunks' loads are in flight; the sums keep their order
+#pragma unroll 4
         for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
             const double v = a[c];
             s[0] += v * dF_new[c];
@@ -1676,6 +1678,7 @@ __global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const doubl
             partial[(size_t)(ncols + j) * nblk + blockIdx.x] = s[1];
         }
     } else {
+#pragma unroll 4
         for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];
         block_sum<BLOCK, 1>(s, lds);
         if (threadIdx.x == 0) partial[(size_t)(2 * ncols) * nblk + blockIdx.x] = s[0];
@@ -1830,6 +1833,8 @@ __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, const 
     const long long Z = N + (long long)S * nb;
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < Z; c += (long long)gridDim.x * blockDim.x) {
         double v = g[c];
+        // unrolled so that several columns' loads are in flight; the updates keep their order
+#pragma unroll 8
         for (int j = 0; j < ncols; j++) v -= gamma[j] * (dX[(size_t)j * Z + c] + dF[(size_t)j * Z + c]);
         x[c] = v;
         if (c < N) {
