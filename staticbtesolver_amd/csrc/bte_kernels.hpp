@@ -917,6 +917,20 @@ __global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, c
     rhs_levels_range(o, S, dens, rhs_lo, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
 }
 
+// The diffuse re-emission (work-groups [0, n_diff)) and the interior level-order right-hand sides
+// (the rest, one row of positions each: local work-group k is system k / nx, column k % nx) in one
+// launch: the two are independent, so the small diffuse pass runs beside the larger one.
+__global__ void k_diffuse_rhs_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const double* __restrict__ ee_prev,
+                                     double* __restrict__ ediff, const double* __restrict__ dens,
+                                     double* __restrict__ rhs_lo, int n_diff) {
+    if ((int)blockIdx.x < n_diff) {
+        diffuse_range(g, q, bd, B, ee_prev, ediff, blockIdx.x, n_diff);
+    } else {
+        const int k = blockIdx.x - n_diff, nx = (o.N + blockDim.x - 1) / blockDim.x, S = B * o.D;
+        rhs_levels_range(o, S, dens, rhs_lo, k % nx, nx, k / nx, S);
+    }
+}
+
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
 // their face loops run in work-groups of their own instead of stalling the interior waves; one
 // thread per (list entry, band).

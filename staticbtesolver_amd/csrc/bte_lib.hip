@@ -803,12 +803,22 @@ int ring_size(const bte_ctx* ctx, int* sync_every, int G) {
     return 0;
 }
 
-// Right-hand sides of all systems in level order (interior cells, then the boundary-cell list).
-void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out) {
+// Right-hand sides of all systems in level order (interior cells, then the boundary-cell list);
+// with_diffuse: the diffuse re-emission of ee_prev first, in the interior pass's launch.
+void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out,
+                        bool with_diffuse = false) {
     const int S = ctx->B * ctx->D;
-    const dim3 grid_lv((ctx->N + 255) / 256, std::min(S, 65535));
-    hipLaunchKernelGGL(k_rhs_levels, grid_lv, dim3(256), 0, st, geo_of(ctx), quad_of(ctx), bnd_of(ctx), bands_of(ctx),
-                       ops_of(ctx), S, ee_prev, dens, out);
+    if (with_diffuse) {
+        const int n_diff = grid_for((long long)ctx->B * ctx->n_bf, 256);
+        const long long n_rhs = (long long)S * ((ctx->N + 255) / 256);
+        hipLaunchKernelGGL(k_diffuse_rhs_levels, dim3((unsigned)(n_diff + n_rhs)), dim3(256), 0, st, geo_of(ctx),
+                           quad_of(ctx), bnd_of(ctx), ops_of(ctx), ctx->B, ee_prev, ctx->ediff.as<double>(), dens,
+                           out, n_diff);
+    } else {
+        const dim3 grid_lv((ctx->N + 255) / 256, std::min(S, 65535));
+        hipLaunchKernelGGL(k_rhs_levels, grid_lv, dim3(256), 0, st, geo_of(ctx), quad_of(ctx), bnd_of(ctx),
+                           bands_of(ctx), ops_of(ctx), S, ee_prev, dens, out);
+    }
     if (ctx->n_bnd_list > 0)
         hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(grid_for((long long)ctx->n_bnd_list * ctx->B, 64)), dim3(64), 0, st,
                            geo_of(ctx),
@@ -818,7 +828,7 @@ void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, con
 
 template <int BLOCK>
 void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const double* ee_prev, double* ee_curr,
-                    const double* dens) {
+                    const double* dens, bool diffuse_first) {
     SolveCfg cfg{S, o->inner_maxit, o->inner_tol, std::max(1, o->refine_steps), o->precond == BTE_PC_JACOBI ? 1 : 0,
                  o->inner_tol32 > 0 ? o->inner_tol32 : 1e-5};
     const Geo g = geo_of(ctx);
@@ -828,6 +838,12 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     const Ops op = ops_of(ctx);
     const Work w = work_of(ctx, ee_curr);
     dim3 grid(S), blk(BLOCK);
+    // diffuse_first: the diffuse re-emission of ee_prev is still due; the pipelined sweep's
+    // right-hand-side pass takes it into its launch, every other form runs it first
+    const bool v4 = direct_path(ctx, o) && variant == 4 && ring_size(ctx) > 0;
+    if (diffuse_first && !v4)
+        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+                           g, q, bd, ctx->B, ee_prev, ctx->ediff.as<double>());
     if (direct_path(ctx, o)) {
         int sync_every = 1;
         const int ring = ring_size(ctx, &sync_every);
@@ -880,7 +896,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
         } else if (variant == 4 && ring > 0) {
             ctx->variant_used = 4;
             // right-hand sides in level order into w.b, then the pipelined sweep
-            enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b);
+            enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b, diffuse_first);
             const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
             auto go = [&](auto kernel) {
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -938,32 +954,26 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
 }
 
 void launch_solve(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_prev, double* ee_curr,
-                  const double* dens) {
+                  const double* dens, bool diffuse_first = false) {
     const int S = ctx->B * ctx->D;
     switch (sh.block) {
-        case 64: launch_solve_b<64>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
-        case 128: launch_solve_b<128>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
-        case 512: launch_solve_b<512>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
-        case 1024: launch_solve_b<1024>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
-        default: launch_solve_b<256>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens); break;
+        case 64: launch_solve_b<64>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
+        case 128: launch_solve_b<128>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
+        case 512: launch_solve_b<512>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
+        case 1024: launch_solve_b<1024>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
+        default: launch_solve_b<256>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
     }
 }
 
 // The kernels of one outer iteration (diffuse BC data, inner solves, T update, margin).
 int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_prev, double* ee_curr,
                       bool timing) {
-    const Geo g = geo_of(ctx);
-    const Quad q = quad_of(ctx);
-    const Bnd bd = bnd_of(ctx);
     const Bands bn = bands_of(ctx);
-    if (ctx->has_diffuse) {
-        Timer tm(ctx, KA_DIFF, timing);
-        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream, g,
-                           q, bd, ctx->B, ee_prev, ctx->ediff.as<double>());
-    }
     {
+        // the diffuse re-emission of ee_prev, which only the right-hand sides read, goes with the
+        // solve launch (beside the pipelined sweep's right-hand-side pass, or just before)
         Timer tm(ctx, KA_SOLVE, timing);
-        launch_solve(ctx, o, sh, ee_prev, ee_curr, ctx->dens.as<double>());
+        launch_solve(ctx, o, sh, ee_prev, ee_curr, ctx->dens.as<double>(), ctx->has_diffuse);
     }
     const int nblk = grid_for((long long)ctx->B * ctx->N, 256);
     {
