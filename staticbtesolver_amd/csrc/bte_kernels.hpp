@@ -1882,6 +1882,18 @@ __global__ void k_margin(int nparts, const double* __restrict__ partial, double 
     margin_block(nparts, partial, scale, out);
 }
 
+// k_tcell over work-groups [0, n_tcell) and k_margin as work-group n_tcell, in one launch: both
+// read only the band-temperature step's results.
+__global__ __launch_bounds__(256) void k_tcell_margin(int N, Bands bands, const double* __restrict__ Tband,
+                                                      double* __restrict__ T, double* __restrict__ dens, int n_tcell,
+                                                      int nparts, const double* __restrict__ partial, double scale,
+                                                      double* __restrict__ out) {
+    if ((int)blockIdx.x < n_tcell)
+        tcell_range(N, bands, Tband, T, dens, blockIdx.x, n_tcell);
+    else
+        margin_block(nparts, partial, scale, out);
+}
+
 // ----------------------------------------------------------------------------------------------
 // Uneven batch: the per-problem steps of an outer iteration around the batched sweep, one launch
 // per step for all problems.  Work-group t belongs to the problem p with off[p] <= t < off[p + 1]

@@ -980,13 +980,11 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
         Timer tm(ctx, KA_TEMP, timing);
         hipLaunchKernelGGL(k_tband<256>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
                            ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), ctx->partial.as<double>());
-        hipLaunchKernelGGL(k_tcell, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, bn,
-                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>());
-    }
-    {
-        Timer tm(ctx, KA_MARGIN, timing);
-        hipLaunchKernelGGL(k_margin, dim3(1), dim3(256), 0, ctx->stream, nblk, ctx->partial.as<double>(),
-                           1.0 / ctx->B / ctx->N, ctx->margin.as<double>());
+        // cell temperatures and densities, and the margin, in one launch (both read the band step)
+        const int n_tcell = grid_for(ctx->N, 256);
+        hipLaunchKernelGGL(k_tcell_margin, dim3(n_tcell + 1), dim3(256), 0, ctx->stream, ctx->N, bn,
+                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>(), n_tcell, nblk,
+                           ctx->partial.as<double>(), 1.0 / ctx->B / ctx->N, ctx->margin.as<double>());
     }
     HIPCHK(hipGetLastError());
     return BTE_OK;
