@@ -39,7 +39,9 @@
 #include <fstream>
 #include <memory>
 #include <sstream>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -106,6 +108,16 @@ struct Case {
     long long inner_iter_total = 0;
     double worst_true_resid = 0;
     double stop_margin = 1e-5;
+    // exact mode: topological sweep order and lagged entries per direction.  The upwind pattern of
+    // (b, d) depends on d only (the band enters as the scalar v_b tau_b), so the order found for
+    // one band of d serves every band.  Geometry and quadrature are fixed when the case is created,
+    // so the cache holds for the case's lifetime.
+    std::vector<std::vector<int>> order_of_dir;
+    std::vector<std::vector<char>> lagged_of_dir;
+    // workers of an outer iteration's systems (directions split round-robin, as the reference
+    // splits them over MPI ranks); guards err when a worker reports a bad boundary type
+    int n_threads = 1;
+    std::mutex err_mu;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -675,11 +687,13 @@ bool rhs(Case& cs, int b, int d, std::vector<double>& Re) {
                 } else if (type == 31 || type == 32 || type == 33) {
                     int m = mirror_dir(cs, type, d);
                     if (m < 0 || m >= cs.N_dir) {
+                        std::lock_guard<std::mutex> lk(cs.err_mu);
                         cs.err = "boundary condition type " + std::to_string(type) + " not supported here";
                         return false;
                     }
                     Re[c] -= eprev(cs, b, m, c) * a;
                 } else {
+                    std::lock_guard<std::mutex> lk(cs.err_mu);
                     cs.err = "boundary condition type " + std::to_string(type) + " not supported";
                     return false;
                 }
@@ -980,7 +994,7 @@ int bicgstab_ref(const Csr& A, const std::vector<double>& b, std::vector<double>
     return it;
 }
 
-int inner_solve(Case& cs, const Csr& A, const std::vector<double>& b, std::vector<double>& x) {
+int inner_solve(Case& cs, const Csr& A, const std::vector<double>& b, std::vector<double>& x, int dir = -1) {
     const int n = (int)b.size();
     x.assign(n, 0.0);
     if (cs.inner_mode == 1) {  // PETSc default restatement
@@ -997,9 +1011,23 @@ int inner_solve(Case& cs, const Csr& A, const std::vector<double>& b, std::vecto
         return bicgstab_ref(A, b, x, tol, maxit);
     }
     // exact
-    std::vector<int> order;
-    std::vector<char> lagged;
-    sweep_order(A, order, lagged);
+    std::vector<int> order_tmp;
+    std::vector<char> lagged_tmp;
+    const std::vector<int>* op = &order_tmp;
+    const std::vector<char>* lp = &lagged_tmp;
+    if (dir >= 0) {
+        if ((int)cs.order_of_dir.size() <= dir) {
+            cs.order_of_dir.resize(dir + 1);
+            cs.lagged_of_dir.resize(dir + 1);
+        }
+        if (cs.order_of_dir[dir].empty()) sweep_order(A, cs.order_of_dir[dir], cs.lagged_of_dir[dir]);
+        op = &cs.order_of_dir[dir];
+        lp = &cs.lagged_of_dir[dir];
+    } else {
+        sweep_order(A, order_tmp, lagged_tmp);
+    }
+    const std::vector<int>& order = *op;
+    const std::vector<char>& lagged = *lp;
     bool cyclic = false;
     for (char l : lagged) cyclic |= (l != 0);
     if (!cyclic) {
@@ -1072,6 +1100,53 @@ void heat_flux(Case& cs) {
     }
 }
 
+// The systems [sys_begin, sys_end) of one outer iteration (StaticBTESolver.cpp:724-780): each
+// (band, direction) system assembled (_get_Ke), given its right-hand side (_get_Re) and solved into
+// its slice of ee_curr.  With W = n_threads workers, worker w takes the directions d = w, w + W, ...
+// of every band -- the reference's split of directions over MPI ranks (:725) -- so the workers
+// write disjoint slices and each direction's cached sweep order belongs to one worker.  Every
+// system's arithmetic is the serial loop's.  Returns false on a right-hand-side error.
+bool solve_systems(Case& cs, int sys_begin, int sys_end, bool check_resid) {
+    const int N = cs.N_cell, D = cs.N_dir;
+    const int W = std::max(1, std::min(cs.n_threads, D));
+    if ((int)cs.order_of_dir.size() < D) {
+        cs.order_of_dir.resize(D);
+        cs.lagged_of_dir.resize(D);
+    }
+    std::vector<long long> its(W, 0);
+    std::vector<double> worst(W, 0.0);
+    std::vector<char> ok(W, 1);
+    auto work = [&](int w) {
+        Csr A;
+        std::vector<double> Re, sol;
+        for (int s = sys_begin; s < sys_end; s++) {
+            const int b = s / D, d = s % D;
+            if (d % W != w) continue;
+            assemble(cs, b, d, A);
+            if (!rhs(cs, b, d, Re)) {
+                ok[w] = 0;
+                return;
+            }
+            its[w] += inner_solve(cs, A, Re, sol, d);
+            if (check_resid) worst[w] = std::max(worst[w], true_rel_resid(A, sol, Re));
+            std::copy(sol.begin(), sol.end(), cs.ee_curr.begin() + ((size_t)b * D + d) * N);
+        }
+    };
+    if (W == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < W; w++) pool.emplace_back(work, w);
+        for (auto& t : pool) t.join();
+    }
+    for (int w = 0; w < W; w++) {
+        cs.inner_iter_total += its[w];
+        cs.worst_true_resid = std::max(cs.worst_true_resid, worst[w]);
+        if (!ok[w]) return false;
+    }
+    return true;
+}
+
 // _iteration (:690-808) + _postprocess heat flux (:810-835), all systems in order.
 int solve(Case& cs, int max_iter, int verbose) {
     if (!preprocess(cs)) return -1;
@@ -1083,23 +1158,13 @@ int solve(Case& cs, int max_iter, int verbose) {
     cs.iters = 0;
     cs.inner_iter_total = 0;
     cs.worst_true_resid = 0;
-    Csr A;
-    std::vector<double> Re, sol;
     if (verbose)
         std::printf("N_cell: %d\nN_dir: %d\nN_band: %d\nnum_theta: %d\nnum_phi: %d\n\n", N, D, B, cs.num_theta,
                     cs.num_phi);
     for (int it = 0; it < max_iter; it++) {
         cs.ee_prev = cs.ee_curr;
         std::fill(cs.ee_curr.begin(), cs.ee_curr.end(), 0.0);
-        for (int b = 0; b < B; b++)
-            for (int d = 0; d < D; d++) {
-                assemble(cs, b, d, A);
-                if (!rhs(cs, b, d, Re)) return -1;
-                cs.inner_iter_total += inner_solve(cs, A, Re, sol);
-                double tr = true_rel_resid(A, sol, Re);
-                if (tr > cs.worst_true_resid) cs.worst_true_resid = tr;
-                std::copy(sol.begin(), sol.end(), cs.ee_curr.begin() + ((size_t)b * D + d) * N);
-            }
+        if (!solve_systems(cs, 0, B * D, true)) return -1;
         update_temperature(cs);
         double m = margin(cs);
         cs.margins.push_back(m);
@@ -1283,21 +1348,17 @@ int orc_sweep_levels(int n, const int* rowptr, const int* col, int* level) {
 
 int orc_solve(orc_case* cs, int max_iter, int verbose) { return solve(*cs, max_iter, verbose); }
 
+// Workers of the outer iteration's systems (1: the serial loop, as one reference MPI rank).
+void orc_set_threads(orc_case* cs, int n) { cs->n_threads = std::max(1, n); }
+
 // Runs `n_outer` outer iterations on a subset of systems [sys_begin, sys_end) of the first
 // iteration only (timing sample for the CPU baseline).  Returns elapsed seconds.
 double orc_time_systems(orc_case* cs, int sys_begin, int sys_end) {
     const int N = cs->N_cell, B = cs->N_band, D = cs->N_dir;
     cs->ee_prev.assign((size_t)B * D * N, 0.0);
     cs->ee_curr.assign((size_t)B * D * N, 0.0);
-    Csr A;
-    std::vector<double> Re, sol;
     auto t0 = std::chrono::steady_clock::now();
-    for (int s = sys_begin; s < sys_end; s++) {
-        int b = s / D, d = s % D;
-        assemble(*cs, b, d, A);
-        if (!rhs(*cs, b, d, Re)) return -1;
-        cs->inner_iter_total += inner_solve(*cs, A, Re, sol);
-    }
+    if (!solve_systems(*cs, sys_begin, sys_end, false)) return -1;
     auto t1 = std::chrono::steady_clock::now();
     return std::chrono::duration<double>(t1 - t0).count();
 }
@@ -1329,19 +1390,11 @@ int orc_begin(orc_case* cs) {
 }
 
 double orc_step(orc_case* cs, int n) {
-    const int N = cs->N_cell, B = cs->N_band, D = cs->N_dir;
-    Csr A;
-    std::vector<double> Re, sol;
+    const int B = cs->N_band, D = cs->N_dir;
     double m = 0;
     for (int it = 0; it < n; it++) {
         cs->ee_prev = cs->ee_curr;
-        for (int b = 0; b < B; b++)
-            for (int d = 0; d < D; d++) {
-                assemble(*cs, b, d, A);
-                if (!rhs(*cs, b, d, Re)) return -1;
-                inner_solve(*cs, A, Re, sol);
-                std::copy(sol.begin(), sol.end(), cs->ee_curr.begin() + ((size_t)b * D + d) * N);
-            }
+        if (!solve_systems(*cs, 0, B * D, false)) return -1;
         update_temperature(*cs);
         m = margin(*cs);
         cs->margins.push_back(m);

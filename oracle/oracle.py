@@ -44,6 +44,7 @@ def lib():
     L.orc_params.argtypes = [C.c_void_p, dp]
     L.orc_set_inner.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int]
     L.orc_set_stop.argtypes = [C.c_void_p, C.c_double]
+    L.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
     L.orc_quadrature.argtypes = [C.c_void_p, dp, dp, dp]
     L.orc_geometry.argtypes = [C.c_void_p, dp, dp, dp, dp, ip]
     L.orc_mesh.argtypes = [C.c_void_p, dp, ip]
@@ -144,6 +145,10 @@ class OracleCase:
 
     def set_stop(self, stop):
         lib().orc_set_stop(self.h, stop)
+
+    def set_threads(self, n):
+        """Workers of an outer iteration's systems, directions split round-robin (1: serial)."""
+        lib().orc_set_threads(self.h, int(n))
 
     def quadrature(self):
         D = self.N_dir

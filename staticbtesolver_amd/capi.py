@@ -326,6 +326,42 @@ class Case:
                     band_group=s.band_group)
 
 
+def ctx_set_bands(ctx, v, tau, Cv, Lr):
+    """bte_set_bands on a context (the C-ABI setter, without the facade's re-upload)."""
+    v, tau, Cv, Lr = [np.ascontiguousarray(a, dtype=np.float64) for a in (v, tau, Cv, Lr)]
+    rc = lib().bte_set_bands(ctx, len(v), _dp(v), _dp(tau), _dp(Cv), _dp(Lr))
+    if rc != BTE_OK:
+        raise BTEError(rc, lib().bte_last_error(ctx).decode())
+
+
+def ctx_set_bcs(ctx, index, types, temps):
+    """bte_set_bcs on a context (index = -k-1 per the reference's convention)."""
+    index = np.ascontiguousarray(index, dtype=np.int32)
+    types = np.ascontiguousarray(types, dtype=np.int32)
+    temps = np.ascontiguousarray(temps, dtype=np.float64)
+    rc = lib().bte_set_bcs(ctx, len(index), _ip(index), _ip(types), _dp(temps))
+    if rc != BTE_OK:
+        raise BTEError(rc, lib().bte_last_error(ctx).decode())
+
+
+def ctx_solve(ctx, N, B, nbc, opts: Opts | None = None, **kw):
+    """bte_solve on a context with the inputs it already holds."""
+    if opts is None:
+        opts = default_opts(**kw)
+    cap = max(1, opts.max_iter)
+    T, Tb, fl, bfl, m = np.zeros(N), np.zeros(N * B), np.zeros(max(nbc, 1)), np.zeros(max(nbc, 1) * B), np.zeros(cap)
+    r = Result()
+    r.T, r.T_band, r.bc_flux, r.bc_band_flux, r.margins = _dp(T), _dp(Tb), _dp(fl), _dp(bfl), _dp(m)
+    r.margins_capacity = cap
+    rc = lib().bte_solve(ctx, C.byref(opts), C.byref(r))
+    if rc not in (BTE_OK, BTE_ERR_NOCONV):
+        raise BTEError(rc, lib().bte_last_error(ctx).decode())
+    it = r.iterations
+    return dict(T=T, T_band=Tb.reshape(N, B), flux=fl[:nbc], band_flux=bfl[:nbc * B].reshape(nbc, B),
+                margins=m[:min(it, cap)].copy(), iters=it, converged=bool(r.converged),
+                inner_iters=int(r.inner_iters), inner_failures=int(r.inner_failures), status=rc)
+
+
 class Batch:
     """Several loaded cases solved together (bte_batch_*): each outer iteration sweeps the systems of
     all still-iterating cases in one launch.  The cases must stay open while the batch exists."""

@@ -1568,55 +1568,20 @@ __global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg c
     }
 }
 
-// ----------------------------------------------------------------------------------------------
-// Temperature update (StaticBTESolver.cpp:96-120) fused with the margin partial sums (:311-321).
-// One thread per cell; per-block partial of sum (e_curr - e_prev)^2.
-// ----------------------------------------------------------------------------------------------
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_temperature(int N, int D, Bands bands, const double* __restrict__ omega,
-                                                      const double* __restrict__ ee_curr,
-                                                      const double* __restrict__ ee_prev, double* __restrict__ Tband,
-                                                      double* __restrict__ T, double* __restrict__ dens,
-                                                      double* __restrict__ partial) {
-    __shared__ double lds[BLOCK / 64];
-    double mp[1] = {0};
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    if (c < N) {
-        double rrn = 0, rnn = 0;
-        for (int b = 0; b < bands.B; b++) {
-            double e0 = 0;
-            const size_t base = (size_t)b * D * N + c;
-            for (int d = 0; d < D; d++) {
-                const double ec = ee_curr[base + (size_t)d * N];
-                const double ep = ee_prev[base + (size_t)d * N];
-                e0 += ec * omega[d];
-                const double df = ec - ep;
-                mp[0] += df * df;
-            }
-            const double tb = e0 / bands.C[b] + bands.T_ref;
-            Tband[(size_t)b * N + c] = tb;
-            rrn += bands.Lr[b] * tb;
-            rnn += bands.Lr[b];
-        }
-        const double tc = rrn / rnn;
-        T[c] = tc;
-        for (int b = 0; b < bands.B; b++) dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
-    }
-    block_sum<BLOCK, 1>(mp, lds);
-    if (threadIdx.x == 0) partial[blockIdx.x] = mp[0];
-}
-
 // Band temperatures (StaticBTESolver.cpp:96-104) with one thread per (band, cell) -- B times more
 // threads than a cell loop, so enough loads are in flight to stream both intensity arrays at HBM
-// rate -- fused with the margin partial sums (:311-321).
+// rate -- fused with the margin partial sums (:311-321).  Work-group blk of nblk strides over the
+// B*N pairs (grids are capped, B*N is not) and writes partial[blk]; each thread adds its pairs in
+// increasing order, so the sums are the same on every call with the same grid.
 template <int BLOCK>
 __device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, const double* __restrict__ omega,
                                             const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
-                                            double* __restrict__ Tband, double* __restrict__ partial, int blk) {
+                                            double* __restrict__ Tband, double* __restrict__ partial, int blk,
+                                            int nblk) {
     __shared__ double lds[BLOCK / 64];
     double mp[1] = {0};
-    const long long t = blk * (long long)BLOCK + threadIdx.x;
-    if (t < (long long)bands.B * N) {
+    const long long total = (long long)bands.B * N;
+    for (long long t = blk * (long long)BLOCK + threadIdx.x; t < total; t += (long long)nblk * BLOCK) {
         const int b = (int)(t / N), c = (int)(t % N);
         const size_t base = (size_t)b * D * N + c;
         double e0 = 0;
@@ -1637,7 +1602,7 @@ template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, const double* __restrict__ omega,
                                                 const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
                                                 double* __restrict__ Tband, double* __restrict__ partial) {
-    tband_block<BLOCK>(N, D, bands, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x);
+    tband_block<BLOCK>(N, D, bands, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x, gridDim.x);
 }
 
 // Equilibrium temperature and emission density per cell (StaticBTESolver.cpp:106-120).
@@ -1965,7 +1930,7 @@ __global__ __launch_bounds__(256) void k_tband_multi(const IterDesc* __restrict_
                                                      int n) {
     const MultiIdx m = multi_index(off, n);
     const IterDesc& D = desc[m.p];
-    tband_block<256>(D.o.N, D.o.D, D.bands, D.omega, D.ee_curr, D.ee_prev, D.Tband, D.partial, m.blk);
+    tband_block<256>(D.o.N, D.o.D, D.bands, D.omega, D.ee_curr, D.ee_prev, D.Tband, D.partial, m.blk, m.nblk);
 }
 
 __global__ void k_tcell_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {

@@ -69,9 +69,11 @@ struct bte_ctx {
     double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients, then [129] their dot products
     int aa_slot = 0;            // history column of the step between aa_launch and aa_finish
     bool aa_skip_now = false;   // that step follows an accepted correction
-    // graphs
+    // graphs, and the options and launch shape they were captured with (graph_valid: they exist)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_key = -1;
+    bool graph_valid = false;
+    bte_opts graph_opts{};
+    int graph_block = 0, graph_variant = 0;
     // stats
     double ms[KA_COUNT] = {0};
     long long calls[KA_COUNT] = {0};
@@ -283,7 +285,7 @@ void drop_graphs(bte_ctx* ctx) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
-    ctx->graph_key = -1;
+    ctx->graph_valid = false;
     ctx->run_active = false;  // captured launches referenced buffers that may change
 }
 
@@ -372,6 +374,7 @@ int bte_set_geometry(bte_ctx* ctx, int dim, int n_cell, int n_face, const double
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_geo = true;
     ctx->prepared = false;
+    ctx->learn_stat_ok = false;
     ctx->geo_tag = ++g_geo_tags;
     drop_graphs(ctx);
     return BTE_OK;
@@ -420,6 +423,7 @@ int bte_set_bands(bte_ctx* ctx, int n_band, const double* v, const double* tau, 
     HIPCHK(upload(ctx->vt, vt, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_bands = true;
+    ctx->learn_stat_ok = false;
     drop_graphs(ctx);
     return BTE_OK;
 }
@@ -435,6 +439,7 @@ int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const
     ctx->h_bcT.assign(T, T + n_bc);
     ctx->have_bcs = true;
     ctx->prepared = false;
+    ctx->learn_stat_ok = false;
     drop_graphs(ctx);
     return BTE_OK;
 }
@@ -487,7 +492,6 @@ int prepare(bte_ctx* ctx, bool timing) {
     HIPCHK(upload(ctx->bf_face, bff, ctx->stream));
     HIPCHK(upload(ctx->bfid, bfid, ctx->stream));
     HIPCHK(upload(ctx->mirror, mir, ctx->stream));
-    HIPCHK(ctx->ediff.ensure(sizeof(double) * std::max<size_t>(1, (size_t)ctx->B * ctx->n_bf)));
     // effective angular weight of the volume term: omega_d in 2D/3D, 1 in 1D (StaticBTESolver.cpp:143-147)
     std::vector<double> weff(ctx->h_omega);
     if (ctx->dim == 1) std::fill(weff.begin(), weff.end(), 1.0);
@@ -669,9 +673,21 @@ int ensure_fp32(bte_ctx* ctx) {
     return BTE_OK;
 }
 
+// Buffers sized by inputs that the setters change without changing B*D*N: the heat flux per (BC,
+// band) and the diffuse re-emission per (band, boundary face).  A buffer that had to grow moves,
+// so graphs captured with the old address are dropped.
+int ensure_small(bte_ctx* ctx) {
+    const void* before[2] = {ctx->bflux.p, ctx->ediff.p};
+    HIPCHK(ctx->bflux.ensure(sizeof(double) * std::max<size_t>(1, ctx->h_bci.size() * ctx->B)));
+    HIPCHK(ctx->ediff.ensure(sizeof(double) * std::max<size_t>(1, (size_t)ctx->B * ctx->n_bf)));
+    if (before[0] != ctx->bflux.p || before[1] != ctx->ediff.p) drop_graphs(ctx);
+    return BTE_OK;
+}
+
 int ensure_work(bte_ctx* ctx, int precision) {
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
     const int S = ctx->B * ctx->D;
+    if (int rc = ensure_small(ctx)) return rc;
     if (ctx->work_elems == SN && ctx->work_prec == precision) return BTE_OK;
     // Krylov vectors are FP64-sized in both precisions: the mixed form re-solves in FP64 a system
     // whose FP32 inner solve stalls
@@ -692,7 +708,6 @@ int ensure_work(bte_ctx* ctx, int precision) {
     HIPCHK(ctx->Tband.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
     HIPCHK(ctx->partial.ensure(sizeof(double) * grid_for((long long)ctx->B * ctx->N, 256)));
     HIPCHK(ctx->margin.ensure(sizeof(double) * 2));
-    HIPCHK(ctx->bflux.ensure(sizeof(double) * std::max<size_t>(1, ctx->h_bci.size() * ctx->B)));
     ctx->work_elems = SN;
     ctx->work_prec = precision;
     drop_graphs(ctx);
@@ -1525,6 +1540,17 @@ static int check_opts(bte_ctx* ctx, const bte_opts& o) {
     return BTE_OK;
 }
 
+// Whether the captured iteration graphs were recorded with the options and launch shape that
+// enqueue_iteration would use now: the solver form and every SolveCfg field baked into the launches.
+static bool same_capture(const bte_ctx* ctx, const bte_opts& o, const Shape& sh) {
+    const bte_opts& g = ctx->graph_opts;
+    return ctx->graph_valid && ctx->graph[0] && ctx->graph[1] && ctx->graph_block == sh.block &&
+           ctx->graph_variant == sh.variant && g.precision == o.precision && g.precond == o.precond &&
+           (g.direct_sweep != 0) == (o.direct_sweep != 0) && g.inner_maxit == o.inner_maxit &&
+           g.inner_tol == o.inner_tol && std::max(1, g.refine_steps) == std::max(1, o.refine_steps) &&
+           g.inner_tol32 == o.inner_tol32;
+}
+
 int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     if (!ctx) return BTE_ERR_ARG;
     bte_opts o;
@@ -1565,11 +1591,8 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
-    // graph key: everything that changes the captured launches
-    const int key = (o.precision << 0) | (o.precond << 1) | ((o.direct_sweep ? 1 : 0) << 2) | (shape.variant << 3) |
-                    (shape.block << 5);
     const bool use_graph = o.use_graph && !timing;
-    if (use_graph && ctx->graph_key != key) {
+    if (use_graph && !same_capture(ctx, o, shape)) {
         drop_graphs(ctx);
         for (int par = 0; par < 2; par++) {
             hipGraph_t g = nullptr;
@@ -1581,10 +1604,12 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
             HIPCHK(hipGraphInstantiate(&ctx->graph[par], g, nullptr, nullptr, 0));
             HIPCHK(hipGraphDestroy(g));
         }
-        ctx->graph_key = key;
+        ctx->graph_valid = true;
+        ctx->graph_opts = o;
+        ctx->graph_block = shape.block;
+        ctx->graph_variant = shape.variant;
     }
     ctx->run_opts = o;
-    ctx->learn_stat_ok = false;
     ctx->aa_plain = o.accel != BTE_ACCEL_ANDERSON;
     if (o.accel == BTE_ACCEL_ANDERSON || o.learned) {
         if ((rc = aa_reset(ctx))) return rc;
