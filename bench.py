@@ -201,24 +201,57 @@ def alg_bytes(case, w, direct, per_sys=1.0, jacobi=False):
     return ((2.0 * shared + 21.0 * w * n * S + jac) * per_sys, (2.0 * canon + 21.0 * w * n * S) * per_sys)
 
 
+def cpu_model():
+    """(model name, logical CPUs online, CPUs this process may use) of the host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, os.cpu_count(), usable
+
+
 def cpu_baseline(kw, outer_iters, budget_s=15.0):
-    """PETSc-default restatement (GMRES(30) + ILU(0), rtol 1e-9) timed on a bounded sample: the
-    first outer iteration's systems, taken in order until budget_s of CPU time, then extrapolated
-    to all systems and to the plain form's outer-iteration count."""
+    """The oracle's PETSc-default restatement (GMRES(30) + ILU(0), rtol 1e-9, one system at a time
+    like the reference's _solve_matrix) timed on bounded samples of the first outer iteration and
+    extrapolated to all systems and to the plain form's outer-iteration count.
+
+    - 1 core: systems in order until budget_s of CPU time.
+    - all cores: W workers, W the largest divisor of N_dir not above the usable CPUs (at most 16),
+      each taking the directions d = w, w + W, ... of every band, as the reference splits
+      directions over MPI ranks (StaticBTESolver.cpp:725); sampled over whole bands (every
+      direction of the first k bands) to about budget_s of wall time."""
     from oracle import oracle as O
 
     o = O.OracleCase(**kw)
     o.set_inner(O.INNER_PETSC)
-    S = o.N_band * o.N_dir
+    D, B = o.N_dir, o.N_band
+    S = B * D
+    o.set_threads(1)
     done, spent = 0, 0.0
     while done < S and spent < budget_s:
         spent += o.time_systems(done, done + 1)
         done += 1
     per_sys = spent / done
-    est = per_sys * S * outer_iters
-    return dict(value=est, unit="s (extrapolated)", cores=1, kind="port",
-                sample=f"first outer iteration, {done}/{S} systems in {spent:.1f}s on 1 core (GMRES(30)+ILU(0) "
-                       f"restatement); x{S} systems x{outer_iters} outer iterations of the plain form")
+    one = dict(value=per_sys * S * outer_iters, unit="s (extrapolated)", cores=1,
+               sample=f"first outer iteration, {done}/{S} systems in {spent:.2f}s")
+    model, n_online, n_usable = cpu_model()
+    cap = max(1, min(16, n_usable or 1))
+    W = max(w for w in range(1, D + 1) if D % w == 0 and w <= cap)
+    o.set_threads(W)
+    k = int(max(1, min(B, budget_s * W / max(per_sys * D, 1e-9))))
+    wall = o.time_systems(0, k * D)
+    per_sys_all = wall / (k * D)
+    return dict(value=per_sys_all * S * outer_iters, unit="s (extrapolated)", cores=W, kind="port",
+                sample=f"first outer iteration, {k * D}/{S} systems (bands 0..{k - 1}, all directions) in "
+                       f"{wall:.1f}s on {W} worker threads, directions round-robin (GMRES(30)+ILU(0) "
+                       f"restatement); x{S} systems x{outer_iters} outer iterations of the plain form",
+                cpu_model=model, nproc=n_online, cpus_usable=n_usable, one_core=one)
 
 
 def roofline_only(case, form, k, prec):

@@ -140,6 +140,81 @@ def cube_mesh(n: int, seed: int, jitter: float = 0.12):
     return pts, tet, faces, ents
 
 
+# ---------------------------------------------------------------------------------------------
+# 3D: Delaunay tetrahedralisation of seeded points in the unit cube
+# ---------------------------------------------------------------------------------------------
+def cube_points(n_tets: int, seed: int) -> np.ndarray:
+    """Seeded points of the unit cube for about n_tets Delaunay tetrahedra (~7 per point): the 8
+    corners, m-1 jittered points on each edge, a jittered (m-1)^2 grid on each face and uniform
+    random interior points.  Boundary points are jittered within their edge or face, so no four of
+    them are cocircular on a face and the triangulation has no flat tetrahedra."""
+    rng = np.random.default_rng(seed)
+    n_in = max(8, int(n_tets / 7.0))
+    m = max(2, int(round(n_in ** (1.0 / 3.0))))
+    h = 1.0 / m
+    parts = [np.array([[x, y, z] for x in (0.0, 1.0) for y in (0.0, 1.0) for z in (0.0, 1.0)])]
+    for ax in range(3):
+        o = [k for k in range(3) if k != ax]
+        for a in (0.0, 1.0):
+            for b in (0.0, 1.0):
+                q = np.zeros((m - 1, 3))
+                q[:, ax] = (np.arange(1, m) + rng.uniform(-0.3, 0.3, m - 1)) * h
+                q[:, o[0]], q[:, o[1]] = a, b
+                parts.append(q)
+    for ax in range(3):
+        o = [k for k in range(3) if k != ax]
+        for val in (0.0, 1.0):
+            q = np.zeros(((m - 1) ** 2, 3))
+            q[:, o[0]] = (np.repeat(np.arange(1, m), m - 1) + rng.uniform(-0.35, 0.35, (m - 1) ** 2)) * h
+            q[:, o[1]] = (np.tile(np.arange(1, m), m - 1) + rng.uniform(-0.35, 0.35, (m - 1) ** 2)) * h
+            q[:, ax] = val
+            parts.append(q)
+    parts.append(rng.uniform(0.3 * h, 1.0 - 0.3 * h, (n_in, 3)))
+    return np.concatenate(parts)
+
+
+def _signed_volumes(pts, tet):
+    P = pts[tet]
+    return np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0]) / 6
+
+
+def cube_delaunay_mesh(n_tets: int, seed: int, distort: float = 0.0):
+    """Delaunay tetrahedra of cube_points(n_tets, seed).  Returns (points, tets, boundary faces,
+    boundary entity ids).
+
+    The in-front relation of a Delaunay triangulation is acyclic from every viewpoint, so its upwind
+    graphs have no cycles for any direction.  With distort > 0 each interior point then moves, in
+    random order and three passes, by a normal step of distort * h per coordinate (h the mean point
+    spacing), kept only where every tetrahedron around the point keeps its orientation and at least
+    2 % of the mean volume: the connectivity stays, the mesh stops being Delaunay, and some
+    directions' upwind graphs get cycles."""
+    from scipy.spatial import Delaunay
+
+    pts = cube_points(n_tets, seed)
+    tet = Delaunay(pts).simplices.astype(np.int64)
+    vol = _signed_volumes(pts, tet)
+    if np.abs(vol).min() <= 1e-9 * np.abs(vol).mean():
+        raise RuntimeError("degenerate tetrahedron in the triangulation")
+    if distort > 0:
+        rng = np.random.default_rng(seed + 7919)
+        sign = np.sign(vol)
+        floor = 0.02 * np.abs(vol).mean()
+        h = len(pts) ** (-1.0 / 3.0)
+        inner = np.nonzero(~np.any((pts < 1e-12) | (pts > 1 - 1e-12), axis=1))[0]
+        order = np.argsort(tet.ravel(), kind="stable")
+        starts = np.searchsorted(tet.ravel()[order], np.arange(len(pts) + 1))
+        around = [order[starts[v]:starts[v + 1]] // 4 for v in range(len(pts))]
+        for _ in range(3):
+            for v in rng.permutation(inner):
+                old = pts[v].copy()
+                pts[v] = old + rng.normal(size=3) * distort * h
+                t = around[v]
+                if not np.all(_signed_volumes(pts, tet[t]) * sign[t] > floor):
+                    pts[v] = old
+    faces, ents = _boundary_faces(tet, pts, 3)
+    return pts, tet, faces, ents
+
+
 def _boundary_faces(cells, pts, dim):
     """Faces used by exactly one cell, with the entity id of the box side they lie on."""
     nv = dim + 1
@@ -209,9 +284,15 @@ def make_square_case(directory, n_cells, seed, n_band, band_seed, L=1e-7, bc_row
     return dict(mesh=mesh, band=band, bc=bc, n_cells=len(tri), L=L)
 
 
-def make_cube_case(directory, n, seed, n_band, band_seed, L=1e-7, bc_rows=None, name="cube"):
+def make_cube_case(directory, n, seed, n_band, band_seed, L=1e-7, bc_rows=None, name="cube", delaunay_tets=0,
+                   distort=0.0):
+    """Cube case files: the Kuhn lattice of n^3 cubes, or with delaunay_tets > 0 the Delaunay
+    tetrahedra of about that many from seeded points (cube_delaunay_mesh, distorted by distort)."""
     os.makedirs(directory, exist_ok=True)
-    pts, tet, faces, ents = cube_mesh(n, seed)
+    if delaunay_tets > 0:
+        pts, tet, faces, ents = cube_delaunay_mesh(delaunay_tets, seed, distort)
+    else:
+        pts, tet, faces, ents = cube_mesh(n, seed)
     mesh = os.path.join(directory, f"{name}.mphtxt")
     write_mphtxt(mesh, pts, tet, faces, ents)
     band = os.path.join(directory, f"{name}_band.dat")

@@ -73,3 +73,43 @@ def test_oracle_2dcase2_against_cpu_and_gpu_logs():
     assert rel[:4].max() < 1e-5 and rel.max() < 0.1
     main = np.abs(cf) > 1e-3 * np.abs(cf).max()
     assert (np.abs(r["flux"] - cf)[main] / np.abs(cf)[main]).max() < 2e-5
+
+
+def test_bicgstab_restatement_restarts_on_vanishing_denominator():
+    """bicgstab.hpp:211-222: when <r_hat, A M^-1 p> vanishes the solve restarts from x = 0 with
+    r_hat = 6.8 b + 0.6.  A = [[1, -2.5], [0, 1]] (Jacobi M = I), b = (1, 2): the first step has
+    p = (1 + rho) b = 6 b (quirk Q14) and <b, A 6b> = 6 (5 - 2.5 * 2) = 0 exactly, so only the
+    restart reaches the solution (6, 2); without it alpha = rho / 0."""
+    rp = np.array([0, 2, 3], dtype=np.int32)
+    col = np.array([0, 1, 1], dtype=np.int32)
+    val = np.array([1.0, -2.5, 1.0])
+    b = np.array([1.0, 2.0])
+    x, it = O.inner_solve(O.INNER_BICGSTAB, rp, col, val, b)
+    assert 2 <= it <= 4
+    np.testing.assert_allclose(x, [6.0, 2.0], rtol=1e-12)
+    # the same solver has no guard for s = r - alpha q = 0 (a step that solves the system exactly):
+    # omega = <t, s> / <t, t> = 0 / 0 and the result is NaN, as in bicgstab.hpp:271-274
+    x2, _ = O.inner_solve(O.INNER_BICGSTAB, np.array([0, 1, 2], dtype=np.int32), np.array([0, 1], dtype=np.int32),
+                          np.array([1.0, -1.0]), np.array([1.0, 1.0]))
+    assert np.all(np.isnan(x2))
+
+
+@pytest.mark.slow
+def test_bicgstab_restatement_on_2dcase2_gpu_log():
+    """The 2Dcase2 GPU log was printed by the reference's BiCGSTAB (bicgstab.hpp).  The restatement
+    stops its inner solves at the same 1e-9 recursive residual, so its margins carry noise of that
+    size: they agree with the log to 1.2e-4 over the first 30 iterations and 3e-3 at the tail, the
+    fluxes to 5e-5.  (The exact inner solves sit closer to the log: 7e-6 over the first 30.)"""
+    d = os.path.join(G, "2Dcase2")
+    bc31 = os.path.join(os.path.dirname(G), "2Dcase2_bc31.dat")
+    c = O.OracleCase(os.path.join(d, "2D_12000.mphtxt"), os.path.join(d, "Input-dispersion-relation-fp.dat"),
+                     bc31, 2, 8, 8, 300.0, Lx=1e-8, Ly=1e-8)
+    c.set_inner(O.INNER_BICGSTAB)
+    c.set_stop(1e-4)
+    c.set_threads(min(8, os.cpu_count() or 1))
+    r = c.solve(1000)
+    gm, gf = parse_sol(os.path.join(d, "StaticBTESolver-2DM-1np.sol"))
+    assert r["iters"] == len(gm)
+    rel = np.abs(r["margins"] - gm) / gm
+    assert rel[:30].max() < 2e-4 and rel.max() < 5e-3
+    assert (np.abs(r["flux"] - gf) / np.abs(gf)).max() < 5e-5
