@@ -153,6 +153,17 @@ int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const
  * BTE_ERR_NOCONV when some inner solve hit inner_maxit (results are still filled in). */
 int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res);
 
+/* bte_solve without device memory allocation.  No counterpart in the reference: for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The reductions (margin partial sums, squared norms, Anderson dot products, heat-flux staging) use
+ * a 1 MiB array per device in the library's code object, with room for 56,992 (BC, band) pairs
+ * (more are rejected with BTE_ERR_ARG); every other device buffer must already exist from
+ * an earlier bte_solve of the same form with inputs at least as large, else BTE_ERR_STATE.  No
+ * graphs are captured or replayed and the launch shape is not autotuned here.  All contexts on one
+ * device share that device's array: calls that overlap on one device race.  Results equal
+ * bte_solve's with use_graph = 0 and the same launch shape. */
+int bte_solve_noalloc(bte_ctx* ctx, const bte_opts* opts, bte_result* res);
+
 /* The same iteration in steps (bte_solve = bte_begin + bte_step(max_iter) + bte_finish), for
  * callers that inspect or correct the temperature field between outer iterations.
  * bte_begin: prepares and resets the state (e = 0, density = 0 as StaticBTESolver.cpp:702-705).

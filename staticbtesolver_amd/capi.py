@@ -67,7 +67,7 @@ class Stats(C.Structure):
 
 # symbols declared in include/bte.h and include/bte_case.h
 HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_last_error", "bte_set_geometry",
-               "bte_set_quadrature", "bte_set_bands", "bte_set_bcs", "bte_solve", "bte_get_intensity",
+               "bte_set_quadrature", "bte_set_bands", "bte_set_bcs", "bte_solve", "bte_solve_noalloc", "bte_get_intensity",
                "bte_get_stats", "bte_debug_assemble", "bte_debug_rhs", "bte_debug_inner", "bte_begin", "bte_step",
                "bte_get_temperature", "bte_set_temperature", "bte_finish", "bte_gnn_create", "bte_gnn_destroy",
                "bte_gnn_last_error", "bte_gnn_num_params", "bte_gnn_get_params", "bte_gnn_set_params",
@@ -101,6 +101,7 @@ def lib():
     L.bte_set_bands.argtypes = [vp, C.c_int, dp, dp, dp, dp]
     L.bte_set_bcs.argtypes = [vp, C.c_int, ip, ip, dp]
     L.bte_solve.argtypes = [vp, C.POINTER(Opts), C.POINTER(Result)]
+    L.bte_solve_noalloc.argtypes = [vp, C.POINTER(Opts), C.POINTER(Result)]
     L.bte_get_intensity.argtypes = [vp, dp]
     L.bte_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bte_debug_assemble.argtypes = [vp, C.c_int, C.c_int, dp, dp]
@@ -344,8 +345,8 @@ def ctx_set_bcs(ctx, index, types, temps):
         raise BTEError(rc, lib().bte_last_error(ctx).decode())
 
 
-def ctx_solve(ctx, N, B, nbc, opts: Opts | None = None, **kw):
-    """bte_solve on a context with the inputs it already holds."""
+def ctx_solve(ctx, N, B, nbc, opts: Opts | None = None, noalloc=False, **kw):
+    """bte_solve (bte_solve_noalloc with noalloc) on a context with the inputs it already holds."""
     if opts is None:
         opts = default_opts(**kw)
     cap = max(1, opts.max_iter)
@@ -353,7 +354,8 @@ def ctx_solve(ctx, N, B, nbc, opts: Opts | None = None, **kw):
     r = Result()
     r.T, r.T_band, r.bc_flux, r.bc_band_flux, r.margins = _dp(T), _dp(Tb), _dp(fl), _dp(bfl), _dp(m)
     r.margins_capacity = cap
-    rc = lib().bte_solve(ctx, C.byref(opts), C.byref(r))
+    fn = lib().bte_solve_noalloc if noalloc else lib().bte_solve
+    rc = fn(ctx, C.byref(opts), C.byref(r))
     if rc not in (BTE_OK, BTE_ERR_NOCONV):
         raise BTEError(rc, lib().bte_last_error(ctx).decode())
     it = r.iterations

@@ -9,6 +9,20 @@
 
 namespace bte {
 
+// Set (per thread) while an entry point that must not allocate device memory runs: a DBuf that
+// would have to grow then fails with hipErrorOutOfMemory instead of calling hipMalloc, and the
+// flag records that it happened.
+struct AllocGuard {
+    static bool& on() {
+        static thread_local bool f = false;
+        return f;
+    }
+    static bool& tripped() {
+        static thread_local bool t = false;
+        return t;
+    }
+};
+
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -27,6 +41,10 @@ struct DBuf {
     }
     hipError_t ensure(size_t n) {
         if (n <= bytes && p) return hipSuccess;
+        if (AllocGuard::on()) {
+            AllocGuard::tripped() = true;
+            return hipErrorOutOfMemory;
+        }
         release();
         hipError_t e = hipMalloc(&p, std::max<size_t>(n, 16));
         if (e == hipSuccess) bytes = n;

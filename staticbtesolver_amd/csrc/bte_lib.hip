@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -19,7 +20,44 @@
 
 using namespace bte;
 
+// Reduction scratch of bte_solve_noalloc, in this library's code object: one copy of the array per
+// device, 1 MiB.  Host code reaches a device's copy only through the address hipGetSymbolAddress
+// returns for that device (g_scratch), never through the array's name.
+constexpr size_t kScratchDoubles = size_t(1) << 17;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+// Slices of the scratch (offsets in doubles, multiples of 32 = 256 bytes):
+//   margin partial sums of the band-temperature pass (one per work-group, <= 65536 by grid_for),
+//   the margin, the partial sums and total of sq_norm, the Anderson dot-product partials
+//   ((2m+1) x 64, m <= 64) and dots, and the heat flux per (BC, band) before its copy to the host.
+constexpr size_t kScrPartial = 0, kScrMargin = 65536, kScrRedPart = kScrMargin + 32, kScrRedOut = kScrRedPart + 64,
+                 kScrAaPart = kScrRedOut + 32, kScrAaDots = kScrAaPart + 129 * 64, kScrFlux = kScrAaDots + 160,
+                 kScrFluxMax = kScratchDoubles - kScrFlux;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+hipError_t g_scratch_rc[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Address of `device`'s scratch copy, resolved once per device with that device current.
+double* scratch_of(int device, hipError_t* rc) {
+    if (device < 0 || device >= kMaxDevices) {
+        *rc = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* ptr = nullptr;
+        g_scratch_rc[device] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        g_scratch[device] = g_scratch_rc[device] == hipSuccess ? static_cast<double*>(ptr) : nullptr;
+        g_scratch_done[device] = true;
+    }
+    *rc = g_scratch_rc[device];
+    return g_scratch[device];
+}
 
 const char* kKernelNames[] = {"assemble", "sweep_schedule", "diffuse", "solve", "temperature", "margin", "flux"};
 enum { KA_ASM = 0, KA_SCHED, KA_DIFF, KA_SOLVE, KA_TEMP, KA_MARGIN, KA_FLUX, KA_COUNT };
@@ -62,8 +100,6 @@ struct bte_ctx {
     // state + work
     DBuf eeA, eeB, dens, T, Tband, partial, margin, bflux;
     DBuf wb, wr64, wr, wrh, wp, wv, wph, wsh, wt, wy, wf, iters, fails;
-    size_t work_elems = 0;
-    int work_prec = -1;
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
     double* h_margin_pinned = nullptr;
     double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients, then [129] their dot products
@@ -110,6 +146,11 @@ struct bte_ctx {
     int learn_streak = 0;       // consecutive rejections
     int run_rejected = 0;
     int tuned_variant[2][2][2][4] = {};  // sweep variant (direct form) or band group (Krylov forms)
+    // bte_solve_noalloc: this device's scratch copy (null if it could not be resolved) and whether
+    // the reductions use it for the current call
+    double* scratch_dev = nullptr;
+    hipError_t scratch_rc = hipSuccess;
+    bool use_scratch = false;
 };
 
 #define HIPCHK(expr)                                                                              \
@@ -168,6 +209,15 @@ Bnd bnd_of(bte_ctx* c) {
     b.has_diffuse = c->has_diffuse ? 1 : 0;
     return b;
 }
+
+// Reduction buffers: the context's own, or the device's scratch slices during bte_solve_noalloc.
+double* partial_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrPartial : c->partial.as<double>(); }
+double* margin_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrMargin : c->margin.as<double>(); }
+double* red_part_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrRedPart : c->red_part.as<double>(); }
+double* red_out_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrRedOut : c->red_out.as<double>(); }
+double* aa_part_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrAaPart : c->aa_part.as<double>(); }
+double* aa_dots_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrAaDots : c->aa_dots.as<double>(); }
+double* bflux_of(bte_ctx* c) { return c->use_scratch ? c->scratch_dev + kScrFlux : c->bflux.as<double>(); }
 
 // Mirror direction of the specular types, StaticBTESolver.cpp:205-252 (-1 where the reference exits).
 int mirror_of(int dim, int num_phi, int type, int d) {
@@ -334,6 +384,8 @@ int bte_ctx_create(int device, bte_ctx** out) {
         delete ctx;
         return BTE_ERR_HIP;
     }
+    // this device's copy of the scratch array, with the device current (bte_solve_noalloc)
+    ctx->scratch_dev = scratch_of(device, &ctx->scratch_rc);
     *out = ctx;
     return BTE_OK;
 }
@@ -673,44 +725,41 @@ int ensure_fp32(bte_ctx* ctx) {
     return BTE_OK;
 }
 
-// Buffers sized by inputs that the setters change without changing B*D*N: the heat flux per (BC,
-// band) and the diffuse re-emission per (band, boundary face).  A buffer that had to grow moves,
-// so graphs captured with the old address are dropped.
-int ensure_small(bte_ctx* ctx) {
-    const void* before[2] = {ctx->bflux.p, ctx->ediff.p};
-    HIPCHK(ctx->bflux.ensure(sizeof(double) * std::max<size_t>(1, ctx->h_bci.size() * ctx->B)));
-    HIPCHK(ctx->ediff.ensure(sizeof(double) * std::max<size_t>(1, (size_t)ctx->B * ctx->n_bf)));
-    if (before[0] != ctx->bflux.p || before[1] != ctx->ediff.p) drop_graphs(ctx);
-    return BTE_OK;
-}
-
+// Device buffers of an iteration, sized from the current inputs: intensities and Krylov vectors
+// (B*D*N), per-system counters (B*D), per-(band, cell) and per-cell fields, the heat flux per (BC,
+// band), the diffuse re-emission per (band, boundary face) and the margin reduction.  Each is
+// checked every time (a no-op when it is large enough): the setters change B, D, N, the BC count
+// and the boundary faces independently.  A buffer that had to grow moves, so graphs captured with
+// the old address are dropped.  During bte_solve_noalloc the reduction buffers are the device's
+// scratch slices and the context's own are not needed.
 int ensure_work(bte_ctx* ctx, int precision) {
-    const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
+    const size_t SN = (size_t)ctx->B * ctx->D * ctx->N, BN = (size_t)ctx->B * ctx->N;
     const int S = ctx->B * ctx->D;
-    if (int rc = ensure_small(ctx)) return rc;
-    if (ctx->work_elems == SN && ctx->work_prec == precision) return BTE_OK;
     // Krylov vectors are FP64-sized in both precisions: the mixed form re-solves in FP64 a system
     // whose FP32 inner solve stalls
-    const size_t ks = sizeof(double);
-    HIPCHK(ctx->eeA.ensure(sizeof(double) * SN));
-    HIPCHK(ctx->eeB.ensure(sizeof(double) * SN));
-    HIPCHK(ctx->wb.ensure(sizeof(double) * SN));
+    std::vector<std::pair<DBuf*, size_t>> want = {
+        {&ctx->eeA, 8 * SN}, {&ctx->eeB, 8 * SN}, {&ctx->wb, 8 * SN}, {&ctx->wr, 8 * SN}, {&ctx->wrh, 8 * SN},
+        {&ctx->wp, 8 * SN}, {&ctx->wv, 8 * SN}, {&ctx->wph, 8 * SN}, {&ctx->wsh, 8 * SN}, {&ctx->wt, 8 * SN},
+        {&ctx->iters, sizeof(int) * S}, {&ctx->fails, sizeof(int) * S}, {&ctx->dens, 8 * BN},
+        {&ctx->T, 8 * (size_t)ctx->N}, {&ctx->Tband, 8 * BN},
+        {&ctx->ediff, 8 * std::max<size_t>(1, (size_t)ctx->B * ctx->n_bf)}};
     if (precision == BTE_PREC_MIXED) {
-        HIPCHK(ctx->wr64.ensure(sizeof(double) * SN));
-        HIPCHK(ctx->wy.ensure(sizeof(float) * SN));
-        HIPCHK(ctx->wf.ensure(sizeof(float) * SN));
+        want.push_back({&ctx->wr64, 8 * SN});
+        want.push_back({&ctx->wy, 4 * SN});
+        want.push_back({&ctx->wf, 4 * SN});
     }
-    for (DBuf* b : {&ctx->wr, &ctx->wrh, &ctx->wp, &ctx->wv, &ctx->wph, &ctx->wsh, &ctx->wt}) HIPCHK(b->ensure(ks * SN));
-    HIPCHK(ctx->iters.ensure(sizeof(int) * S));
-    HIPCHK(ctx->fails.ensure(sizeof(int) * S));
-    HIPCHK(ctx->dens.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
-    HIPCHK(ctx->T.ensure(sizeof(double) * ctx->N));
-    HIPCHK(ctx->Tband.ensure(sizeof(double) * (size_t)ctx->B * ctx->N));
-    HIPCHK(ctx->partial.ensure(sizeof(double) * grid_for((long long)ctx->B * ctx->N, 256)));
-    HIPCHK(ctx->margin.ensure(sizeof(double) * 2));
-    ctx->work_elems = SN;
-    ctx->work_prec = precision;
-    drop_graphs(ctx);
+    if (!ctx->use_scratch) {
+        want.push_back({&ctx->partial, 8 * (size_t)grid_for((long long)BN, 256)});
+        want.push_back({&ctx->margin, 8 * 2});
+        want.push_back({&ctx->bflux, 8 * std::max<size_t>(1, ctx->h_bci.size() * ctx->B)});
+    }
+    bool moved = false;
+    for (auto& w : want) {
+        const void* before = w.first->p;
+        HIPCHK(w.first->ensure(w.second));
+        moved |= before != w.first->p;
+    }
+    if (moved) drop_graphs(ctx);
     return BTE_OK;
 }
 
@@ -994,12 +1043,12 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
     {
         Timer tm(ctx, KA_TEMP, timing);
         hipLaunchKernelGGL(k_tband<256>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
-                           ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), ctx->partial.as<double>());
+                           ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), partial_of(ctx));
         // cell temperatures and densities, and the margin, in one launch (both read the band step)
         const int n_tcell = grid_for(ctx->N, 256);
         hipLaunchKernelGGL(k_tcell_margin, dim3(n_tcell + 1), dim3(256), 0, ctx->stream, ctx->N, bn,
                            ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>(), n_tcell, nblk,
-                           ctx->partial.as<double>(), 1.0 / ctx->B / ctx->N, ctx->margin.as<double>());
+                           partial_of(ctx), 1.0 / ctx->B / ctx->N, margin_of(ctx));
     }
     HIPCHK(hipGetLastError());
     return BTE_OK;
@@ -1155,8 +1204,10 @@ static int aa_reset(bte_ctx* ctx) {
         HIPCHK(b->ensure(sizeof(double) * Z));
     HIPCHK(ctx->aa_dX.ensure(sizeof(double) * (size_t)m * Z));
     HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * Z));
-    HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
-    HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
+    if (!ctx->use_scratch) {
+        HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
+        HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
+    }
     HIPCHK(ctx->aa_gamma.ensure(sizeof(double) * m));
     HIPCHK(hipMemsetAsync(ctx->aa_dX.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->aa_dF.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
@@ -1241,10 +1292,10 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     const int nq = 2 * m + 1, nblk = 64;
     hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk, m + 1), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
                        skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
-                       ctx->aa_f.as<double>(), ctx->aa_part.as<double>());
+                       ctx->aa_f.as<double>(), aa_part_of(ctx));
     hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
-                       ctx->aa_part.as<double>(), ctx->aa_dots.as<double>());
-    HIPCHK(hipMemcpyAsync(ctx->h_gamma + 64, ctx->aa_dots.p, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
+                       aa_part_of(ctx), aa_dots_of(ctx));
+    HIPCHK(hipMemcpyAsync(ctx->h_gamma + 64, aa_dots_of(ctx), sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
     ctx->aa_slot = slot;
     ctx->aa_skip_now = skip;
     *finish = true;
@@ -1303,13 +1354,14 @@ static int aa_step(bte_ctx* ctx) {
 // sum over [lo, hi) of (a - b)^2 (b may be null), reduced in a fixed order; synchronises the stream
 static int sq_norm(bte_ctx* ctx, const double* a, const double* b, long long lo, long long hi, double& out) {
     const int nblk = 64;
-    HIPCHK(ctx->red_part.ensure(sizeof(double) * nblk));
-    HIPCHK(ctx->red_out.ensure(sizeof(double)));
-    hipLaunchKernelGGL(k_sqdiff<256>, dim3(nblk), dim3(256), 0, ctx->stream, lo, hi, a, b, ctx->red_part.as<double>());
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, ctx->stream, 1, nblk, ctx->red_part.as<double>(),
-                       ctx->red_out.as<double>());
+    if (!ctx->use_scratch) {
+        HIPCHK(ctx->red_part.ensure(sizeof(double) * nblk));
+        HIPCHK(ctx->red_out.ensure(sizeof(double)));
+    }
+    hipLaunchKernelGGL(k_sqdiff<256>, dim3(nblk), dim3(256), 0, ctx->stream, lo, hi, a, b, red_part_of(ctx));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, ctx->stream, 1, nblk, red_part_of(ctx), red_out_of(ctx));
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned + 1, ctx->red_out.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned + 1, red_out_of(ctx), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     out = ctx->h_margin_pinned[1];
     return BTE_OK;
@@ -1569,7 +1621,9 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     const int N = ctx->N, B = ctx->B, D = ctx->D, S = B * D;
     const size_t SN = (size_t)S * N;
     Shape shape;
-    if ((rc = pick_shape(ctx, &o, shape, true))) return rc;
+    // the autotuner creates events and overwrites state; bte_solve_noalloc takes a tuned or the
+    // default shape instead
+    if ((rc = pick_shape(ctx, &o, shape, !ctx->use_scratch))) return rc;
     ctx->block_used = shape.block;
     // the sweep variant that actually runs (a requested form the mesh cannot use falls back to 2)
     ctx->variant_used = !direct_path(ctx, &o)                              ? 0
@@ -1655,7 +1709,7 @@ static int run_iteration(bte_ctx* ctx, double& m) {
         int rc = enqueue_iteration(ctx, &o, sh, buf[par], buf[1 - par], o.timing != 0);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, ctx->margin.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, margin_of(ctx), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     m = *ctx->h_margin_pinned;
     ctx->ee_final = buf[1 - par];
@@ -1823,7 +1877,7 @@ int bte_finish(bte_ctx* ctx, bte_result* res) {
     if (nbc > 0 && it > 0) {
         Timer tm(ctx, KA_FLUX, o.timing != 0);
         hipLaunchKernelGGL(k_flux<256>, dim3(nbc * B), dim3(256), 0, ctx->stream, geo_of(ctx), quad_of(ctx), bnd_of(ctx),
-                           bands_of(ctx), ctx->DM, ctx->ee_final, ctx->bflux.as<double>());
+                           bands_of(ctx), ctx->DM, ctx->ee_final, bflux_of(ctx));
         HIPCHK(hipGetLastError());
     }
     std::vector<int> hit(S), hfail(S);
@@ -1833,7 +1887,7 @@ int bte_finish(bte_ctx* ctx, bte_result* res) {
     HIPCHK(hipMemcpyAsync(hT.data(), ctx->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(hTb.data(), ctx->Tband.p, sizeof(double) * B * N, hipMemcpyDeviceToHost, ctx->stream));
     if (nbc > 0 && it > 0)
-        HIPCHK(hipMemcpyAsync(hbf.data(), ctx->bflux.p, sizeof(double) * nbc * B, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hbf.data(), bflux_of(ctx), sizeof(double) * nbc * B, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     res->inner_iters = 0;
     res->inner_failures = 0;
@@ -1878,6 +1932,51 @@ int bte_solve(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
     if (rc) return rc;
     if ((rc = bte_step(ctx, o.max_iter, nullptr, nullptr))) return rc;
     return bte_finish(ctx, res);
+}
+
+// bte_solve without device allocations: the reductions (margin partial sums and margin, squared
+// norms, Anderson dot products, heat flux staging) use this device's scratch slices, graphs are
+// neither captured nor replayed (instantiation may allocate), the launch shape is a tuned or the
+// default one, and every other device buffer must already be large enough -- from an earlier solve
+// of this form with inputs at least this large.  If one is not, the solve stops with
+// BTE_ERR_STATE before any work depends on it.
+int bte_solve_noalloc(bte_ctx* ctx, const bte_opts* opts, bte_result* res) {
+    if (!ctx || !res) return BTE_ERR_ARG;
+    if (!ctx->scratch_dev)
+        SETERR(BTE_ERR_HIP, std::string("bte_solve_noalloc: the device's scratch array could not be resolved: ") +
+                                hipGetErrorString(ctx->scratch_rc));
+    bte_opts o;
+    if (opts)
+        o = *opts;
+    else
+        bte_default_opts(&o);
+    o.use_graph = 0;
+    o.timing = 0;
+    const size_t flux = ctx->h_bci.size() * (size_t)std::max(ctx->B, 1);
+    if (flux > kScrFluxMax)
+        SETERR(BTE_ERR_ARG, "bte_solve_noalloc: " + std::to_string(flux) + " (boundary condition, band) pairs exceed "
+                                "the scratch's " + std::to_string(kScrFluxMax));
+    struct Mode {
+        bte_ctx* c;
+        explicit Mode(bte_ctx* x) : c(x) {
+            c->use_scratch = true;
+            AllocGuard::on() = true;
+            AllocGuard::tripped() = false;
+        }
+        ~Mode() {
+            c->use_scratch = false;
+            AllocGuard::on() = false;
+        }
+    } mode(ctx);
+    int rc = bte_begin(ctx, &o);
+    if (!rc) rc = bte_step(ctx, o.max_iter, nullptr, nullptr);
+    if (!rc) rc = bte_finish(ctx, res);
+    if (AllocGuard::tripped()) {
+        ctx->run_active = false;
+        SETERR(BTE_ERR_STATE, "bte_solve_noalloc: the context's device buffers are smaller than this solve needs; "
+                              "run bte_solve once with this form and inputs at least this large");
+    }
+    return rc;
 }
 
 int bte_get_intensity(bte_ctx* ctx, double* ee) {
