@@ -361,6 +361,8 @@ def main():
                     help="only time K outer iterations of the form's inner solve and print its roofline")
     ap.add_argument("--block", type=int, default=0, choices=[0, 64, 128, 256, 512, 1024],
                     help="work-group size of the inner solve (0: autotuned)")
+    ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2, 3, 4, 5, 6, 7],
+                    help="sweep variant of the direct form (0: autotuned)")
     ap.add_argument("--band-group", type=int, default=0, choices=[0, 1, 2, 4],
                     help="jacobi: bands per work-group of the batched BiCGSTAB (0: autotuned)")
     ap.add_argument("--tol32", type=float, default=0.0, help="mixed: FP32 inner tolerance (0: library default)")
@@ -382,6 +384,8 @@ def main():
     form = dict(precision=prec, max_iter=100000, precond=capi.PC_JACOBI if args.precond == "jacobi" else capi.PC_SWEEP)
     if args.block:
         form.update(block=args.block)
+    if args.variant:
+        form.update(sweep_variant=args.variant)
     if args.band_group:
         form.update(band_group=args.band_group)
     if args.tol32:

@@ -901,9 +901,11 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
 // band from L2.
 __device__ __forceinline__ void rhs_levels_range(const Ops& o, int S, const double* __restrict__ dens,
                                                  double* __restrict__ rhs_lo, int bx, int nbx, int by, int nby) {
-    const int N = o.N, D = o.D;
-    for (int s = by; s < S; s += nby) {
-        const int b = s / D, d = s % D;
+    const int N = o.N, D = o.D, B = S / D;
+    // row k of work-groups is band k % B of direction k / B: the bands of a direction follow each
+    // other, so the direction's level-order operands come from L2 after the first band
+    for (int k = by; k < S; k += nby) {
+        const int b = k % B, d = k / B, s = b * D + d;
         const size_t od = (size_t)d * N;
         for (int i = bx * blockDim.x + threadIdx.x; i < N; i += nbx * blockDim.x) {
             if (o.swp_bnd[od + i]) continue;  // cells with an incoming boundary face: k_rhs_levels_bnd
@@ -1220,11 +1222,27 @@ __device__ __forceinline__ void sweep_ring_dispatch(const Ops& o, const Work& w,
         sweep_ring_system<BLOCK, W, false, PD>(o, w, s, ring, ring_mask);
 }
 
+// Accelerator complex dies (XCDs) of the MI355X; work-group blk of a launch runs on XCD blk % 8, each
+// XCD with its own L2.
+constexpr int kXcds = 8;
+
+// System swept by work-group blk of a launch of S = B*D work-groups, one per system: XCD x sweeps
+// the directions d = x, x + 8, ... and each direction's B bands in consecutive work-groups of that
+// XCD, so the bands of a direction run at the same time, level by level, and all but the first
+// read the direction's operator records from the XCD's L2.  The identity when D is not a multiple
+// of 8.  A bijection of [0, S).
+__device__ __forceinline__ int system_of_block(int blk, int B, int D) {
+    if (D % kXcds != 0) return blk;
+    const int x = blk % kXcds, r = blk / kXcds;
+    return (r % B) * D + (r / B) * kXcds + x;
+}
+
 // W = incoming slots per cell (F - 1: 1 in 1D, 2 in 2D, 3 in 3D)
 template <int BLOCK, int W, bool FB, int PD = 3>
 __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
-    for (int s = blockIdx.x; s < S; s += gridDim.x) sweep_ring_dispatch<BLOCK, W, FB, PD>(o, w, s, ring, ring_mask);
+    for (int blk = blockIdx.x; blk < S; blk += gridDim.x)
+        sweep_ring_dispatch<BLOCK, W, FB, PD>(o, w, system_of_block(blk, S / o.D, o.D), ring, ring_mask);
 }
 
 // Band-batched pipelined sweep (variants 6 and 7): one work-group sweeps G bands of one direction.

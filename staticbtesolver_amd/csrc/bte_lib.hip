@@ -74,6 +74,12 @@ struct bte_ctx {
     double solid_angle = 4 * kPi, T_ref = 0;
     std::vector<double> h_vol, h_area, h_normal, h_omega, h_dir, h_S, h_v, h_tau, h_C, h_Lr, h_bcT;
     std::vector<int> h_nbr, h_bci, h_bct;
+    // Cells are stored in an internal order chosen for memory locality (locality_order); cell_of[i]
+    // is the caller's index of internal cell i (empty: the caller's order).  Every per-cell array
+    // the library holds, uploads or computes is in internal order; the entry points map the
+    // per-cell arrays they take and return.
+    std::vector<int> cell_of;
+    bool ordered = false;  // locality_order has run for the current geometry
     bool have_geo = false, have_quad = false, have_bands = false, have_bcs = false;
     bool prepared = false;
     // device: geometry / quadrature / bands
@@ -242,6 +248,8 @@ int mirror_of(int dim, int num_phi, int type, int d) {
 
 // Downwind level schedule of one direction from the ELL columns (Kahn's algorithm, level by level).
 // Edges whose source is still unplaced when no cell is ready are lagged (dropped from the sweep).
+// Each level lists its cells in increasing index, so the lanes of a wave that write a level's
+// intensities store to increasing addresses, and cells adjacent in memory share store transactions.
 void level_schedule(int N, int W, const int* col /* [W*N] of this direction */, std::vector<int>& order,
                     std::vector<int>& lptr, std::vector<int>& scol, int& lagged) {
     std::vector<int> indeg(N, 0), outcnt(N + 1, 0);
@@ -282,6 +290,7 @@ void level_schedule(int N, int W, const int* col /* [W*N] of this direction */, 
             }
             frontier.push_back(pick);
         }
+        std::sort(frontier.begin(), frontier.end());
         for (int c : frontier) {
             placed[c] = 1;
             order.push_back(c);
@@ -426,6 +435,8 @@ int bte_set_geometry(bte_ctx* ctx, int dim, int n_cell, int n_face, const double
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_geo = true;
     ctx->prepared = false;
+    ctx->ordered = false;
+    ctx->cell_of.clear();
     ctx->learn_stat_ok = false;
     ctx->geo_tag = ++g_geo_tags;
     drop_graphs(ctx);
@@ -503,11 +514,143 @@ namespace {
 // LDS ring size of the pipelined sweeps (G bands per work-group), 0 when they cannot run
 int ring_size(const bte_ctx* ctx, int* sync_every = nullptr, int G = 1);
 
-// One-time preparation: boundary tables, assembly, sweep schedule, FP32 copies.
+// Kahn level of every cell in the upwind graph of direction v (cell c waits for its neighbour
+// across each face with v.n < 0); a cycle is broken at the lowest unplaced cell.
+std::vector<int> upwind_levels(const bte_ctx* ctx, const double* v) {
+    const int N = ctx->N, F = ctx->F;
+    std::vector<int> indeg(N, 0), lvl(N, 0), start(N + 1, 0);
+    std::vector<char> up((size_t)N * F, 0);
+    for (int c = 0; c < N; c++)
+        for (int f = 0; f < F; f++) {
+            const size_t k = (size_t)c * F + f;
+            const int nb = ctx->h_nbr[k];
+            const double* n = ctx->h_normal.data() + 3 * k;
+            if (nb >= 0 && v[0] * n[0] + v[1] * n[1] + v[2] * n[2] < 0) {
+                up[k] = 1;
+                indeg[c]++;
+                start[nb + 1]++;
+            }
+        }
+    for (int i = 0; i < N; i++) start[i + 1] += start[i];
+    std::vector<int> down(start[N]), fill(start.begin(), start.end() - 1);
+    for (int c = 0; c < N; c++)
+        for (int f = 0; f < F; f++)
+            if (up[(size_t)c * F + f]) down[fill[ctx->h_nbr[(size_t)c * F + f]]++] = c;
+    std::vector<int> queue;
+    queue.reserve(N);
+    std::vector<char> placed(N, 0);
+    for (int c = 0; c < N; c++)
+        if (!indeg[c]) queue.push_back(c);
+    int scan = 0;
+    for (size_t h = 0; (int)queue.size() < N || h < queue.size();) {
+        if (h == queue.size()) {
+            while (placed[scan] || indeg[scan] == 0) scan++;
+            indeg[scan] = 0;
+            queue.push_back(scan);
+        }
+        const int c = queue[h++];
+        placed[c] = 1;
+        for (int e = start[c]; e < start[c + 1]; e++) {
+            const int o = down[e];
+            lvl[o] = std::max(lvl[o], lvl[c] + 1);
+            if (--indeg[o] == 0) queue.push_back(o);
+        }
+    }
+    return lvl;
+}
+
+// Internal cell order for memory locality.  The pipelined sweeps write each system's intensities
+// cell by cell as the levels of its direction arrive, and the levels of different directions cut
+// the mesh along different planes; an order in which cells close in space are close in memory
+// lets the writes of one level share cache lines.  The cells' coordinates are not inputs, so the
+// upwind levels of the quadrature directions closest to +x, +y (and +z) stand in for them (a cell's
+// level grows with its distance from the upwind wall), and the cells are sorted along the Morton
+// curve of those level coordinates.  Stored geometry (host and device) is permuted in place and
+// the neighbour indices renumbered; cell_of keeps the caller's index of each internal cell.
+int locality_order(bte_ctx* ctx) {
+    ctx->ordered = true;
+    ctx->cell_of.clear();
+    const int N = ctx->N, F = ctx->F, dim = ctx->dim;
+    if (dim < 2 || N < 4096) return BTE_OK;
+    std::vector<std::vector<int>> coord;
+    for (int a = 0; a < dim; a++) {
+        int best = 0;
+        for (int d = 1; d < ctx->D; d++)
+            if (ctx->h_dir[3 * d + a] > ctx->h_dir[3 * best + a]) best = d;
+        coord.push_back(upwind_levels(ctx, ctx->h_dir.data() + 3 * best));
+    }
+    auto spread = [](unsigned v, int dim) {  // bit i of v to bit dim*i
+        unsigned long long r = 0;
+        for (int i = 0; i < 21; i++) r |= (unsigned long long)((v >> i) & 1u) << (dim * i);
+        return r;
+    };
+    std::vector<unsigned long long> key(N);
+    for (int c = 0; c < N; c++) {
+        unsigned long long k = 0;
+        for (int a = 0; a < dim; a++) k |= spread((unsigned)coord[a][c], dim) << a;
+        key[c] = k;
+    }
+    std::vector<int> cell_of(N), slot(N);
+    for (int i = 0; i < N; i++) cell_of[i] = i;
+    std::stable_sort(cell_of.begin(), cell_of.end(), [&](int x, int y) { return key[x] < key[y]; });
+    for (int i = 0; i < N; i++) slot[cell_of[i]] = i;
+    std::vector<double> vol(N), area((size_t)N * F), normal((size_t)N * F * 3);
+    std::vector<int> nbr((size_t)N * F);
+    for (int i = 0; i < N; i++) {
+        const int c = cell_of[i];
+        vol[i] = ctx->h_vol[c];
+        for (int f = 0; f < F; f++) {
+            const size_t ki = (size_t)i * F + f, kc = (size_t)c * F + f;
+            area[ki] = ctx->h_area[kc];
+            for (int a = 0; a < 3; a++) normal[3 * ki + a] = ctx->h_normal[3 * kc + a];
+            const int nb = ctx->h_nbr[kc];
+            nbr[ki] = nb >= 0 ? slot[nb] : nb;
+        }
+    }
+    ctx->h_vol.swap(vol);
+    ctx->h_area.swap(area);
+    ctx->h_normal.swap(normal);
+    ctx->h_nbr.swap(nbr);
+    ctx->cell_of.swap(cell_of);
+    HIPCHK(upload(ctx->vol, ctx->h_vol, ctx->stream));
+    HIPCHK(upload(ctx->area, ctx->h_area, ctx->stream));
+    HIPCHK(upload(ctx->normal, ctx->h_normal, ctx->stream));
+    HIPCHK(upload(ctx->nbr, ctx->h_nbr, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->geo_tag = ++g_geo_tags;  // the network's graph of this mesh must be rebuilt
+    ctx->learn_stat_ok = false;
+    return BTE_OK;
+}
+
+// Caller's index of internal cell i.
+inline int ext_cell(const bte_ctx* ctx, int i) { return ctx->cell_of.empty() ? i : ctx->cell_of[i]; }
+
+// Copies between the caller's cell order and the internal one: rows of `width` values per cell,
+// `blocks` consecutive groups of N rows (e.g. [B][D][N] intensities: width 1, blocks B*D).
+template <class T>
+void to_internal(const bte_ctx* ctx, const T* ext, T* in, int width = 1, size_t blocks = 1) {
+    const size_t N = ctx->N;
+    for (size_t g = 0; g < blocks; g++)
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < width; k++)
+                in[(g * N + i) * width + k] = ext[(g * N + ext_cell(ctx, (int)i)) * width + k];
+}
+template <class T>
+void to_external(const bte_ctx* ctx, const T* in, T* ext, int width = 1, size_t blocks = 1) {
+    const size_t N = ctx->N;
+    for (size_t g = 0; g < blocks; g++)
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < width; k++)
+                ext[(g * N + ext_cell(ctx, (int)i)) * width + k] = in[(g * N + i) * width + k];
+}
+
+// One-time preparation: cell order, boundary tables, assembly, sweep schedule, FP32 copies.
 int prepare(bte_ctx* ctx, bool timing) {
     if (!ctx->have_geo || !ctx->have_quad || !ctx->have_bands || !ctx->have_bcs)
         SETERR(BTE_ERR_STATE, "bte_solve: geometry, quadrature, bands and boundary conditions must be set first");
     if (ctx->prepared) return BTE_OK;
+    if (!ctx->ordered)
+        if (int rc = locality_order(ctx)) return rc;
     const int N = ctx->N, F = ctx->F, D = ctx->D;
     // boundary faces + validation of the BC types they reference (StaticBTESolver.cpp:173-257)
     std::vector<int> bfc, bff, bfid((size_t)N * F, -1);
@@ -1820,9 +1963,13 @@ int bte_learn_features(bte_ctx* ctx, float* X, double* x_next, double* span) {
     double ffT = 0, ffZ = 0;
     int rc;
     if ((rc = learn_features(ctx, ctx->learn_X.as<float>(), ffT, ffZ))) return rc;
-    HIPCHK(hipMemcpyAsync(X, ctx->learn_X.p, sizeof(float) * (size_t)N * kGnnIn, hipMemcpyDeviceToHost, ctx->stream));
-    if (x_next) HIPCHK(hipMemcpyAsync(x_next, ctx->aa_x.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<float> hX((size_t)N * kGnnIn);
+    std::vector<double> hx(N);
+    HIPCHK(hipMemcpyAsync(hX.data(), ctx->learn_X.p, sizeof(float) * hX.size(), hipMemcpyDeviceToHost, ctx->stream));
+    if (x_next) HIPCHK(hipMemcpyAsync(hx.data(), ctx->aa_x.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    to_external(ctx, hX.data(), X, kGnnIn);
+    if (x_next) to_external(ctx, hx.data(), x_next);
     if (span) *span = ctx->learn_span;
     return BTE_OK;
 }
@@ -1832,16 +1979,20 @@ int bte_get_temperature(bte_ctx* ctx, double* T, double* T_band) {
     if (!ctx->run_active) SETERR(BTE_ERR_STATE, "no iteration state");
     HIPCHK(hipSetDevice(ctx->device));
     const int N = ctx->N, B = ctx->B;
-    if (T) HIPCHK(hipMemcpyAsync(T, ctx->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<double> hTb;
+    std::vector<double> hT, hTb;
+    if (T) {
+        hT.resize(N);
+        HIPCHK(hipMemcpyAsync(hT.data(), ctx->T.p, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->stream));
+    }
     if (T_band) {
         hTb.resize((size_t)B * N);
         HIPCHK(hipMemcpyAsync(hTb.data(), ctx->Tband.p, sizeof(double) * B * N, hipMemcpyDeviceToHost, ctx->stream));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (T) to_external(ctx, hT.data(), T);
     if (T_band)
-        for (int c = 0; c < N; c++)
-            for (int b = 0; b < B; b++) T_band[(size_t)c * B + b] = hTb[(size_t)b * N + c];
+        for (int i = 0; i < N; i++)
+            for (int b = 0; b < B; b++) T_band[(size_t)ext_cell(ctx, i) * B + b] = hTb[(size_t)b * N + i];
     return BTE_OK;
 }
 
@@ -1849,11 +2000,13 @@ int bte_set_temperature(bte_ctx* ctx, const double* T) {
     if (!ctx || !T) return BTE_ERR_ARG;
     if (!ctx->run_active) SETERR(BTE_ERR_STATE, "no iteration state");
     HIPCHK(hipSetDevice(ctx->device));
-    HIPCHK(hipMemcpyAsync(ctx->T.p, T, sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> hT(ctx->N);
+    to_internal(ctx, T, hT.data());
+    HIPCHK(hipMemcpyAsync(ctx->T.p, hT.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_set_density, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, bands_of(ctx),
                        ctx->T.as<double>(), ctx->dens.as<double>());
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));  // hT is a host temporary
     return BTE_OK;
 }
 
@@ -1895,10 +2048,10 @@ int bte_finish(bte_ctx* ctx, bte_result* res) {
         res->inner_iters += hit[s];
         res->inner_failures += hfail[s];
     }
-    if (res->T) std::copy(hT.begin(), hT.end(), res->T);
+    if (res->T) to_external(ctx, hT.data(), res->T);
     if (res->T_band)
-        for (int c = 0; c < N; c++)
-            for (int b = 0; b < B; b++) res->T_band[(size_t)c * B + b] = hTb[(size_t)b * N + c];
+        for (int i = 0; i < N; i++)
+            for (int b = 0; b < B; b++) res->T_band[(size_t)ext_cell(ctx, i) * B + b] = hTb[(size_t)b * N + i];
     if (res->bc_band_flux) std::copy(hbf.begin(), hbf.end(), res->bc_band_flux);
     if (res->bc_flux)
         for (int k = 0; k < nbc; k++) {
@@ -1983,7 +2136,10 @@ int bte_get_intensity(bte_ctx* ctx, double* ee) {
     if (!ctx || !ee) return BTE_ERR_ARG;
     if (!ctx->ee_final) SETERR(BTE_ERR_STATE, "no solve has run");
     HIPCHK(hipSetDevice(ctx->device));
-    HIPCHK(hipMemcpy(ee, ctx->ee_final, sizeof(double) * (size_t)ctx->B * ctx->D * ctx->N, hipMemcpyDeviceToHost));
+    const size_t S = (size_t)ctx->B * ctx->D;
+    std::vector<double> h(S * ctx->N);
+    HIPCHK(hipMemcpy(h.data(), ctx->ee_final, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    to_external(ctx, h.data(), ee, 1, S);
     return BTE_OK;
 }
 
@@ -2023,14 +2179,15 @@ int bte_debug_assemble(bte_ctx* ctx, int band, int dir, double* diag, double* of
     const double vt = ctx->h_v[band] * ctx->h_tau[band];
     const double w = (ctx->dim > 1) ? ctx->h_omega[dir] : 1.0;
     for (int c = 0; c < N; c++) {
-        diag[c] = vt * dg[c] + w * ctx->h_vol[c];
-        for (int f = 0; f < F; f++) off[(size_t)c * F + f] = 0;
+        const size_t e = ext_cell(ctx, c);  // rows go out in the caller's cell order
+        diag[e] = vt * dg[c] + w * ctx->h_vol[c];
+        for (int f = 0; f < F; f++) off[e * F + f] = 0;
         for (int k = 0; k < W; k++) {
             const int j = col[(size_t)k * N + c];
             if (j < 0) continue;
             for (int f = 0; f < F; f++)
                 if (ctx->h_nbr[(size_t)c * F + f] == j) {
-                    off[(size_t)c * F + f] = vt * val[(size_t)k * N + c];
+                    off[e * F + f] = vt * val[(size_t)k * N + c];
                     break;
                 }
         }
@@ -2040,11 +2197,13 @@ int bte_debug_assemble(bte_ctx* ctx, int band, int dir, double* diag, double* of
 
 static int upload_state(bte_ctx* ctx, const double* ee_prev, const double* density) {
     const int N = ctx->N, B = ctx->B;
-    const size_t SN = (size_t)B * ctx->D * N;
-    HIPCHK(hipMemcpy(ctx->eeB.p, ee_prev, sizeof(double) * SN, hipMemcpyHostToDevice));
+    const size_t S = (size_t)B * ctx->D, SN = S * N;
+    std::vector<double> ee(SN);
+    to_internal(ctx, ee_prev, ee.data(), 1, S);
+    HIPCHK(hipMemcpy(ctx->eeB.p, ee.data(), sizeof(double) * SN, hipMemcpyHostToDevice));
     std::vector<double> dT((size_t)B * N);
-    for (int c = 0; c < N; c++)
-        for (int b = 0; b < B; b++) dT[(size_t)b * N + c] = density[(size_t)c * B + b];
+    for (int i = 0; i < N; i++)
+        for (int b = 0; b < B; b++) dT[(size_t)b * N + i] = density[(size_t)ext_cell(ctx, i) * B + b];
     HIPCHK(hipMemcpy(ctx->dens.p, dT.data(), sizeof(double) * dT.size(), hipMemcpyHostToDevice));
     return BTE_OK;
 }
@@ -2063,8 +2222,10 @@ int bte_debug_rhs(bte_ctx* ctx, const double* ee_prev, const double* density, do
     hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)SN, 256)), dim3(256), 0, ctx->stream, geo_of(ctx), quad_of(ctx),
                        bnd_of(ctx), bands_of(ctx), ctx->eeB.as<double>(), ctx->dens.as<double>(), ctx->wb.as<double>());
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(rhs, ctx->wb.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> h(SN);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->wb.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    to_external(ctx, h.data(), rhs, 1, SN / ctx->N);
     return BTE_OK;
 }
 
@@ -2093,9 +2254,11 @@ int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, c
     if ((rc = pick_shape(ctx, &o, shape, false))) return rc;
     launch_solve(ctx, &o, shape, ctx->eeB.as<double>(), ctx->eeA.as<double>(), ctx->dens.as<double>());
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(x, ctx->eeA.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> h(SN);
+    HIPCHK(hipMemcpyAsync(h.data(), ctx->eeA.p, sizeof(double) * SN, hipMemcpyDeviceToHost, ctx->stream));
     if (iters) HIPCHK(hipMemcpyAsync(iters, ctx->iters.p, sizeof(int) * S, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    to_external(ctx, h.data(), x, 1, S);
     return BTE_OK;
 }
 
