@@ -267,6 +267,7 @@ def roofline_only(case, form, k, prec):
     st = case.stats()
     launches = max(1, st["calls"]["solve"])
     solve_ms = st["ms"]["solve"] / launches
+    rhs_ms = st["ms"]["rhs"] / max(1, st["calls"]["rhs"])
     S = case.N_band * case.N_dir
     per_sys = r["inner_iters"] / launches / S
     w = 8 if prec == capi.PREC_FP64 else 4
@@ -276,6 +277,7 @@ def roofline_only(case, form, k, prec):
     achieved = b_alg / (solve_ms * 1e-3) / 1e9
     return {"roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel_ms": round(solve_ms, 4),
+                         "rhs_ms": round(rhs_ms, 4),
                          "alg_bytes_per_launch": b_alg,
                          "achieved_canonical": round(b_canon / (solve_ms * 1e-3) / 1e9, 2),
                          "inner_iterations_per_system": round(per_sys, 2),
@@ -423,10 +425,13 @@ def main():
     tta = statistics.median(r["tta_s"] for r in runs)
     t_stop = statistics.median(r["stop_s"] for r in runs)
     last = runs[-1]
-    # roofline of the dominant kernel: a separate solve with per-kernel hipEvents on the library stream
+    # roofline of the dominant kernel (the inner solve: one launch sweeps all systems): a separate
+    # solve with hipEvents on the library stream around that kernel alone, and around the
+    # right-hand-side pass before it
     rt = case.solve(timing=1, **form)
     st = case.stats()
     solve_ms = st["ms"]["solve"] / max(1, st["calls"]["solve"])
+    rhs_ms = st["ms"]["rhs"] / max(1, st["calls"]["rhs"])
     # per launch: one exact sweep per system (FP64 direct form) or the refinement loop's products
     per_sys = rt["inner_iters"] / max(1, st["calls"]["solve"]) / (case.N_band * case.N_dir)
     w = 8 if prec == capi.PREC_FP64 else 4
@@ -490,7 +495,9 @@ def main():
         "comparison": cmp,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": prof["bytes"] if prof else None,
-                     "kernel": "inner solve", "kernel_ms": round(solve_ms, 5), "alg_bytes_per_launch": b_alg,
+                     "kernel": {4: "k_sweep_ring", 5: "k_sweep_wave", 6: "k_sweep_ring_bands",
+                                7: "k_sweep_ring_bands"}.get(st["variant"], "inner solve"),
+                     "kernel_ms": round(solve_ms, 5), "rhs_ms": round(rhs_ms, 5), "alg_bytes_per_launch": b_alg,
                      "alg_bytes_formula": "B_shared: each direction's operator once + 2 w n per system",
                      "achieved_canonical": round(b_canon / (solve_ms * 1e-3) / 1e9, 2),
                      "traffic_source": prof},

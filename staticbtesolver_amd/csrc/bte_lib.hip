@@ -59,8 +59,10 @@ double* scratch_of(int device, hipError_t* rc) {
     return g_scratch[device];
 }
 
-const char* kKernelNames[] = {"assemble", "sweep_schedule", "diffuse", "solve", "temperature", "margin", "flux"};
-enum { KA_ASM = 0, KA_SCHED, KA_DIFF, KA_SOLVE, KA_TEMP, KA_MARGIN, KA_FLUX, KA_COUNT };
+// Kinds timed with opts.timing: "rhs" covers the right-hand sides and the diffuse re-emission
+// before the inner solve, "solve" the inner-solve kernel alone (the sweep, or BiCGSTAB).
+const char* kKernelNames[] = {"assemble", "sweep_schedule", "rhs", "solve", "temperature", "margin", "flux"};
+enum { KA_ASM = 0, KA_SCHED, KA_RHS, KA_SOLVE, KA_TEMP, KA_MARGIN, KA_FLUX, KA_COUNT };
 std::atomic<unsigned long long> g_geo_tags{0};
 
 }  // namespace
@@ -1033,9 +1035,15 @@ void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, con
                            ctx->n_bnd_list, ee_prev, dens, out);
 }
 
+// marks (null, or three events): recorded before the right-hand sides, between them and the inner
+// solve, and after it
+inline void mark(hipEvent_t* marks, int k, hipStream_t st) {
+    if (marks) (void)hipEventRecord(marks[k], st);
+}
+
 template <int BLOCK>
 void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const double* ee_prev, double* ee_curr,
-                    const double* dens, bool diffuse_first) {
+                    const double* dens, bool diffuse_first, hipEvent_t* marks) {
     SolveCfg cfg{S, o->inner_maxit, o->inner_tol, std::max(1, o->refine_steps), o->precond == BTE_PC_JACOBI ? 1 : 0,
                  o->inner_tol32 > 0 ? o->inner_tol32 : 1e-5};
     const Geo g = geo_of(ctx);
@@ -1048,6 +1056,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     // diffuse_first: the diffuse re-emission of ee_prev is still due; the pipelined sweep's
     // right-hand-side pass takes it into its launch, every other form runs it first
     const bool v4 = direct_path(ctx, o) && variant == 4 && ring_size(ctx) > 0;
+    mark(marks, 0, ctx->stream);
     if (diffuse_first && !v4)
         hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
                            g, q, bd, ctx->B, ee_prev, ctx->ediff.as<double>());
@@ -1057,6 +1066,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
         if (variant == 5 && ring > 0) {
             ctx->variant_used = 5;
             enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b);
+            mark(marks, 1, ctx->stream);
             const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
             auto go = [&](auto kernel) {
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1078,6 +1088,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
             const int ring_g = ring_size(ctx, &sync_g, G);
             ctx->variant_used = variant;
             enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b);
+            mark(marks, 1, ctx->stream);
             Ops opg = op;
             opg.ring_sync = sync_g;
             const size_t lds = ring_lds_bytes(G * ring_g, ctx->max_levels);
@@ -1104,6 +1115,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
             ctx->variant_used = 4;
             // right-hand sides in level order into w.b, then the pipelined sweep
             enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b, diffuse_first);
+            mark(marks, 1, ctx->stream);
             const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
             auto go = [&](auto kernel) {
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1118,6 +1130,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                 fb ? go(k_sweep_ring<BLOCK, 3, true>) : go(k_sweep_ring<BLOCK, 3, false>);
         } else if (variant == 3 && lds_fits(ctx)) {
             ctx->variant_used = 3;
+            mark(marks, 1, ctx->stream);  // these sweeps form their right-hand sides themselves
             const size_t lds = (size_t)ctx->N * sizeof(double);
             (void)hipFuncSetAttribute((const void*)k_sweep_fast<BLOCK, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds);
@@ -1125,18 +1138,22 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                                dens);
         } else if (variant == 1) {
             ctx->variant_used = 1;
+            mark(marks, 1, ctx->stream);
             hipLaunchKernelGGL((k_sweep_direct<BLOCK>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev, dens);
         } else {
             ctx->variant_used = 2;
+            mark(marks, 1, ctx->stream);
             hipLaunchKernelGGL((k_sweep_fast<BLOCK, false>), grid, blk, 0, ctx->stream, g, q, bd, bn, op, w, S, ee_prev,
                                dens);
         }
+        mark(marks, 2, ctx->stream);
         return;
     }
     // Krylov forms: the right-hand sides of all systems first (keeps the face loop's registers out
     // of the solve kernel, whose occupancy sets its bandwidth)
     hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)S * ctx->N, 256)), dim3(256), 0, ctx->stream, g, q, bd, bn,
                        ee_prev, dens, w.b);
+    mark(marks, 1, ctx->stream);
     const int G = group_ok(ctx, o, variant, BLOCK) ? variant : 1;
     if constexpr (BLOCK <= 512) if (G > 1) {
         // band-batched: one work-group per (direction, group of G bands)
@@ -1145,6 +1162,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
             hipLaunchKernelGGL((k_solve64_bands<BLOCK, 2>), gg, blk, 0, ctx->stream, op, w, cfg);
         else
             hipLaunchKernelGGL((k_solve64_bands<BLOCK, 4>), gg, blk, 0, ctx->stream, op, w, cfg);
+        mark(marks, 2, ctx->stream);
         return;
     }
     if (o->precision == BTE_PREC_MIXED) {
@@ -1158,17 +1176,18 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
         else
             hipLaunchKernelGGL((k_solve64<BLOCK, true>), grid, blk, 0, ctx->stream, op, w, cfg);
     }
+    mark(marks, 2, ctx->stream);
 }
 
 void launch_solve(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_prev, double* ee_curr,
-                  const double* dens, bool diffuse_first = false) {
+                  const double* dens, bool diffuse_first = false, hipEvent_t* marks = nullptr) {
     const int S = ctx->B * ctx->D;
     switch (sh.block) {
-        case 64: launch_solve_b<64>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
-        case 128: launch_solve_b<128>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
-        case 512: launch_solve_b<512>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
-        case 1024: launch_solve_b<1024>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
-        default: launch_solve_b<256>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first); break;
+        case 64: launch_solve_b<64>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first, marks); break;
+        case 128: launch_solve_b<128>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first, marks); break;
+        case 512: launch_solve_b<512>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first, marks); break;
+        case 1024: launch_solve_b<1024>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first, marks); break;
+        default: launch_solve_b<256>(ctx, o, sh.variant, S, ee_prev, ee_curr, dens, diffuse_first, marks); break;
     }
 }
 
@@ -1176,10 +1195,24 @@ void launch_solve(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_pr
 int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* ee_prev, double* ee_curr,
                       bool timing) {
     const Bands bn = bands_of(ctx);
-    {
+    if (timing) {
+        // the right-hand sides (with the diffuse re-emission of ee_prev, which only they read) and
+        // the inner-solve kernel timed apart with events on the library stream
+        hipEvent_t marks[3];
+        for (auto& e : marks) HIPCHK(hipEventCreate(&e));
+        launch_solve(ctx, o, sh, ee_prev, ee_curr, ctx->dens.as<double>(), ctx->has_diffuse, marks);
+        HIPCHK(hipEventSynchronize(marks[2]));
+        float t01 = 0, t12 = 0;
+        HIPCHK(hipEventElapsedTime(&t01, marks[0], marks[1]));
+        HIPCHK(hipEventElapsedTime(&t12, marks[1], marks[2]));
+        for (auto& e : marks) (void)hipEventDestroy(e);
+        ctx->ms[KA_RHS] += t01;
+        ctx->calls[KA_RHS]++;
+        ctx->ms[KA_SOLVE] += t12;
+        ctx->calls[KA_SOLVE]++;
+    } else {
         // the diffuse re-emission of ee_prev, which only the right-hand sides read, goes with the
         // solve launch (beside the pipelined sweep's right-hand-side pass, or just before)
-        Timer tm(ctx, KA_SOLVE, timing);
         launch_solve(ctx, o, sh, ee_prev, ee_curr, ctx->dens.as<double>(), ctx->has_diffuse);
     }
     const int nblk = grid_for((long long)ctx->B * ctx->N, 256);

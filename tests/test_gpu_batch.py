@@ -55,3 +55,30 @@ def test_batch_rejects_krylov_forms():
     c = capi.Case(**case_kwargs("2DM2DG"))
     with pytest.raises(capi.BTEError):
         capi.Batch([c]).solve(precision=capi.PREC_MIXED)
+
+
+def test_batch_with_learned_correction(tmp_path):
+    """Learned corrections (one trained network shared by the problems) proposed, judged and undone
+    per problem inside the batch: every problem's result equals its own single-problem solve."""
+    from staticbtesolver_amd.gnn import Gnn
+    from staticbtesolver_amd.train_gnn import DEFAULT_WEIGHTS
+
+    say("batch: learned")
+    net = Gnn(layers=4, seed=0, device=0)
+    net.set_params(np.load(DEFAULT_WEIGHTS))
+    cases = []
+    for k, (n, nb) in enumerate([(1500, 5), (4500, 10), (2500, 8)]):
+        p = meshgen.make_square_case(str(tmp_path / f"m{k}"), n, 40 + k, nb, 40 + k, L=1e-5)
+        cases.append(capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=1e-5, Ly=1e-5))
+    opts = dict(accel=capi.ACCEL_ANDERSON, learned=1, accel_memory=20, max_iter=20000)
+    single = []
+    for c in cases:
+        c.attach_gnn(net)
+        single.append(c.solve(sweep_variant=2, block=256, **opts))
+    assert sum(s["learned_proposed"] for s in single) > 0
+    out = capi.Batch(cases).solve(**opts)
+    for s, b in zip(single, out):
+        same(s, b)
+        assert s["learned_accepted"] == b["learned_accepted"] and s["learned_proposed"] == b["learned_proposed"]
+    for c in cases:
+        c.attach_gnn(None)
