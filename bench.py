@@ -133,6 +133,10 @@ def bench_uneven(args):
     elif args.form == "plain":
         form.update(accel=capi.ACCEL_NONE)
     batch = capi.Batch(cases)
+    if args.profile_run:
+        out = batch.solve(**form)
+        say(f"profile run: outer iterations {min(r['iters'] for r in out)}..{max(r['iters'] for r in out)}")
+        return
     for i in range(args.warmup):
         t = time.perf_counter()
         out = batch.solve(**form)
@@ -153,6 +157,33 @@ def bench_uneven(args):
         worst = max(worst, float(np.max(np.abs(s["T"] - r["T"]) / np.abs(s["T"]))))
     t_med = statistics.median(times)
     say(f"one at a time: {t_seq:.3f}s; batch {t_med:.3f}s")
+    # roofline of the batched sweep: a timed solve (hipEvents around each launch); member k sweeps
+    # its B_shared bytes (each direction's operator once, right-hand sides in, intensities out) in
+    # each of its outer iterations
+    if net is not None:
+        for c in cases:
+            c.attach_gnn(net)
+    timed = batch.solve(timing=1, **form)
+    st = batch.stats()
+    sweep_ms = st["ms"]["solve"] / max(1, st["calls"]["solve"])
+    member_bytes = [alg_bytes(c, 8, True)[0] for c in cases]
+    total_bytes = sum(b * r["iters"] for b, r in zip(member_bytes, timed))
+    achieved = total_bytes / (st["ms"]["solve"] * 1e-3) / 1e9
+    prof = profiled_traffic("uneven-64", args.form, "fp64")
+    # mixed precision on three members alone (FP32 BiCGSTAB + FP64 refinement, the sweep as the
+    # preconditioner) against the FP64 direct sweep: outer iteration time
+    by_size = sorted(range(len(cases)), key=lambda k: cases[k].N_cell)
+    mixed = []
+    for k in (by_size[0], by_size[len(by_size) // 2], by_size[-1]):
+        c = cases[k]
+        c.attach_gnn(None)
+        f64 = c.solve(max_iter=4, stop_margin=0.0)
+        mx = c.solve(max_iter=4, stop_margin=0.0, precision=capi.PREC_MIXED)
+        mixed.append(dict(n_cell=c.N_cell, fp64_ms_per_iteration=round(1e3 * f64["solve_seconds"] / 4, 3),
+                          mixed_ms_per_iteration=round(1e3 * mx["solve_seconds"] / 4, 3)))
+        say(f"member {c.N_cell} cells: fp64 {mixed[-1]['fp64_ms_per_iteration']} ms, mixed "
+            f"{mixed[-1]['mixed_ms_per_iteration']} ms per outer iteration")
+    cpu = None if args.no_cpu else cpu_baseline_uneven(paths, cases, [r["iters"] for r in timed])
     line = {
         "metric": METRIC, "value": round(t_med, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * t_med, 4), "higher_is_better": False, "scaling": "weak",
@@ -161,10 +192,54 @@ def bench_uneven(args):
                    "form": args.form, "precision": "fp64"},
         "comparison": {"one_at_a_time_s": round(t_seq, 4), "speedup_vs_one_at_a_time": round(t_seq / t_med, 2),
                        "max_rel_diff_T_vs_one_at_a_time": worst,
-                       "outer_iterations": [r["iters"] for r in out]},
-        "roofline": None, "cpu_baseline": None,
+                       "outer_iterations": [r["iters"] for r in out],
+                       "learned_accepted": sum(r["learned_accepted"] for r in out),
+                       "learned_proposed": sum(r["learned_proposed"] for r in out),
+                       "mixed_precision_members": mixed},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": prof["bytes"] if prof else None,
+                     "kernel": "k_sweep_batch", "kernel_ms": round(sweep_ms, 5),
+                     "rhs_ms": round(st["ms"]["rhs"] / max(1, st["calls"]["rhs"]), 5),
+                     "alg_bytes_per_launch": total_bytes / max(1, st["calls"]["solve"]),
+                     "alg_bytes_formula": "sum over the problems still iterating of B_shared",
+                     "traffic_source": prof},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_uneven(paths, cases, iters, budget_s=15.0):
+    """The oracle's PETSc-default restatement on the uneven batch: the first outer iteration's
+    systems of the smallest, the median and the largest problem timed on all usable cores (at most
+    16, directions round-robin), within budget_s, give a CPU time per system and cell; extrapolated
+    to every problem's systems and cells times the outer iterations the batch form took for it (the
+    oracle's iteration is the plain one; the mixing steps are not counted)."""
+    from oracle import oracle as O
+
+    model, n_online, n_usable = cpu_model()
+    cap = max(1, min(16, n_usable or 1))
+    order = sorted(range(len(cases)), key=lambda k: cases[k].N_cell)
+    pick = [order[0], order[len(order) // 2], order[-1]]
+    spent, cell_systems = 0.0, 0
+    W = 1
+    for k in pick:
+        p, c = paths[k], cases[k]
+        o = O.OracleCase(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=p["L"], Ly=p["L"])
+        o.set_inner(O.INNER_PETSC)
+        D = o.N_dir
+        W = max(w for w in range(1, D + 1) if D % w == 0 and w <= cap)
+        o.set_threads(W)
+        spent += o.time_systems(0, D)  # every direction of the first band
+        cell_systems += D * o.N_cell
+        if spent > budget_s:
+            break
+    per_cell_system = spent / cell_systems
+    total = sum(per_cell_system * c.N_band * c.N_dir * c.N_cell * it for c, it in zip(cases, iters))
+    return dict(value=total, unit="s (extrapolated)", cores=W, kind="port",
+                sample=f"first outer iteration, band 0 of {len(pick)} problems ({cell_systems} system-cells) in "
+                       f"{spent:.2f}s on {W} worker threads (GMRES(30)+ILU(0) restatement); x every problem's "
+                       f"systems and cells x its outer iterations in the batch form",
+                cpu_model=model, nproc=n_online, cpus_usable=n_usable)
 
 
 def spmv_bytes(case, w=8):

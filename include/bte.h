@@ -230,6 +230,10 @@ const char* bte_batch_last_error(const bte_batch* b);
 /* results: [n] with caller-set output arrays as for bte_solve (may be NULL); each solve_seconds is the
  * wall time of the whole batch. */
 int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results);
+/* Timing of the last bte_batch_solve run with opts.timing = 1 (else zeros): "rhs" (the diffuse and
+ * right-hand-side launches of all problems) and "solve" (the batched sweep launch), summed hipEvent
+ * milliseconds and launch counts, as bte_get_stats reports them for one context. */
+int bte_batch_get_stats(const bte_batch* b, bte_stats* st);
 
 /* ---- piecewise entry points (used by the parity tests; same kernels as bte_solve) ---- */
 

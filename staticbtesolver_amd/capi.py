@@ -73,7 +73,7 @@ HIP_SYMBOLS = ["bte_default_opts", "bte_ctx_create", "bte_ctx_destroy", "bte_las
                "bte_gnn_last_error", "bte_gnn_num_params", "bte_gnn_get_params", "bte_gnn_set_params",
                "bte_gnn_get_grads", "bte_gnn_set_graph", "bte_gnn_forward", "bte_gnn_backward", "bte_gnn_adam",
                "bte_attach_gnn", "bte_learn_features", "bte_batch_create", "bte_batch_destroy",
-               "bte_batch_last_error", "bte_batch_solve"]
+               "bte_batch_last_error", "bte_batch_solve", "bte_batch_get_stats"]
 CASE_SYMBOLS = ["bte_case_open", "bte_case_close", "bte_case_sizes", "bte_case_params", "bte_case_solve",
                 "bte_case_geometry", "bte_case_quadrature", "bte_case_ctx", "bte_case_error"]
 
@@ -119,6 +119,7 @@ def lib():
     L.bte_batch_last_error.restype = C.c_char_p
     L.bte_batch_last_error.argtypes = [vp]
     L.bte_batch_solve.argtypes = [vp, C.POINTER(Opts), C.POINTER(Result)]
+    L.bte_batch_get_stats.argtypes = [vp, C.POINTER(Stats)]
     L.bte_case_open.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_char_p, C.c_int,
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp), C.c_char_p, C.c_int]
     L.bte_case_close.argtypes = [vp]
@@ -410,3 +411,12 @@ class Batch:
                             inner_iters=int(r.inner_iters), solve_seconds=r.solve_seconds,
                             learned_accepted=r.learned_accepted, learned_proposed=r.learned_proposed))
         return out
+
+    def stats(self):
+        """hipEvent timing of the last solve run with timing=1: ms and calls of "rhs" and "solve"."""
+        st = Stats()
+        rc = lib().bte_batch_get_stats(self.h, C.byref(st))
+        if rc != BTE_OK:
+            raise BTEError(rc, lib().bte_batch_last_error(self.h).decode())
+        names = [st.names[i].decode() for i in range(st.n_kernels)]
+        return dict(ms={n: st.ms[i] for i, n in enumerate(names)}, calls={n: st.calls[i] for i, n in enumerate(names)})

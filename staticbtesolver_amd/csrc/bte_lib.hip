@@ -2309,6 +2309,9 @@ struct bte_batch {
     double* h_m = nullptr;  // pinned [K] margins
     int ring = 0, block = 256, n_items = 0, max_levels = 0;
     double seconds = 0;
+    bool timing = false;               // bte_batch_solve with opts.timing: events around the launches
+    double ms_rhs = 0, ms_sweep = 0;   // summed hipEvent milliseconds of the last solve
+    long long n_rhs = 0, n_sweep = 0;
 };
 
 #define BCHK(expr)                                                                                \
@@ -2348,6 +2351,21 @@ void bte_batch_destroy(bte_batch* b) {
 }
 
 const char* bte_batch_last_error(const bte_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int bte_batch_get_stats(const bte_batch* b, bte_stats* st) {
+    if (!b || !st) return BTE_ERR_ARG;
+    std::memset(st, 0, sizeof(*st));
+    st->n_kernels = KA_COUNT;
+    for (int k = 0; k < KA_COUNT; k++) st->names[k] = kKernelNames[k];
+    st->ms[KA_RHS] = b->ms_rhs;
+    st->calls[KA_RHS] = b->n_rhs;
+    st->ms[KA_SOLVE] = b->ms_sweep;
+    st->calls[KA_SOLVE] = b->n_sweep;
+    st->max_levels = b->max_levels;
+    st->block = b->block;
+    st->variant = b->ring > 0 ? 4 : 2;
+    return BTE_OK;
+}
 
 // Runs the member contexts on the batch stream for the duration of a call.
 struct BatchStreams {
@@ -2469,18 +2487,35 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         if (total > 0)
             hipLaunchKernelGGL(kernel, dim3(total), dim3(block), 0, b->stream, did, doff + step * (P + 1), P);
     };
+    BCHK(b->desc.ensure(sizeof(SweepDesc) * hd.size()));
+    BCHK(b->mout.ensure(sizeof(double) * mp.size()));
+    BCHK(hipMemcpyAsync(b->desc.p, hd.data(), sizeof(SweepDesc) * hd.size(), hipMemcpyHostToDevice, b->stream));
+    hipEvent_t ev[3] = {};
+    if (b->timing)
+        for (auto& e : ev) BCHK(hipEventCreate(&e));
+    if (b->timing) BCHK(hipEventRecord(ev[0], b->stream));
     // boundary data, then the right-hand sides in level order (interior, then boundary cells)
     multi(k_diffuse_multi, ST_DIFF, 256);
     multi(k_rhs_levels_multi, ST_RHS, 256);
     multi(k_rhs_bnd_multi, ST_BND, 64);
-    BCHK(b->desc.ensure(sizeof(SweepDesc) * hd.size()));
-    BCHK(b->mout.ensure(sizeof(double) * mp.size()));
-    BCHK(hipMemcpyAsync(b->desc.p, hd.data(), sizeof(SweepDesc) * hd.size(), hipMemcpyHostToDevice, b->stream));
+    if (b->timing) BCHK(hipEventRecord(ev[1], b->stream));
     switch (b->block) {
         case 64: launch_batch_b<64>(b, b->ring); break;
         case 128: launch_batch_b<128>(b, b->ring); break;
         case 512: launch_batch_b<512>(b, b->ring); break;
         default: launch_batch_b<256>(b, b->ring); break;
+    }
+    if (b->timing) {
+        BCHK(hipEventRecord(ev[2], b->stream));
+        BCHK(hipEventSynchronize(ev[2]));
+        float t01 = 0, t12 = 0;
+        BCHK(hipEventElapsedTime(&t01, ev[0], ev[1]));
+        BCHK(hipEventElapsedTime(&t12, ev[1], ev[2]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        b->ms_rhs += t01;
+        b->ms_sweep += t12;
+        b->n_rhs++;
+        b->n_sweep++;
     }
     // band and cell temperatures, emission densities and margins
     multi(k_tband_multi, ST_TBAND, 256);
@@ -2504,6 +2539,9 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
     else
         bte_default_opts(&o);
     o.use_graph = 0;
+    b->timing = o.timing != 0;
+    b->ms_rhs = b->ms_sweep = 0;
+    b->n_rhs = b->n_sweep = 0;
     o.timing = 0;
     const int K = (int)b->ctx.size();
     BatchStreams streams(b->ctx, b->stream);
