@@ -267,7 +267,7 @@ __device__ __forceinline__ double rhs_at(const Geo& g, const Quad& q, const Bnd&
         }
     }
     const double w = (g.dim != 1) ? q.omega[d] : 1.0;
-    re += dens[(size_t)b * g.N + c] * g.vol[c] * w;
+    re += dens[(size_t)c * bands.B + b] * g.vol[c] * w;
     return re;
 }
 
@@ -317,8 +317,14 @@ __device__ __forceinline__ void block_sum(double* v, double* lds) {
 // Batched inner solve: one work-group per system, the whole BiCGSTAB loop inside the kernel
 // (per-system convergence, no host round trips).  Systems are s = b*D + d.
 // ----------------------------------------------------------------------------------------------
+// Row stride of the level-order right-hand sides rhs_lo (cells, N rounded up to 64): every wave
+// of the pass that writes them stores whole, aligned 512-byte segments (a partial line written to
+// HBM costs a read of it as well).
+__host__ __device__ inline int lo_stride(int N) { return (N + 63) & ~63; }
+
 struct Ops {
     int N, D, W;
+    int Np;               // lo_stride(N)
     const double* vol;
     const double* omega;  // w_d (already 1 in 1D)
     const double* vt;     // [B] v_b * tau_b
@@ -850,7 +856,7 @@ __device__ __forceinline__ void sweep_fast_system(const Geo& g, const Quad& q, c
         const double* __restrict__ sdg = o.swp_dg + (size_t)d * N;
         const double* __restrict__ svw = o.swp_vw + (size_t)d * N;
         const unsigned char* __restrict__ sbnd = o.swp_bnd + (size_t)d * N;
-        const double* __restrict__ densb = dens + (size_t)b * N;
+        const double* __restrict__ densb = dens + b;  // cell-major [N][B]: cell c at densb[c * B]
         const int* lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
         for (int l = 0; l < L; l++) {
@@ -866,7 +872,8 @@ __device__ __forceinline__ void sweep_fast_system(const Geo& g, const Quad& q, c
                     j[k] = (k < W) ? scol[(size_t)k * N + i] : -1;
                     a[k] = (k < W) ? sval[(size_t)k * N + i] : 0.0;
                 }
-                const double rhs = sbnd[i] ? rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens) : densb[c] * vw;
+                const double rhs =
+                    sbnd[i] ? rhs_at(g, q, bd, bands, b, d, c, ee_prev, dens) : densb[(size_t)c * bands.B] * vw;
                 double acc = 0;
 #pragma unroll
                 for (int k = 0; k < 3; k++)
@@ -892,45 +899,51 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
         sweep_fast_system<BLOCK, XLDS>(g, q, bd, bands, o, w, s, xs, ee_prev, dens);
 }
 
-// Right-hand sides of every system in sweep (level) order, rhs_lo[s*N + i] for cell perm_d[i]: the
+// Right-hand sides of every system in sweep (level) order, rhs_lo[s*Np + i] for cell perm_d[i]: the
 // emission term for interior cells, rhs_at for cells with an incoming boundary face -- the values
 // the level sweeps form on the fly, computed up front, fully parallel, so that the pipelined sweep
 // streams them with its other operands instead of gathering by cell inside the level chain.
-// One thread per (system, level position): consecutive threads take consecutive positions of one
-// system, so the stores are coalesced; the level-ordered operands of a direction are re-read per
-// band from L2.
-__device__ __forceinline__ void rhs_levels_range(const Ops& o, int S, const double* __restrict__ dens,
-                                                 double* __restrict__ rhs_lo, int bx, int nbx, int by, int nby) {
-    const int N = o.N, D = o.D, B = S / D;
-    // row k of work-groups is band k % B of direction k / B: the bands of a direction follow each
-    // other, so the direction's level-order operands come from L2 after the first band
-    for (int k = by; k < S; k += nby) {
-        const int b = k % B, d = k / B, s = b * D + d;
-        const size_t od = (size_t)d * N;
-        for (int i = bx * blockDim.x + threadIdx.x; i < N; i += nbx * blockDim.x) {
-            if (o.swp_bnd[od + i]) continue;  // cells with an incoming boundary face: k_rhs_levels_bnd
-            rhs_lo[(size_t)s * N + i] = dens[(size_t)b * N + o.perm[od + i]] * o.swp_vw[od + i];
+// One thread per level position (d, i), for all bands: the cell's densities are one contiguous
+// record (dens is cell-major, [N][B]), so a wave's gather touches each line once for all bands, and
+// for each band consecutive threads store consecutive positions of one system (coalesced).
+__device__ __forceinline__ void rhs_levels_range(const Ops& o, int B, const double* __restrict__ dens,
+                                                 double* __restrict__ rhs_lo, int blk, int nblk) {
+    const int N = o.N, D = o.D, Np = o.Np;
+    const long long total = (long long)D * Np;
+    for (long long t = blk * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)nblk * blockDim.x) {
+        const int d = (int)(t / Np), i = (int)(t % Np);
+        const size_t u = (size_t)d * N + i;  // the direction's level-order operands are unpadded
+        if (i >= N || o.swp_bnd[u]) continue;  // padding; cells with an incoming boundary face: k_rhs_levels_bnd
+        const double* __restrict__ dc = dens + (size_t)o.perm[u] * B;
+        const double vw = o.swp_vw[u];
+        double* __restrict__ out = rhs_lo + t;
+        const size_t stride = (size_t)D * Np;  // system (b, d) -> (b + 1, d)
+        if ((B & 1) == 0) {  // the record is 16-byte aligned: pairs of bands per load
+            for (int b = 0; b < B; b += 2) {
+                const double2 v = *(const double2*)(dc + b);
+                out[(size_t)b * stride] = v.x * vw;
+                out[(size_t)(b + 1) * stride] = v.y * vw;
+            }
+        } else {
+            for (int b = 0; b < B; b++) out[(size_t)b * stride] = dc[b] * vw;
         }
     }
 }
 
-__global__ void k_rhs_levels(Geo g, Quad q, Bnd bd, Bands bands, Ops o, int S, const double* __restrict__ ee_prev,
-                             const double* __restrict__ dens, double* __restrict__ rhs_lo) {
-    rhs_levels_range(o, S, dens, rhs_lo, blockIdx.x, gridDim.x, blockIdx.y, gridDim.y);
+__global__ void k_rhs_levels(Ops o, int B, const double* __restrict__ dens, double* __restrict__ rhs_lo) {
+    rhs_levels_range(o, B, dens, rhs_lo, blockIdx.x, gridDim.x);
 }
 
 // The diffuse re-emission (work-groups [0, n_diff)) and the interior level-order right-hand sides
-// (the rest, one row of positions each: local work-group k is system k / nx, column k % nx) in one
-// launch: the two are independent, so the small diffuse pass runs beside the larger one.
+// (the rest) in one launch: the two are independent, so the small diffuse pass runs beside the
+// larger one.
 __global__ void k_diffuse_rhs_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const double* __restrict__ ee_prev,
                                      double* __restrict__ ediff, const double* __restrict__ dens,
                                      double* __restrict__ rhs_lo, int n_diff) {
-    if ((int)blockIdx.x < n_diff) {
+    if ((int)blockIdx.x < n_diff)
         diffuse_range(g, q, bd, B, ee_prev, ediff, blockIdx.x, n_diff);
-    } else {
-        const int k = blockIdx.x - n_diff, nx = (o.N + blockDim.x - 1) / blockDim.x, S = B * o.D;
-        rhs_levels_range(o, S, dens, rhs_lo, k % nx, nx, k / nx, S);
-    }
+    else
+        rhs_levels_range(o, B, dens, rhs_lo, blockIdx.x - n_diff, gridDim.x - n_diff);
 }
 
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
@@ -946,7 +959,7 @@ __device__ __forceinline__ void rhs_bnd_range(const Geo& g, const Quad& q, const
         const int b = (int)(u / n_list), k = (int)(u % n_list);
         const int t = list[k];
         const int d = t / N, i = t % N;
-        rhs_lo[((size_t)b * D + d) * N + i] = rhs_at(g, q, bd, bands, b, d, o.perm[t], ee_prev, dens);
+        rhs_lo[((size_t)b * D + d) * o.Np + i] = rhs_at(g, q, bd, bands, b, d, o.perm[t], ee_prev, dens);
     }
 }
 
@@ -1014,7 +1027,7 @@ __device__ __forceinline__ PkSrc pk_src(const Ops& o, const Work& w, int s, int 
     r.pkd = buffer_of(o.swp_pkd + d * N * 4, N * 32);
     r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
     r.svw = buffer_of(o.swp_vw + d * N, N * 8);
-    r.rhs = buffer_of(w.b + (size_t)s * N, N * 8);
+    r.rhs = buffer_of(w.b + (size_t)s * o.Np, N * 8);
     r.x = buffer_of(w.x + (size_t)s * N, N * 8);
     r.n4 = (unsigned)(N * 4);
     return r;
@@ -1269,7 +1282,7 @@ __device__ __forceinline__ void sweep_ring_bands(const Ops& o, const Work& w, in
         const int s = (b0 + (g < nb ? g : 0)) * o.D + d;
         vt[g] = o.vt[b0 + (g < nb ? g : 0)];
         xg[g] = to_global(w.x + (size_t)s * N);
-        rhsr[g] = buffer_of(w.b + (size_t)s * N, (size_t)N * 8);
+        rhsr[g] = buffer_of(w.b + (size_t)s * o.Np, (size_t)N * 8);
     }
     const PkSrc src = pk_src<W>(o, w, b0 * o.D + d, d);
     const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
@@ -1634,7 +1647,7 @@ __device__ __forceinline__ void tcell_range(int N, const Bands& bands, const dou
         }
         const double tc = rrn / rnn;
         T[c] = tc;
-        for (int b = 0; b < bands.B; b++) dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
+        for (int b = 0; b < bands.B; b++) dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
     }
 }
 
@@ -1645,8 +1658,9 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 
 // ----------------------------------------------------------------------------------------------
 // Anderson mixing of the outer iteration (fixed-point map T -> G(T) of one source iteration).
-// History columns dX_j = x_{j+1} - x_j and dF_j = f_{j+1} - f_j (f = G(x) - x) live in ring
-// buffers [m][Z]; the m x m least-squares system is solved on the host.
+// History columns dF_j = f_{j+1} - f_j (f = G(x) - x) and dG_j = dX_j + dF_j (dX_j = x_{j+1} - x_j)
+// live in ring buffers [m][Z]; the latest dX alone is kept too (the learned correction's features).
+// The m x m least-squares system is solved on the host.
 // ----------------------------------------------------------------------------------------------
 
 // partial[q * nblk + blk] for q < ncols: dF_q . dF_new, q in [ncols, 2 ncols): dF_{q-ncols} . f,
@@ -1705,7 +1719,7 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
                                     const double* __restrict__ ee, double* __restrict__ z, int first,
                                     const double* __restrict__ x, double* __restrict__ x_prev,
                                     double* __restrict__ f_prev, double* __restrict__ f_cur, double* __restrict__ dX,
-                                    double* __restrict__ dF) {
+                                    double* __restrict__ dF, double* __restrict__ dG) {
     const long long total = N + (long long)S * nb;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
@@ -1721,8 +1735,10 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
         const double xv = x[t];
         const double f = gv - xv;
         if (!first) {
-            dX[t] = xv - x_prev[t];
-            dF[t] = f - f_prev[t];
+            const double dx = xv - x_prev[t], df = f - f_prev[t];
+            dX[t] = dx;
+            dF[t] = df;
+            dG[t] = dx + df;
         }
         x_prev[t] = xv;
         f_prev[t] = f;
@@ -1817,14 +1833,20 @@ __global__ void k_fill(int n, double v, double* __restrict__ x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
 }
 
+// Mixing coefficients of one Anderson step, passed by value (kernel arguments): no upload precedes
+// the update.
+constexpr int kAaMaxCols = 64;
+struct AaCoef {
+    double g[kAaMaxCols];
+};
 
-// Emission density of a given temperature field (StaticBTESolver.cpp:115-119).
-// The mixing update x = g - sum_j gamma_j (dX_j + dF_j) over the Z = N + S nb state entries, and
-// in the same pass the state's loads for the next iteration from the new x: the emission density
-// of its temperature part (as k_set_density) and its boundary intensities (as k_state_scatter).
-__global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, const double* __restrict__ gamma,
-                                  const double* __restrict__ g, const double* __restrict__ dX,
-                                  const double* __restrict__ dF, double* __restrict__ x, Bands bands,
+// The mixing update x = g - sum_j gamma_j (dX_j + dF_j) over the Z = N + S nb state entries (the
+// columns dG_j = dX_j + dF_j are stored as such), and in the same pass the state's loads for the
+// next iteration from the new x: the emission density of its temperature part (as k_set_density)
+// and its boundary intensities (as k_state_scatter).
+__global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, AaCoef gamma,
+                                  const double* __restrict__ g, const double* __restrict__ dG,
+                                  double* __restrict__ x, Bands bands,
                                   double* __restrict__ dens, const int* __restrict__ bcells,
                                   const double* __restrict__ sc, double* __restrict__ ee) {
     const long long Z = N + (long long)S * nb;
@@ -1832,11 +1854,11 @@ __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, const 
         double v = g[c];
         // unrolled so that several columns' loads are in flight; the updates keep their order
 #pragma unroll 8
-        for (int j = 0; j < ncols; j++) v -= gamma[j] * (dX[(size_t)j * Z + c] + dF[(size_t)j * Z + c]);
+        for (int j = 0; j < ncols; j++) v -= gamma.g[j] * dG[(size_t)j * Z + c];
         x[c] = v;
         if (c < N) {
             for (int b = 0; b < bands.B; b++)
-                dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (v - bands.T_ref);
+                dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (v - bands.T_ref);
         } else {
             const long long u = c - N;
             const int s = (int)(u / nb), j = (int)(u % nb);
@@ -1845,10 +1867,11 @@ __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, const 
     }
 }
 
+// Emission density of a given temperature field (StaticBTESolver.cpp:115-119), cell-major [N][B].
 __global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x)
         for (int b = 0; b < bands.B; b++)
-            dens[(size_t)b * N + c] = bands.C[b] / bands.solid_angle * (T[c] - bands.T_ref);
+            dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (T[c] - bands.T_ref);
 }
 
 // Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).
@@ -1928,13 +1951,10 @@ __global__ void k_diffuse_multi(const IterDesc* __restrict__ desc, const int* __
     diffuse_range(D.g, D.q, D.bd, D.bands.B, D.ee_prev, D.ediff, m.blk, m.nblk);
 }
 
-// one row of positions per work-group: local work-group k is system k / nx, column k % nx
 __global__ void k_rhs_levels_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
     const IterDesc& D = desc[m.p];
-    const int nx = (D.o.N + blockDim.x - 1) / blockDim.x;
-    const int S = D.bands.B * D.o.D;
-    rhs_levels_range(D.o, S, D.dens, D.rhs, m.blk % nx, nx, m.blk / nx, S);
+    rhs_levels_range(D.o, D.bands.B, D.dens, D.rhs, m.blk, m.nblk);
 }
 
 __global__ void k_rhs_bnd_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {

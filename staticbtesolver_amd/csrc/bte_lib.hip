@@ -111,6 +111,12 @@ struct bte_ctx {
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
     double* h_margin_pinned = nullptr;
     double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients, then [129] their dot products
+    // Anderson step launched behind the margin of the iteration (one synchronisation for both), and
+    // the host mixing state from before it, restored when the margin ends the solve instead
+    bool aa_spec = false, aa_spec_finish = false;
+    int aa_saved_steps = 0;
+    bool aa_saved_skip = false;
+    std::vector<int> aa_saved_order;
     int aa_slot = 0;            // history column of the step between aa_launch and aa_finish
     bool aa_skip_now = false;   // that step follows an accepted correction
     // graphs, and the options and launch shape they were captured with (graph_valid: they exist)
@@ -134,7 +140,7 @@ struct bte_ctx {
     std::vector<double> run_margins;
     double run_solve_seconds = 0, run_setup_seconds = 0;
     // Anderson mixing state (ring of m difference columns)
-    DBuf aa_x, aa_xprev, aa_fprev, aa_f, aa_dX, aa_dF, aa_part, aa_dots, aa_gamma, aa_g, aa_bcells, aa_sc;
+    DBuf aa_x, aa_xprev, aa_fprev, aa_f, aa_dX, aa_dF, aa_dG, aa_part, aa_dots, aa_g, aa_bcells, aa_sc;
     int aa_m = 0, aa_nb = 0;
     long long aa_Z = 0;  // state length N + S * nb
     bool aa_first = true;
@@ -144,8 +150,8 @@ struct bte_ctx {
     double aa_last_ff = 0;      // squared residual norm of the last evaluated Anderson state
     int aa_steps = 0;
     unsigned long long geo_tag = 0;  // changes with every bte_set_geometry
-    bool aa_plain = false;
-    bool aa_skip = false;       // the next state step follows an accepted correction      // state bookkeeping without mixing (plain iteration + learned correction)
+    bool aa_plain = false;      // state bookkeeping without mixing (plain iteration + learned correction)
+    bool aa_skip = false;       // the next state step follows an accepted correction
     // learned correction
     bte_gnn* gnn = nullptr;     // attached network (not owned)
     DBuf learn_stat, learn_backup, learn_X, red_part, red_out;
@@ -881,9 +887,11 @@ int ensure_work(bte_ctx* ctx, int precision) {
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N, BN = (size_t)ctx->B * ctx->N;
     const int S = ctx->B * ctx->D;
     // Krylov vectors are FP64-sized in both precisions: the mixed form re-solves in FP64 a system
-    // whose FP32 inner solve stalls
+    // whose FP32 inner solve stalls; the right-hand-side buffer also holds the level-order rows of
+    // stride lo_stride(N)
     std::vector<std::pair<DBuf*, size_t>> want = {
-        {&ctx->eeA, 8 * SN}, {&ctx->eeB, 8 * SN}, {&ctx->wb, 8 * SN}, {&ctx->wr, 8 * SN}, {&ctx->wrh, 8 * SN},
+        {&ctx->eeA, 8 * SN}, {&ctx->eeB, 8 * SN}, {&ctx->wb, 8 * (size_t)S * lo_stride(ctx->N)},
+        {&ctx->wr, 8 * SN}, {&ctx->wrh, 8 * SN},
         {&ctx->wp, 8 * SN}, {&ctx->wv, 8 * SN}, {&ctx->wph, 8 * SN}, {&ctx->wsh, 8 * SN}, {&ctx->wt, 8 * SN},
         {&ctx->iters, sizeof(int) * S}, {&ctx->fails, sizeof(int) * S}, {&ctx->dens, 8 * BN},
         {&ctx->T, 8 * (size_t)ctx->N}, {&ctx->Tband, 8 * BN},
@@ -913,6 +921,7 @@ Ops ops_of(bte_ctx* ctx) {
     o.N = ctx->N;
     o.D = ctx->D;
     o.W = ctx->W;
+    o.Np = lo_stride(ctx->N);
     o.vol = ctx->vol.as<double>();
     o.omega = ctx->w_eff.as<double>();
     o.vt = ctx->vt.as<double>();
@@ -1017,16 +1026,14 @@ int ring_size(const bte_ctx* ctx, int* sync_every, int G) {
 void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out,
                         bool with_diffuse = false) {
     const int S = ctx->B * ctx->D;
+    const int n_rhs = grid_for((long long)ctx->D * lo_stride(ctx->N), 256);
     if (with_diffuse) {
         const int n_diff = grid_for((long long)ctx->B * ctx->n_bf, 256);
-        const long long n_rhs = (long long)S * ((ctx->N + 255) / 256);
         hipLaunchKernelGGL(k_diffuse_rhs_levels, dim3((unsigned)(n_diff + n_rhs)), dim3(256), 0, st, geo_of(ctx),
                            quad_of(ctx), bnd_of(ctx), ops_of(ctx), ctx->B, ee_prev, ctx->ediff.as<double>(), dens,
                            out, n_diff);
     } else {
-        const dim3 grid_lv((ctx->N + 255) / 256, std::min(S, 65535));
-        hipLaunchKernelGGL(k_rhs_levels, grid_lv, dim3(256), 0, st, geo_of(ctx), quad_of(ctx), bnd_of(ctx),
-                           bands_of(ctx), ops_of(ctx), S, ee_prev, dens, out);
+        hipLaunchKernelGGL(k_rhs_levels, dim3(n_rhs), dim3(256), 0, st, ops_of(ctx), ctx->B, dens, out);
     }
     if (ctx->n_bnd_list > 0)
         hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(grid_for((long long)ctx->n_bnd_list * ctx->B, 64)), dim3(64), 0, st,
@@ -1376,21 +1383,21 @@ static int aa_reset(bte_ctx* ctx) {
     for (int b = 0; b < ctx->B; b++) sc[b] = ctx->solid_angle / ctx->h_C[b];
     HIPCHK(upload(ctx->aa_bcells, bcells, ctx->stream));
     HIPCHK(upload(ctx->aa_sc, sc, ctx->stream));
-    for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f, &ctx->aa_g})
+    for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f, &ctx->aa_g, &ctx->aa_dX})
         HIPCHK(b->ensure(sizeof(double) * Z));
-    HIPCHK(ctx->aa_dX.ensure(sizeof(double) * (size_t)m * Z));
+    HIPCHK(ctx->aa_dG.ensure(sizeof(double) * (size_t)m * Z));
     HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * Z));
     if (!ctx->use_scratch) {
         HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
         HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
     }
-    HIPCHK(ctx->aa_gamma.ensure(sizeof(double) * m));
-    HIPCHK(hipMemsetAsync(ctx->aa_dX.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->aa_dG.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->aa_dF.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->aa_x.p, 0, sizeof(double) * Z, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));  // sc and bcells are host temporaries
     ctx->aa_first = true;
     ctx->aa_skip = false;
+    ctx->aa_spec = false;
     ctx->aa_steps = 0;
     ctx->aa_last_ff = 0;
     ctx->learn_pending = false;
@@ -1427,17 +1434,18 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     *finish = false;
     ctx->aa_steps++;
     // g = G(x): the new temperature field and reflecting-boundary intensities, gathered in the same
-    // pass as the differences f = g - x (and, for a history step, the columns dX and dF)
-    auto gather_diff = [&](int first, double* dX, double* dF) {
+    // pass as the differences f = g - x (and, for a history step, the columns dF and dG and the
+    // latest dX)
+    auto gather_diff = [&](int first, double* dF, double* dG) {
         hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
                            ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
                            ctx->aa_g.as<double>(), first, ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(),
-                           ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), dX, dF);
+                           ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), dF, dG);
     };
     if (ctx->aa_plain) {
         // plain iteration: x_{k+1} = G(x_k); the iteration already left T, density and the boundary
         // intensities of x_{k+1} in place
-        gather_diff(1, ctx->aa_dX.as<double>(), ctx->aa_dF.as<double>());
+        gather_diff(1, ctx->aa_dF.as<double>(), ctx->aa_dG.as<double>());
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, ctx->aa_g.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
         HIPCHK(hipGetLastError());
         return BTE_OK;
@@ -1456,8 +1464,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
         }
     }
     const double* g = ctx->aa_g.as<double>();
-    gather_diff((ctx->aa_first || skip) ? 1 : 0, ctx->aa_dX.as<double>() + (size_t)slot * Z,
-                ctx->aa_dF.as<double>() + (size_t)slot * Z);
+    gather_diff((ctx->aa_first || skip) ? 1 : 0, ctx->aa_dF.as<double>() + (size_t)slot * Z,
+                ctx->aa_dG.as<double>() + (size_t)slot * Z);
     if (ctx->aa_first) {
         // plain step: x_1 = G(x_0)
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1478,9 +1486,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     return BTE_OK;
 }
 
-// Second half, once the dot products have arrived: the m x m least squares on the host, the mixing
-// coefficients to the device (from pinned memory: the copy runs in stream order, and the buffer is
-// rewritten only after the next step's synchronising read of its dots), the update and the state.
+// Second half, once the dot products have arrived: the m x m least squares on the host, then the
+// update (the coefficients travel as its kernel arguments) and the state.
 static int aa_finish(bte_ctx* ctx) {
     const int m = ctx->aa_m;
     const int Z = (int)ctx->aa_Z;
@@ -1493,8 +1500,8 @@ static int aa_finish(bte_ctx* ctx) {
     ctx->aa_last_ff = ff;
     // divergence guard: restart the history when the residual grew far beyond the best seen
     if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
-    double* gamma = ctx->h_gamma;
-    for (int j = 0; j < m; j++) gamma[j] = 0.0;
+    AaCoef coef{};
+    double* gamma = coef.g;
     if (ff > 1e6 * ctx->aa_best) {
         ctx->aa_order.clear();
     } else {
@@ -1507,11 +1514,10 @@ static int aa_finish(bte_ctx* ctx) {
         small_solve(H, b, n, y);
         for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
     }
-    HIPCHK(hipMemcpyAsync(ctx->aa_gamma.p, gamma, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
     // the update and the state's density and boundary intensities in one pass (load_state's work)
     hipLaunchKernelGGL(k_aa_update_state, dim3(gb), dim3(256), 0, ctx->stream, ctx->N, ctx->D, ctx->B * ctx->D,
-                       ctx->aa_nb, m, ctx->aa_gamma.as<double>(), ctx->aa_g.as<double>(), ctx->aa_dX.as<double>(),
-                       ctx->aa_dF.as<double>(), ctx->aa_x.as<double>(), bands_of(ctx), ctx->dens.as<double>(),
+                       ctx->aa_nb, m, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<double>(), ctx->aa_x.as<double>(),
+                       bands_of(ctx), ctx->dens.as<double>(),
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
     return BTE_OK;
@@ -1680,7 +1686,7 @@ static int learn_features(bte_ctx* ctx, float* X, double& ffT, double& ffZ) {
     const double* dFl = nullptr;
     if (!ctx->aa_plain && !ctx->aa_order.empty()) {
         const int slot = ctx->aa_order.back();
-        dXl = ctx->aa_dX.as<double>() + (size_t)slot * Z;
+        dXl = ctx->aa_dX.as<double>();  // the step of that slot
         dFl = ctx->aa_dF.as<double>() + (size_t)slot * Z;
     }
     const float log_s = s > 0 ? (float)(std::log10(s / ctx->learn_span) / 4) : -2.f;
@@ -1886,9 +1892,21 @@ static int run_iteration(bte_ctx* ctx, double& m) {
         if (rc) return rc;
     }
     HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, margin_of(ctx), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->ee_final = buf[1 - par];
+    // A mixing step reads the new state whatever the margin: its gather and dot products follow the
+    // margin's copy, so that one synchronisation brings both.  Not when a learned correction awaits
+    // its judgement (which may undo the iteration) nor for the first state step.
+    ctx->aa_spec = false;
+    if (o.accel == BTE_ACCEL_ANDERSON && !ctx->aa_plain && !ctx->aa_first && !ctx->learn_pending && ctx->aa_Z > 0) {
+        ctx->aa_saved_steps = ctx->aa_steps;
+        ctx->aa_saved_skip = ctx->aa_skip;
+        ctx->aa_saved_order = ctx->aa_order;
+        int rc = aa_launch(ctx, &ctx->aa_spec_finish);
+        if (rc) return rc;
+        ctx->aa_spec = true;
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     m = *ctx->h_margin_pinned;
-    ctx->ee_final = buf[1 - par];
     ctx->run_iter++;
     return BTE_OK;
 }
@@ -1952,7 +1970,19 @@ static int after_mix(bte_ctx* ctx) {
 static int after_iteration(bte_ctx* ctx, double m, bool& counted) {
     bool mix = false;
     int rc = after_judge(ctx, m, counted, mix);
-    if (rc || !mix) return rc;
+    if (rc) return rc;
+    if (ctx->aa_spec) {  // the step's first half was launched with the iteration
+        ctx->aa_spec = false;
+        if (!mix) {  // the margin ended the solve: the step is not taken
+            ctx->aa_steps = ctx->aa_saved_steps;
+            ctx->aa_skip = ctx->aa_saved_skip;
+            ctx->aa_order = ctx->aa_saved_order;
+            return BTE_OK;
+        }
+        if (ctx->aa_spec_finish && (rc = aa_finish(ctx))) return rc;
+        return after_mix(ctx);
+    }
+    if (!mix) return BTE_OK;
     if ((rc = aa_step(ctx))) return rc;
     return after_mix(ctx);
 }
@@ -2236,7 +2266,7 @@ static int upload_state(bte_ctx* ctx, const double* ee_prev, const double* densi
     HIPCHK(hipMemcpy(ctx->eeB.p, ee.data(), sizeof(double) * SN, hipMemcpyHostToDevice));
     std::vector<double> dT((size_t)B * N);
     for (int i = 0; i < N; i++)
-        for (int b = 0; b < B; b++) dT[(size_t)b * N + i] = density[(size_t)ext_cell(ctx, i) * B + b];
+        for (int b = 0; b < B; b++) dT[(size_t)i * B + b] = density[(size_t)ext_cell(ctx, i) * B + b];
     HIPCHK(hipMemcpy(ctx->dens.p, dT.data(), sizeof(double) * dT.size(), hipMemcpyHostToDevice));
     return BTE_OK;
 }
@@ -2443,7 +2473,7 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         e.scale = 1.0 / ctx->B / ctx->N;
         // work-groups per step, as the single-problem launches partition it
         const int nb[ST_N] = {ctx->has_diffuse ? grid_for((long long)ctx->B * ctx->n_bf, 256) : 0,
-                              b->ring > 0 ? ctx->B * ctx->D * ((ctx->N + 255) / 256) : 0,
+                              b->ring > 0 ? grid_for((long long)ctx->D * lo_stride(ctx->N), 256) : 0,
                               (b->ring > 0 && ctx->n_bnd_list > 0)
                                   ? grid_for((long long)ctx->n_bnd_list * ctx->B, 64) : 0,
                               e.nblk_tband, grid_for(ctx->N, 256)};
