@@ -905,27 +905,41 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
 // streams them with its other operands instead of gathering by cell inside the level chain.
 // One thread per level position (d, i), for all bands: the cell's densities are one contiguous
 // record (dens is cell-major, [N][B]), so a wave's gather touches each line once for all bands, and
-// for each band consecutive threads store consecutive positions of one system (coalesced).
+// for each band consecutive threads store consecutive positions of one system (coalesced).  The
+// record is loaded in chunks of up to 12 bands, each whole before its stores (loads and stores
+// interleaved make every store wait for the next load).  Cells with an incoming boundary face get
+// the interior value too: k_rhs_levels_bnd, which runs after this pass, overwrites it.
 __device__ __forceinline__ void rhs_levels_range(const Ops& o, int B, const double* __restrict__ dens,
                                                  double* __restrict__ rhs_lo, int blk, int nblk) {
+    constexpr int K = 12;
     const int N = o.N, D = o.D, Np = o.Np;
     const long long total = (long long)D * Np;
     for (long long t = blk * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)nblk * blockDim.x) {
         const int d = (int)(t / Np), i = (int)(t % Np);
+        if (i >= N) continue;                // padding of the row
         const size_t u = (size_t)d * N + i;  // the direction's level-order operands are unpadded
-        if (i >= N || o.swp_bnd[u]) continue;  // padding; cells with an incoming boundary face: k_rhs_levels_bnd
         const double* __restrict__ dc = dens + (size_t)o.perm[u] * B;
         const double vw = o.swp_vw[u];
         double* __restrict__ out = rhs_lo + t;
         const size_t stride = (size_t)D * Np;  // system (b, d) -> (b + 1, d)
-        if ((B & 1) == 0) {  // the record is 16-byte aligned: pairs of bands per load
-            for (int b = 0; b < B; b += 2) {
-                const double2 v = *(const double2*)(dc + b);
-                out[(size_t)b * stride] = v.x * vw;
-                out[(size_t)(b + 1) * stride] = v.y * vw;
+        for (int b0 = 0; b0 < B; b0 += K) {
+            double v[K];
+            if ((B & 1) == 0) {  // the record is 16-byte aligned: pairs of bands per load
+#pragma unroll
+                for (int k = 0; k < K / 2; k++)
+                    if (b0 + 2 * k < B) {
+                        const double2 p = *(const double2*)(dc + b0 + 2 * k);
+                        v[2 * k] = p.x;
+                        v[2 * k + 1] = p.y;
+                    }
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    if (b0 + k < B) v[k] = dc[b0 + k];
             }
-        } else {
-            for (int b = 0; b < B; b++) out[(size_t)b * stride] = dc[b] * vw;
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                if (b0 + k < B) out[(size_t)(b0 + k) * stride] = v[k] * vw;
         }
     }
 }
