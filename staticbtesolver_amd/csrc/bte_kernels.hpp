@@ -1710,14 +1710,19 @@ __global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const doubl
     }
 }
 
-// out[q] = sum_b partial[q * nblk + b] in a fixed order
-__global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ partial, double* __restrict__ out) {
+// out[q] = sum_b partial[q * nblk + b] in a fixed order; host (or null): the same values also
+// stored into coherent host memory, which the host reads after synchronising (no copy)
+__global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ partial, double* __restrict__ out,
+                              double* __restrict__ host) {
     __shared__ double lds[4];
     for (int q = blockIdx.x; q < nq; q += gridDim.x) {
         double s[1] = {0};
         for (int i = threadIdx.x; i < nblk; i += 256) s[0] += partial[(size_t)q * nblk + i];
         block_sum<256, 1>(s, lds);
-        if (threadIdx.x == 0) out[q] = s[0];
+        if (threadIdx.x == 0) {
+            out[q] = s[0];
+            if (host) host[q] = s[0];
+        }
     }
 }
 
@@ -1888,14 +1893,18 @@ __global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, 
             dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (T[c] - bands.T_ref);
 }
 
-// Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).
+// Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).  host (or null):
+// the margin also stored into coherent host memory, which the host reads after synchronising.
 __device__ __forceinline__ void margin_block(int nparts, const double* __restrict__ partial, double scale,
-                                             double* __restrict__ out) {
+                                             double* __restrict__ out, double* __restrict__ host = nullptr) {
     __shared__ double lds[4];
     double s[1] = {0};
     for (int i = threadIdx.x; i < nparts; i += 256) s[0] += partial[i];
     block_sum<256, 1>(s, lds);
-    if (threadIdx.x == 0) out[0] = s[0] * scale;
+    if (threadIdx.x == 0) {
+        out[0] = s[0] * scale;
+        if (host) host[0] = s[0] * scale;
+    }
 }
 
 __global__ void k_margin(int nparts, const double* __restrict__ partial, double scale, double* __restrict__ out) {
@@ -1907,11 +1916,11 @@ __global__ void k_margin(int nparts, const double* __restrict__ partial, double 
 __global__ __launch_bounds__(256) void k_tcell_margin(int N, Bands bands, const double* __restrict__ Tband,
                                                       double* __restrict__ T, double* __restrict__ dens, int n_tcell,
                                                       int nparts, const double* __restrict__ partial, double scale,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, double* __restrict__ host) {
     if ((int)blockIdx.x < n_tcell)
         tcell_range(N, bands, Tband, T, dens, blockIdx.x, n_tcell);
     else
-        margin_block(nparts, partial, scale, out);
+        margin_block(nparts, partial, scale, out, host);
 }
 
 // ----------------------------------------------------------------------------------------------

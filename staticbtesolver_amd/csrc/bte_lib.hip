@@ -110,7 +110,11 @@ struct bte_ctx {
     DBuf wb, wr64, wr, wrh, wp, wv, wph, wsh, wt, wy, wf, iters, fails;
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
     double* h_margin_pinned = nullptr;
-    double* h_gamma = nullptr;  // pinned [64] Anderson mixing coefficients, then [129] their dot products
+    // coherent pinned host memory that kernels store into directly (no copy, read after a
+    // synchronisation): [0] the margin of the last outer iteration, [64, 64 + 2 m + 1) the dot
+    // products of the last Anderson step; d_sync is its device address
+    double* h_sync = nullptr;
+    double* d_sync = nullptr;
     // Anderson step launched behind the margin of the iteration (one synchronisation for both), and
     // the host mixing state from before it, restored when the margin ends the solve instead
     bool aa_spec = false, aa_spec_finish = false;
@@ -397,7 +401,9 @@ int bte_ctx_create(int device, bte_ctx** out) {
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void**)&ctx->h_margin_pinned, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&ctx->h_gamma, 256 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&ctx->h_sync, 256 * sizeof(double), hipHostMallocCoherent | hipHostMallocMapped) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&ctx->d_sync, ctx->h_sync, 0) != hipSuccess) {
         delete ctx;
         return BTE_ERR_HIP;
     }
@@ -413,7 +419,7 @@ void bte_ctx_destroy(bte_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     drop_graphs(ctx);
     if (ctx->h_margin_pinned) (void)hipHostFree(ctx->h_margin_pinned);
-    if (ctx->h_gamma) (void)hipHostFree(ctx->h_gamma);
+    if (ctx->h_sync) (void)hipHostFree(ctx->h_sync);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1231,7 +1237,7 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
         const int n_tcell = grid_for(ctx->N, 256);
         hipLaunchKernelGGL(k_tcell_margin, dim3(n_tcell + 1), dim3(256), 0, ctx->stream, ctx->N, bn,
                            ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>(), n_tcell, nblk,
-                           partial_of(ctx), 1.0 / ctx->B / ctx->N, margin_of(ctx));
+                           partial_of(ctx), 1.0 / ctx->B / ctx->N, margin_of(ctx), ctx->d_sync);
     }
     HIPCHK(hipGetLastError());
     return BTE_OK;
@@ -1425,7 +1431,7 @@ static int load_state(bte_ctx* ctx) {
 // One Anderson step after an outer iteration produced g = G(x) in ctx->T: forms the next iterate
 // x in aa_x and the emission density of x for the following iteration.
 // First half of an Anderson step: the state gather, the difference columns and, for a mixing
-// step, the dot products, copied to pinned host memory; nothing waits.  Returns in *finish whether
+// step, the dot products, stored into coherent host memory; nothing waits.  Returns in *finish whether
 // aa_finish must follow (after the stream has been synchronised).
 static int aa_launch(bte_ctx* ctx, bool* finish) {
     const int N = ctx->N, m = ctx->aa_m, S = ctx->B * ctx->D, nb = ctx->aa_nb;
@@ -1478,8 +1484,7 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
                        skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
                        ctx->aa_f.as<double>(), aa_part_of(ctx));
     hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
-                       aa_part_of(ctx), aa_dots_of(ctx));
-    HIPCHK(hipMemcpyAsync(ctx->h_gamma + 64, aa_dots_of(ctx), sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
+                       aa_part_of(ctx), aa_dots_of(ctx), ctx->d_sync + 64);
     ctx->aa_slot = slot;
     ctx->aa_skip_now = skip;
     *finish = true;
@@ -1493,7 +1498,7 @@ static int aa_finish(bte_ctx* ctx) {
     const int Z = (int)ctx->aa_Z;
     const int gb = grid_for(Z, 256);
     const int slot = ctx->aa_slot;
-    const double* dots = ctx->h_gamma + 64;
+    const double* dots = ctx->h_sync + 64;
     if (!ctx->aa_skip_now)
         for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
     const double ff = dots[2 * m];
@@ -1541,7 +1546,8 @@ static int sq_norm(bte_ctx* ctx, const double* a, const double* b, long long lo,
         HIPCHK(ctx->red_out.ensure(sizeof(double)));
     }
     hipLaunchKernelGGL(k_sqdiff<256>, dim3(nblk), dim3(256), 0, ctx->stream, lo, hi, a, b, red_part_of(ctx));
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, ctx->stream, 1, nblk, red_part_of(ctx), red_out_of(ctx));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, ctx->stream, 1, nblk, red_part_of(ctx), red_out_of(ctx),
+                       (double*)nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned + 1, red_out_of(ctx), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1891,7 +1897,6 @@ static int run_iteration(bte_ctx* ctx, double& m) {
         int rc = enqueue_iteration(ctx, &o, sh, buf[par], buf[1 - par], o.timing != 0);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned, margin_of(ctx), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ctx->ee_final = buf[1 - par];
     // A mixing step reads the new state whatever the margin: its gather and dot products follow the
     // margin's copy, so that one synchronisation brings both.  Not when a learned correction awaits
@@ -1906,7 +1911,7 @@ static int run_iteration(bte_ctx* ctx, double& m) {
         ctx->aa_spec = true;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    m = *ctx->h_margin_pinned;
+    m = ctx->h_sync[0];  // stored by the iteration's margin kernel
     ctx->run_iter++;
     return BTE_OK;
 }
