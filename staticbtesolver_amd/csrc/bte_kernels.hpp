@@ -213,20 +213,32 @@ __device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const
         }
         const double* n = g.normal + 3 * ((size_t)c * g.F + f);
         double es = 0, tw = 0;
-        for (int d2 = 0; d2 < q.D; d2++) {
-            const double dn = dot_nc(q.dir + 3 * d2, n);
-            if (dn > 0) {
-                const double e = ee_prev[((size_t)b * q.D + d2) * g.N + c];
-                if (g.dim == 3) {
-                    const double sn = dot_nc(q.S + 3 * d2, n);
-                    es += e * sn * q.omega[d2];
-                    tw += sn * q.omega[d2];
-                } else if (g.dim == 2) {
-                    const double sn = dot_nc(q.S + 3 * d2, n);
-                    es += e * sn;
-                    tw += sn;
-                } else {
-                    es += e * dn * q.omega[d2];
+        // the intensities of 8 directions are loaded together (a load under the outgoing test
+        // would make each wait for the last); the sums take the outgoing terms in direction order
+        constexpr int K = 8;
+        for (int d0 = 0; d0 < q.D; d0 += K) {
+            double ev[K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+                ev[k] = d0 + k < q.D ? ee_prev[((size_t)b * q.D + d0 + k) * g.N + c] : 0.0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int d2 = d0 + k;
+                if (d2 >= q.D) break;
+                const double dn = dot_nc(q.dir + 3 * d2, n);
+                if (dn > 0) {
+                    const double e = ev[k];
+                    if (g.dim == 3) {
+                        const double sn = dot_nc(q.S + 3 * d2, n);
+                        es += e * sn * q.omega[d2];
+                        tw += sn * q.omega[d2];
+                    } else if (g.dim == 2) {
+                        const double sn = dot_nc(q.S + 3 * d2, n);
+                        es += e * sn;
+                        tw += sn;
+                    } else {
+                        es += e * dn * q.omega[d2];
+                    }
                 }
             }
         }
@@ -906,40 +918,48 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_fast(Geo g, Quad q, Bnd bd, Ban
 // One thread per level position (d, i), for all bands: the cell's densities are one contiguous
 // record (dens is cell-major, [N][B]), so a wave's gather touches each line once for all bands, and
 // for each band consecutive threads store consecutive positions of one system (coalesced).  The
-// record is loaded in chunks of up to 12 bands, each whole before its stores (loads and stores
-// interleaved make every store wait for the next load).  Cells with an incoming boundary face get
-// the interior value too: k_rhs_levels_bnd, which runs after this pass, overwrites it.
+// record is read in chunks of up to 12 bands, each chunk's band count a compile-time constant (a
+// uniform switch), so that its loads are all issued before its stores (loads and stores interleaved
+// make every store wait for the next load).  Cells with an incoming boundary face get the interior
+// value too: k_rhs_levels_bnd, which runs after this pass, overwrites it.
+template <int NB>
+__device__ __forceinline__ void rhs_band_chunk(const double* __restrict__ dc, double vw, double* __restrict__ out,
+                                               size_t stride) {
+    double v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) v[k] = dc[k];
+#pragma unroll
+    for (int k = 0; k < NB; k++) out[(size_t)k * stride] = v[k] * vw;
+}
+
 __device__ __forceinline__ void rhs_levels_range(const Ops& o, int B, const double* __restrict__ dens,
                                                  double* __restrict__ rhs_lo, int blk, int nblk) {
-    constexpr int K = 12;
     const int N = o.N, D = o.D, Np = o.Np;
     const long long total = (long long)D * Np;
+    const size_t stride = (size_t)D * Np;  // system (b, d) -> (b + 1, d)
     for (long long t = blk * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)nblk * blockDim.x) {
         const int d = (int)(t / Np), i = (int)(t % Np);
         if (i >= N) continue;                // padding of the row
         const size_t u = (size_t)d * N + i;  // the direction's level-order operands are unpadded
         const double* __restrict__ dc = dens + (size_t)o.perm[u] * B;
         const double vw = o.swp_vw[u];
-        double* __restrict__ out = rhs_lo + t;
-        const size_t stride = (size_t)D * Np;  // system (b, d) -> (b + 1, d)
-        for (int b0 = 0; b0 < B; b0 += K) {
-            double v[K];
-            if ((B & 1) == 0) {  // the record is 16-byte aligned: pairs of bands per load
-#pragma unroll
-                for (int k = 0; k < K / 2; k++)
-                    if (b0 + 2 * k < B) {
-                        const double2 p = *(const double2*)(dc + b0 + 2 * k);
-                        v[2 * k] = p.x;
-                        v[2 * k + 1] = p.y;
-                    }
-            } else {
-#pragma unroll
-                for (int k = 0; k < K; k++)
-                    if (b0 + k < B) v[k] = dc[b0 + k];
+        for (int b0 = 0; b0 < B; b0 += 12) {
+            const double* c0 = dc + b0;
+            double* o0 = rhs_lo + t + (size_t)b0 * stride;
+            switch (min(12, B - b0)) {
+                case 1: rhs_band_chunk<1>(c0, vw, o0, stride); break;
+                case 2: rhs_band_chunk<2>(c0, vw, o0, stride); break;
+                case 3: rhs_band_chunk<3>(c0, vw, o0, stride); break;
+                case 4: rhs_band_chunk<4>(c0, vw, o0, stride); break;
+                case 5: rhs_band_chunk<5>(c0, vw, o0, stride); break;
+                case 6: rhs_band_chunk<6>(c0, vw, o0, stride); break;
+                case 7: rhs_band_chunk<7>(c0, vw, o0, stride); break;
+                case 8: rhs_band_chunk<8>(c0, vw, o0, stride); break;
+                case 9: rhs_band_chunk<9>(c0, vw, o0, stride); break;
+                case 10: rhs_band_chunk<10>(c0, vw, o0, stride); break;
+                case 11: rhs_band_chunk<11>(c0, vw, o0, stride); break;
+                default: rhs_band_chunk<12>(c0, vw, o0, stride); break;
             }
-#pragma unroll
-            for (int k = 0; k < K; k++)
-                if (b0 + k < B) out[(size_t)(b0 + k) * stride] = v[k] * vw;
         }
     }
 }
