@@ -1699,29 +1699,36 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 
 // partial[q * nblk + blk] for q < ncols: dF_q . dF_new, q in [ncols, 2 ncols): dF_{q-ncols} . f,
 // q = 2 ncols: f . f
-// Grid (nblk, ncols + 1): work-group (x, j < ncols) reads column dF_j once for both of its dots,
-// (x, ncols) forms f . f; each sums its chunk in the same fixed order for every call.
-template <int BLOCK>
+// Grid (nblk, ceil(ncols / G) + 1): work-group (x, y < ceil(ncols / G)) reads columns
+// dF_{Gy .. Gy+G-1} and, once per entry for all of them, dF_new and f; (x, last) forms f . f.  Each
+// sum runs over the work-group's chunk in the same fixed order for every call.
+template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const double* __restrict__ dF,
                                                   const double* __restrict__ dF_new, const double* __restrict__ f,
                                                   double* __restrict__ partial) {
-    __shared__ double lds[2 * (BLOCK / 64)];
-    const int j = blockIdx.y, nblk = gridDim.x;
-    double s[2] = {0, 0};
-    if (j < ncols) {
-        const double* a = dF + (size_t)j * N;
-        // unrolled so that several chunks' loads are in flight; the sums keep their order
-#pragma unroll 4
+    __shared__ double lds[2 * G * (BLOCK / 64)];
+    const int j0 = blockIdx.y * G, nblk = gridDim.x;
+    double s[2 * G];
+#pragma unroll
+    for (int k = 0; k < 2 * G; k++) s[k] = 0;
+    if (j0 < ncols) {
         for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
-            const double v = a[c];
-            s[0] += v * dF_new[c];
-            s[1] += v * f[c];
+            const double a = dF_new[c], b = f[c];
+            double v[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) v[g] = j0 + g < ncols ? dF[(size_t)(j0 + g) * N + c] : 0.0;
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                s[g] += v[g] * a;
+                s[G + g] += v[g] * b;
+            }
         }
-        block_sum<BLOCK, 2>(s, lds);
-        if (threadIdx.x == 0) {
-            partial[(size_t)j * nblk + blockIdx.x] = s[0];
-            partial[(size_t)(ncols + j) * nblk + blockIdx.x] = s[1];
-        }
+        block_sum<BLOCK, 2 * G>(s, lds);
+        if (threadIdx.x == 0)
+            for (int g = 0; g < G && j0 + g < ncols; g++) {
+                partial[(size_t)(j0 + g) * nblk + blockIdx.x] = s[g];
+                partial[(size_t)(ncols + j0 + g) * nblk + blockIdx.x] = s[G + g];
+            }
     } else {
 #pragma unroll 4
         for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];

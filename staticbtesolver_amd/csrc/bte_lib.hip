@@ -1480,7 +1480,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     }
     if (!skip) ctx->aa_order.push_back(slot);
     const int nq = 2 * m + 1, nblk = 64;
-    hipLaunchKernelGGL(k_aa_dots<256>, dim3(nblk, m + 1), dim3(256), 0, ctx->stream, Z, m, ctx->aa_dF.as<double>(),
+    hipLaunchKernelGGL((k_aa_dots<256, 4>), dim3(nblk, (m + 3) / 4 + 1), dim3(256), 0, ctx->stream, Z, m,
+                       ctx->aa_dF.as<double>(),
                        skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
                        ctx->aa_f.as<double>(), aa_part_of(ctx));
     hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
