@@ -122,6 +122,7 @@ struct bte_ctx {
     bool aa_saved_skip = false;
     std::vector<int> aa_saved_order;
     int aa_slot = 0;            // history column of the step between aa_launch and aa_finish
+    int aa_ncols = 0;           // filled history columns [0, aa_ncols) of that step
     bool aa_skip_now = false;   // that step follows an accepted correction
     // graphs, and the options and launch shape they were captured with (graph_valid: they exist)
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -160,6 +161,7 @@ struct bte_ctx {
     bte_gnn* gnn = nullptr;     // attached network (not owned)
     DBuf learn_stat, learn_backup, learn_X, red_part, red_out;
     bool learn_stat_ok = false, learn_pending = false;
+    bool aa_cells_ok = false;  // aa_bcells / aa_sc / aa_nb hold this geometry's, BC set's and band set's
     double learn_Tmid = 0, learn_span = 1, learn_ref_ff = 0, learn_ratio = 0;
     int learn_streak = 0;       // consecutive rejections
     int run_rejected = 0;
@@ -452,6 +454,7 @@ int bte_set_geometry(bte_ctx* ctx, int dim, int n_cell, int n_face, const double
     ctx->ordered = false;
     ctx->cell_of.clear();
     ctx->learn_stat_ok = false;
+    ctx->aa_cells_ok = false;
     ctx->geo_tag = ++g_geo_tags;
     drop_graphs(ctx);
     return BTE_OK;
@@ -501,6 +504,7 @@ int bte_set_bands(bte_ctx* ctx, int n_band, const double* v, const double* tau, 
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->have_bands = true;
     ctx->learn_stat_ok = false;
+    ctx->aa_cells_ok = false;
     drop_graphs(ctx);
     return BTE_OK;
 }
@@ -517,6 +521,7 @@ int bte_set_bcs(bte_ctx* ctx, int n_bc, const int* index, const int* type, const
     ctx->have_bcs = true;
     ctx->prepared = false;
     ctx->learn_stat_ok = false;
+    ctx->aa_cells_ok = false;
     drop_graphs(ctx);
     return BTE_OK;
 }
@@ -633,6 +638,7 @@ int locality_order(bte_ctx* ctx) {
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->geo_tag = ++g_geo_tags;  // the network's graph of this mesh must be rebuilt
     ctx->learn_stat_ok = false;
+    ctx->aa_cells_ok = false;
     return BTE_OK;
 }
 
@@ -1369,38 +1375,40 @@ static int aa_reset(bte_ctx* ctx) {
     const int N = ctx->N, F = ctx->F;
     const int m = ctx->aa_plain ? 1 : std::max(1, std::min(ctx->run_opts.accel_memory, 64));
     ctx->aa_m = m;
-    // cells whose previous intensities enter the next right-hand sides: reflecting boundary faces
-    std::vector<int> bcells;
-    for (int c = 0; c < N; c++) {
-        bool refl = false;
-        for (int f = 0; f < F; f++) {
-            const int nb = ctx->h_nbr[(size_t)c * F + f];
-            if (nb >= 0) continue;
-            for (size_t k = 0; k < ctx->h_bci.size(); k++)
-                if (ctx->h_bci[k] == nb && ctx->h_bct[k] != 1) refl = true;
+    if (!ctx->aa_cells_ok) {
+        // cells whose previous intensities enter the next right-hand sides: reflecting boundary faces
+        std::vector<int> bcells;
+        for (int c = 0; c < N; c++) {
+            bool refl = false;
+            for (int f = 0; f < F; f++) {
+                const int nb = ctx->h_nbr[(size_t)c * F + f];
+                if (nb >= 0) continue;
+                for (size_t k = 0; k < ctx->h_bci.size(); k++)
+                    if (ctx->h_bci[k] == nb && ctx->h_bct[k] != 1) refl = true;
+            }
+            if (refl) bcells.push_back(c);
         }
-        if (refl) bcells.push_back(c);
+        ctx->aa_nb = (int)bcells.size();
+        std::vector<double> sc(ctx->B);
+        for (int b = 0; b < ctx->B; b++) sc[b] = ctx->solid_angle / ctx->h_C[b];
+        HIPCHK(upload(ctx->aa_bcells, bcells, ctx->stream));
+        HIPCHK(upload(ctx->aa_sc, sc, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));  // sc and bcells are host temporaries
+        ctx->aa_cells_ok = true;
     }
-    ctx->aa_nb = (int)bcells.size();
     const int S = ctx->B * ctx->D;
     const long long Z = N + (long long)S * ctx->aa_nb;
     ctx->aa_Z = Z;
-    std::vector<double> sc(ctx->B);
-    for (int b = 0; b < ctx->B; b++) sc[b] = ctx->solid_angle / ctx->h_C[b];
-    HIPCHK(upload(ctx->aa_bcells, bcells, ctx->stream));
-    HIPCHK(upload(ctx->aa_sc, sc, ctx->stream));
     for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f, &ctx->aa_g, &ctx->aa_dX})
         HIPCHK(b->ensure(sizeof(double) * Z));
+    // history columns: only the filled slots [0, aa_order.size()) are ever read
     HIPCHK(ctx->aa_dG.ensure(sizeof(double) * (size_t)m * Z));
     HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * Z));
     if (!ctx->use_scratch) {
         HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
         HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
     }
-    HIPCHK(hipMemsetAsync(ctx->aa_dG.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->aa_dF.p, 0, sizeof(double) * (size_t)m * Z, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->aa_x.p, 0, sizeof(double) * Z, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));  // sc and bcells are host temporaries
     ctx->aa_first = true;
     ctx->aa_skip = false;
     ctx->aa_spec = false;
@@ -1479,8 +1487,10 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
         return load_state(ctx);
     }
     if (!skip) ctx->aa_order.push_back(slot);
-    const int nq = 2 * m + 1, nblk = 64;
-    hipLaunchKernelGGL((k_aa_dots<256, 4>), dim3(nblk, (m + 3) / 4 + 1), dim3(256), 0, ctx->stream, Z, m,
+    // the filled slots are [0, n): slots are taken in order and reused only once all m are filled
+    const int n = (int)ctx->aa_order.size();
+    const int nq = 2 * n + 1, nblk = 64;
+    hipLaunchKernelGGL((k_aa_dots<256, 4>), dim3(nblk, (n + 3) / 4 + 1), dim3(256), 0, ctx->stream, Z, n,
                        ctx->aa_dF.as<double>(),
                        skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
                        ctx->aa_f.as<double>(), aa_part_of(ctx));
@@ -1488,6 +1498,7 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
                        aa_part_of(ctx), aa_dots_of(ctx), ctx->d_sync + 64);
     ctx->aa_slot = slot;
     ctx->aa_skip_now = skip;
+    ctx->aa_ncols = n;
     *finish = true;
     return BTE_OK;
 }
@@ -1499,10 +1510,11 @@ static int aa_finish(bte_ctx* ctx) {
     const int Z = (int)ctx->aa_Z;
     const int gb = grid_for(Z, 256);
     const int slot = ctx->aa_slot;
+    const int nc = ctx->aa_ncols;  // the dots cover the filled columns [0, nc)
     const double* dots = ctx->h_sync + 64;
     if (!ctx->aa_skip_now)
-        for (int j = 0; j < m; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
-    const double ff = dots[2 * m];
+        for (int j = 0; j < nc; j++) ctx->aa_H[(size_t)slot * m + j] = ctx->aa_H[(size_t)j * m + slot] = dots[j];
+    const double ff = dots[2 * nc];
     ctx->aa_last_ff = ff;
     // divergence guard: restart the history when the residual grew far beyond the best seen
     if (ctx->aa_best == 0 || ff < ctx->aa_best) ctx->aa_best = ff;
@@ -1514,7 +1526,7 @@ static int aa_finish(bte_ctx* ctx) {
         const int n = (int)ctx->aa_order.size();
         std::vector<double> H((size_t)n * n), b(n), y;
         for (int i = 0; i < n; i++) {
-            b[i] = dots[m + ctx->aa_order[i]];
+            b[i] = dots[nc + ctx->aa_order[i]];
             for (int j = 0; j < n; j++) H[(size_t)i * n + j] = ctx->aa_H[(size_t)ctx->aa_order[i] * m + ctx->aa_order[j]];
         }
         small_solve(H, b, n, y);
@@ -1522,7 +1534,7 @@ static int aa_finish(bte_ctx* ctx) {
     }
     // the update and the state's density and boundary intensities in one pass (load_state's work)
     hipLaunchKernelGGL(k_aa_update_state, dim3(gb), dim3(256), 0, ctx->stream, ctx->N, ctx->D, ctx->B * ctx->D,
-                       ctx->aa_nb, m, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<double>(), ctx->aa_x.as<double>(),
+                       ctx->aa_nb, nc, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<double>(), ctx->aa_x.as<double>(),
                        bands_of(ctx), ctx->dens.as<double>(),
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
@@ -1826,16 +1838,18 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     ctx->group_used = direct_path(ctx, &o) ? 0
                       : group_ok(ctx, &o, shape.variant, shape.block) ? shape.variant
                                                                        : 1;
+    // the zero initial intensities (the first iteration's ee_prev); the buffer it writes is cleared
+    // only for the Krylov forms, whose initial guess it is (a sweep writes every cell)
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
-    HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
+    if (!direct_path(ctx, &o)) HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->T.p, 0, sizeof(double) * N, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->iters.p, 0, sizeof(int) * S, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     double* buf[2] = {ctx->eeA.as<double>(), ctx->eeB.as<double>()};
     const bool use_graph = o.use_graph && !timing;
     if (use_graph && !same_capture(ctx, o, shape)) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));  // capture starts on an idle stream
         drop_graphs(ctx);
         for (int par = 0; par < 2; par++) {
             hipGraph_t g = nullptr;
@@ -1860,7 +1874,6 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
         hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->T_ref,
                            ctx->aa_x.as<double>());
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(ctx->stream));
     }
     if (o.learned) {
         if ((rc = learn_prepare(ctx))) return rc;
