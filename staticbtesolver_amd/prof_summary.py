@@ -48,7 +48,7 @@ def summary(prof_dir, title, command, copy_mib=1024):
              "## Kernel trace (rocprofv3 --kernel-trace --stats)", "",
              "| kernel | calls | total us | avg us | % |", "|---|---|---|---|---|"]
     tk = top_kernels(prof_dir)
-    for n, calls, tot, avg, pct in tk[:10]:
+    for n, calls, tot, avg, pct in tk[:20]:
         lines.append(f"| `{short(n)}` | {calls} | {tot:.1f} | {avg:.2f} | {pct:.1f} |")
     # the kernel reported: the busiest one, or (PROF_KERNEL=<name part>) the instantiation of that
     # kernel launched most often -- the one the autotuner kept, not the candidates it timed

@@ -22,5 +22,5 @@ for C in FETCH_SIZE WRITE_SIZE; do
 done
 echo "== summary"
 python3 "$ROOT/staticbtesolver_amd/prof_summary.py" "$OUT" "${TITLE:-bench.py $ARGS}" \
-  "bash staticbtesolver_amd/profile.sh $ARGS" 1024 "$OUT/summary.json" > "$OUT/summary.md"
+  "${PROF_KERNEL:+PROF_KERNEL=$PROF_KERNEL }bash staticbtesolver_amd/profile.sh $ARGS" 1024 "$OUT/summary.json" > "$OUT/summary.md"
 echo "== done"
