@@ -1693,7 +1693,11 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 // ----------------------------------------------------------------------------------------------
 // Anderson mixing of the outer iteration (fixed-point map T -> G(T) of one source iteration).
 // History columns dF_j = f_{j+1} - f_j (f = G(x) - x) and dG_j = dX_j + dF_j (dX_j = x_{j+1} - x_j)
-// live in ring buffers [m][Z]; the latest dX alone is kept too (the learned correction's features).
+// live in ring buffers [m][Z], formed in FP64 and stored rounded to FP32: the least squares and the
+// update then use columns within 6e-8 (relative) of the exact differences, which moves the mixing
+// step by as little and leaves the fixed point unchanged (at g = x the step is g), and both passes
+// over the history read half the bytes.  Dot products, the update and x, f, g stay FP64.  The latest
+// dX alone is kept too, in FP64 (the learned correction's features).
 // The m x m least-squares system is solved on the host.
 // ----------------------------------------------------------------------------------------------
 
@@ -1701,37 +1705,62 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 // q = 2 ncols: f . f
 // Grid (nblk, ceil(ncols / G) + 1): work-group (x, y < ceil(ncols / G)) reads columns
 // dF_{Gy .. Gy+G-1} and, once per entry for all of them, dF_new and f; (x, last) forms f . f.  Each
-// sum runs over the work-group's chunk in the same fixed order for every call.
-template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const double* __restrict__ dF,
-                                                  const double* __restrict__ dF_new, const double* __restrict__ f,
+// thread takes the entries c, c + stride, ... of its work-group's share, U of them per pass with
+// every load of the pass issued first (at clamped, always valid indices; an entry past the end
+// contributes 0), so a pass costs one memory latency.  Each sum runs over its entries in the same
+// fixed order for every call.
+template <int BLOCK, int G, int U>
+__global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const float* __restrict__ dF,
+                                                  const float* __restrict__ dF_new, const double* __restrict__ f,
                                                   double* __restrict__ partial) {
     __shared__ double lds[2 * G * (BLOCK / 64)];
     const int j0 = blockIdx.y * G, nblk = gridDim.x;
+    const int stride = nblk * BLOCK;
     double s[2 * G];
 #pragma unroll
     for (int k = 0; k < 2 * G; k++) s[k] = 0;
     if (j0 < ncols) {
-        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
-            const double a = dF_new[c], b = f[c];
-            double v[G];
+        const int ng = min(G, ncols - j0);  // columns of this work-group (uniform)
+        for (int c0 = blockIdx.x * BLOCK + threadIdx.x; c0 < N; c0 += U * stride) {
+            double a[U], b[U];
+            float v[U][G];
 #pragma unroll
-            for (int g = 0; g < G; g++) v[g] = j0 + g < ncols ? dF[(size_t)(j0 + g) * N + c] : 0.0;
+            for (int u = 0; u < U; u++) {
+                const int c = min(c0 + u * stride, N - 1);
+                a[u] = dF_new[c];
+                b[u] = f[c];
 #pragma unroll
-            for (int g = 0; g < G; g++) {
-                s[g] += v[g] * a;
-                s[G + g] += v[g] * b;
+                for (int g = 0; g < G; g++) v[u][g] = dF[(size_t)(j0 + min(g, ng - 1)) * N + c];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool in = c0 + u * stride < N;
+                const double au = in ? a[u] : 0.0, bu = in ? b[u] : 0.0;
+#pragma unroll
+                for (int g = 0; g < G; g++)
+                    if (g < ng) {
+                        s[g] += (double)v[u][g] * au;
+                        s[G + g] += (double)v[u][g] * bu;
+                    }
             }
         }
         block_sum<BLOCK, 2 * G>(s, lds);
         if (threadIdx.x == 0)
-            for (int g = 0; g < G && j0 + g < ncols; g++) {
+            for (int g = 0; g < ng; g++) {
                 partial[(size_t)(j0 + g) * nblk + blockIdx.x] = s[g];
                 partial[(size_t)(ncols + j0 + g) * nblk + blockIdx.x] = s[G + g];
             }
     } else {
-#pragma unroll 4
-        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];
+        for (int c0 = blockIdx.x * BLOCK + threadIdx.x; c0 < N; c0 += U * stride) {
+            double b[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) b[u] = f[min(c0 + u * stride, N - 1)];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const double bu = c0 + u * stride < N ? b[u] : 0.0;
+                s[0] += bu * bu;
+            }
+        }
         block_sum<BLOCK, 1>(s, lds);
         if (threadIdx.x == 0) partial[(size_t)(2 * ncols) * nblk + blockIdx.x] = s[0];
     }
@@ -1753,6 +1782,12 @@ __global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ parti
     }
 }
 
+// Stores seq into coherent host memory; launched behind the work a host thread waits for, it runs
+// once all of it has completed, so the host can poll that word instead of synchronising the stream.
+__global__ void k_signal(long long seq, long long* __restrict__ host) {
+    if (threadIdx.x == 0) __hip_atomic_store(host, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // The state of the outer iteration is the temperature field plus the intensities at cells with a
 // reflecting (diffuse or specular) boundary face -- the only parts of the previous iterate the next
 // right-hand sides read.  Intensities are carried as temperature equivalents e * Omega / C_b.
@@ -1765,7 +1800,7 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
                                     const double* __restrict__ ee, double* __restrict__ z, int first,
                                     const double* __restrict__ x, double* __restrict__ x_prev,
                                     double* __restrict__ f_prev, double* __restrict__ f_cur, double* __restrict__ dX,
-                                    double* __restrict__ dF, double* __restrict__ dG) {
+                                    float* __restrict__ dF, float* __restrict__ dG) {
     const long long total = N + (long long)S * nb;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
@@ -1783,8 +1818,8 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
         if (!first) {
             const double dx = xv - x_prev[t], df = f - f_prev[t];
             dX[t] = dx;
-            dF[t] = df;
-            dG[t] = dx + df;
+            dF[t] = (float)df;
+            dG[t] = (float)(dx + df);
         }
         x_prev[t] = xv;
         f_prev[t] = f;
@@ -1830,10 +1865,14 @@ __global__ void k_state_scatter(int N, int D, int S, int nb, const int* __restri
 constexpr int kLearnDynamic = 6;
 constexpr int kLearnStatic = 16 - kLearnDynamic;
 
+// ffT: device address of sum_c f_c^2 over the temperature part (k_sqdiff2), from which s follows.
 __global__ void k_learn_features(int N, const double* __restrict__ f, const double* __restrict__ x_next,
                                  const double* __restrict__ x, const double* __restrict__ dXl,
-                                 const double* __restrict__ dFl, const float* __restrict__ stat, double inv_s,
-                                 double T_mid, double inv_span, float log_s, float* __restrict__ X) {
+                                 const float* __restrict__ dFl, const float* __restrict__ stat,
+                                 const double* __restrict__ ffT, double T_mid, double span, float* __restrict__ X) {
+    const double sv = sqrt(*ffT / N), inv_span = 1.0 / span;
+    const double inv_s = sv > 0 ? 1.0 / sv : 0.0;
+    const float log_s = sv > 0 ? (float)(log10(sv / span) / 4) : -2.f;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
         float* o = X + (size_t)c * 16;
         const float* st = stat + (size_t)c * kLearnStatic;
@@ -1851,8 +1890,11 @@ __global__ void k_learn_features(int N, const double* __restrict__ f, const doub
 
 // z += s * y on the temperature part and, as a local equilibrium shift, on the reflecting-boundary
 // intensities of each corrected cell (they are carried as temperature equivalents).
+// ff: device address of the proposal's residual norms {over T, over the whole state}; a state with
+// no residual (or a non-finite one) is left as it is.
 __global__ void k_apply_correction(int N, int S, int nb, const int* __restrict__ bcells, const float* __restrict__ y,
-                                   double s, double* __restrict__ z) {
+                                   double s, const double* __restrict__ ff, double* __restrict__ z) {
+    if (!(ff[0] > 0) || !isfinite(ff[1])) return;
     const long long total = N + (long long)S * nb;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
@@ -1875,6 +1917,34 @@ __global__ __launch_bounds__(BLOCK) void k_sqdiff(long long lo, long long hi, co
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
+// Two squared norms in one pass: over [0, n1) of (a1 - b1)^2 and over [0, n2) of (a2 - b2)^2 (a null
+// b counts as 0); one partial sum per work-group and norm, partial[q * gridDim.x + blk], each summed
+// in the same fixed order for every call.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sqdiff2(long long n1, const double* __restrict__ a1,
+                                                  const double* __restrict__ b1, long long n2,
+                                                  const double* __restrict__ a2, const double* __restrict__ b2,
+                                                  double* __restrict__ partial) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    double s[2] = {0, 0};
+    const long long n = n1 > n2 ? n1 : n2, st = (long long)gridDim.x * BLOCK;
+    for (long long i = blockIdx.x * (long long)BLOCK + threadIdx.x; i < n; i += st) {
+        if (i < n1) {
+            const double d = a1[i] - (b1 ? b1[i] : 0.0);
+            s[0] += d * d;
+        }
+        if (i < n2) {
+            const double d = a2[i] - (b2 ? b2[i] : 0.0);
+            s[1] += d * d;
+        }
+    }
+    block_sum<BLOCK, 2>(s, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s[0];
+        partial[gridDim.x + blockIdx.x] = s[1];
+    }
+}
+
 __global__ void k_fill(int n, double v, double* __restrict__ x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
 }
@@ -1891,7 +1961,7 @@ struct AaCoef {
 // next iteration from the new x: the emission density of its temperature part (as k_set_density)
 // and its boundary intensities (as k_state_scatter).
 __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, AaCoef gamma,
-                                  const double* __restrict__ g, const double* __restrict__ dG,
+                                  const double* __restrict__ g, const float* __restrict__ dG,
                                   double* __restrict__ x, Bands bands,
                                   double* __restrict__ dens, const int* __restrict__ bcells,
                                   const double* __restrict__ sc, double* __restrict__ ee) {
@@ -1900,7 +1970,7 @@ __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, AaCoef
         double v = g[c];
         // unrolled so that several columns' loads are in flight; the updates keep their order
 #pragma unroll 8
-        for (int j = 0; j < ncols; j++) v -= gamma.g[j] * dG[(size_t)j * Z + c];
+        for (int j = 0; j < ncols; j++) v -= gamma.g[j] * (double)dG[(size_t)j * Z + c];
         x[c] = v;
         if (c < N) {
             for (int b = 0; b < bands.B; b++)

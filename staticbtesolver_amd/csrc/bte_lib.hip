@@ -30,7 +30,7 @@ namespace {
 
 // Slices of the scratch (offsets in doubles, multiples of 32 = 256 bytes):
 //   margin partial sums of the band-temperature pass (one per work-group, <= 65536 by grid_for),
-//   the margin, the partial sums and total of sq_norm, the Anderson dot-product partials
+//   the margin, the partial sums (2 x 32) and totals (4) of launch_norms2, the Anderson dot-product partials
 //   ((2m+1) x 64, m <= 64) and dots, and the heat flux per (BC, band) before its copy to the host.
 constexpr size_t kScrPartial = 0, kScrMargin = 65536, kScrRedPart = kScrMargin + 32, kScrRedOut = kScrRedPart + 64,
                  kScrAaPart = kScrRedOut + 32, kScrAaDots = kScrAaPart + 129 * 64, kScrFlux = kScrAaDots + 160,
@@ -115,6 +115,7 @@ struct bte_ctx {
     // products of the last Anderson step; d_sync is its device address
     double* h_sync = nullptr;
     double* d_sync = nullptr;
+    long long sync_seq = 0;  // last value k_signal stored into h_sync[kSyncFlag] (wait_stream)
     // Anderson step launched behind the margin of the iteration (one synchronisation for both), and
     // the host mixing state from before it, restored when the margin ends the solve instead
     bool aa_spec = false, aa_spec_finish = false;
@@ -162,7 +163,8 @@ struct bte_ctx {
     DBuf learn_stat, learn_backup, learn_X, red_part, red_out;
     bool learn_stat_ok = false, learn_pending = false;
     bool aa_cells_ok = false;  // aa_bcells / aa_sc / aa_nb hold this geometry's, BC set's and band set's
-    double learn_Tmid = 0, learn_span = 1, learn_ref_ff = 0, learn_ratio = 0;
+    double learn_Tmid = 0, learn_span = 1, learn_ref_ff = 0, learn_ratio = 0, learn_jump = 0;
+    bool judge_spec = false;  // the pending correction's judge norms were launched behind its iteration
     int learn_streak = 0;       // consecutive rejections
     int run_rejected = 0;
     int tuned_variant[2][2][2][4] = {};  // sweep variant (direct form) or band group (Krylov forms)
@@ -409,6 +411,7 @@ int bte_ctx_create(int device, bte_ctx** out) {
         delete ctx;
         return BTE_ERR_HIP;
     }
+    std::memset(ctx->h_sync, 0, 256 * sizeof(double));  // no sequence number stored yet
     // this device's copy of the scratch array, with the device current (bte_solve_noalloc)
     ctx->scratch_dev = scratch_of(device, &ctx->scratch_rc);
     *out = ctx;
@@ -1344,7 +1347,7 @@ extern "C" {
 
 // Solves (H + lambda I) y = b for a small symmetric positive semi-definite H by Gaussian
 // elimination with partial pivoting; lambda = 1e-12 * max diag keeps it regular.
-static void small_solve(std::vector<double> H, std::vector<double> b, int n, std::vector<double>& y) {
+static void small_solve_ge(std::vector<double> H, std::vector<double> b, int n, std::vector<double>& y) {
     double dmax = 0;
     for (int i = 0; i < n; i++) dmax = std::max(dmax, H[(size_t)i * n + i]);
     for (int i = 0; i < n; i++) H[(size_t)i * n + i] += 1e-12 * dmax + 1e-300;
@@ -1368,6 +1371,77 @@ static void small_solve(std::vector<double> H, std::vector<double> b, int n, std
         double s = b[i];
         for (int j = i + 1; j < n; j++) s -= H[(size_t)i * n + j] * y[j];
         y[i] = s / H[(size_t)i * n + i];
+    }
+}
+
+// The same system by a Cholesky factorisation H + lambda I = U^T U (a third of the elimination's
+// work, no pivot search), right-looking on the upper triangle so that every inner loop runs along
+// a row (independent updates the compiler vectorises); the upper triangle of H is read.  A pivot
+// that is not positive (H + lambda I numerically indefinite) hands the system to small_solve_ge.
+static void small_solve(const std::vector<double>& H, const std::vector<double>& b, int n, std::vector<double>& y) {
+    constexpr int kMax = 64;
+    if (n > kMax) return small_solve_ge(H, b, n, y);
+    double U[kMax * kMax], z[kMax];
+    double dmax = 0;
+    for (int i = 0; i < n; i++) dmax = std::max(dmax, H[(size_t)i * n + i]);
+    const double lambda = 1e-12 * dmax + 1e-300;
+    for (int i = 0; i < n; i++) {
+        for (int k = i; k < n; k++) U[i * kMax + k] = H[(size_t)i * n + k];
+        U[i * kMax + i] += lambda;
+    }
+    for (int j = 0; j < n; j++) {
+        double* Uj = U + j * kMax;
+        const double d = Uj[j];
+        if (!(d > 0) || !std::isfinite(d)) return small_solve_ge(H, b, n, y);
+        const double r = std::sqrt(d), inv = 1.0 / r;
+        Uj[j] = r;
+        for (int i = j + 1; i < n; i++) Uj[i] *= inv;
+        for (int k = j + 1; k < n; k++) {
+            const double u = Uj[k];
+            double* Uk = U + k * kMax;
+            for (int i = k; i < n; i++) Uk[i] -= u * Uj[i];
+        }
+    }
+    for (int i = 0; i < n; i++) z[i] = b[i];
+    for (int k = 0; k < n; k++) {  // U^T z = b, column by column
+        const double* Uk = U + k * kMax;
+        z[k] /= Uk[k];
+        for (int i = k + 1; i < n; i++) z[i] -= Uk[i] * z[k];
+    }
+    y.assign(n, 0.0);
+    for (int i = n - 1; i >= 0; i--) {  // U y = z
+        const double* Ui = U + i * kMax;
+        double s = z[i];
+        for (int k = i + 1; k < n; k++) s -= Ui[k] * y[k];
+        y[i] = s / Ui[i];
+    }
+}
+
+// Slot of h_sync that k_signal stores its sequence numbers into.
+constexpr int kSyncFlag = 255;
+
+// Waits until the stream has run everything enqueued so far: a one-thread kernel behind that work
+// stores a new sequence number into coherent host memory and the host polls it, which ends the wait
+// sooner than a stream synchronisation.  The stream is queried every few thousand polls: an error
+// ends the wait with that error, and an idle stream whose store is not yet visible falls back to a
+// stream synchronisation, so the wait never outlasts the stream's own work.
+static int wait_stream(bte_ctx* ctx) {
+    const long long seq = ++ctx->sync_seq;
+    hipLaunchKernelGGL(k_signal, dim3(1), dim3(64), 0, ctx->stream, seq, (long long*)(ctx->d_sync + kSyncFlag));
+    HIPCHK(hipGetLastError());
+    const long long* flag = (const long long*)(ctx->h_sync + kSyncFlag);
+    for (unsigned spin = 1;; spin++) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return BTE_OK;
+        if ((spin & 4095u) == 0) {
+            const hipError_t e = hipStreamQuery(ctx->stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) return BTE_OK;
+                HIPCHK(hipStreamSynchronize(ctx->stream));
+                return BTE_OK;
+            }
+            if (e != hipErrorNotReady) HIPCHK(e);
+        }
+        __builtin_ia32_pause();
     }
 }
 
@@ -1402,8 +1476,8 @@ static int aa_reset(bte_ctx* ctx) {
     for (DBuf* b : {&ctx->aa_x, &ctx->aa_xprev, &ctx->aa_fprev, &ctx->aa_f, &ctx->aa_g, &ctx->aa_dX})
         HIPCHK(b->ensure(sizeof(double) * Z));
     // history columns: only the filled slots [0, aa_order.size()) are ever read
-    HIPCHK(ctx->aa_dG.ensure(sizeof(double) * (size_t)m * Z));
-    HIPCHK(ctx->aa_dF.ensure(sizeof(double) * (size_t)m * Z));
+    HIPCHK(ctx->aa_dG.ensure(sizeof(float) * (size_t)m * Z));
+    HIPCHK(ctx->aa_dF.ensure(sizeof(float) * (size_t)m * Z));
     if (!ctx->use_scratch) {
         HIPCHK(ctx->aa_part.ensure(sizeof(double) * (size_t)(2 * m + 1) * 64));
         HIPCHK(ctx->aa_dots.ensure(sizeof(double) * (2 * m + 1)));
@@ -1450,7 +1524,7 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     // g = G(x): the new temperature field and reflecting-boundary intensities, gathered in the same
     // pass as the differences f = g - x (and, for a history step, the columns dF and dG and the
     // latest dX)
-    auto gather_diff = [&](int first, double* dF, double* dG) {
+    auto gather_diff = [&](int first, float* dF, float* dG) {
         hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
                            ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
                            ctx->aa_g.as<double>(), first, ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(),
@@ -1459,7 +1533,7 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     if (ctx->aa_plain) {
         // plain iteration: x_{k+1} = G(x_k); the iteration already left T, density and the boundary
         // intensities of x_{k+1} in place
-        gather_diff(1, ctx->aa_dF.as<double>(), ctx->aa_dG.as<double>());
+        gather_diff(1, ctx->aa_dF.as<float>(), ctx->aa_dG.as<float>());
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, ctx->aa_g.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
         HIPCHK(hipGetLastError());
         return BTE_OK;
@@ -1478,8 +1552,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
         }
     }
     const double* g = ctx->aa_g.as<double>();
-    gather_diff((ctx->aa_first || skip) ? 1 : 0, ctx->aa_dF.as<double>() + (size_t)slot * Z,
-                ctx->aa_dG.as<double>() + (size_t)slot * Z);
+    gather_diff((ctx->aa_first || skip) ? 1 : 0, ctx->aa_dF.as<float>() + (size_t)slot * Z,
+                ctx->aa_dG.as<float>() + (size_t)slot * Z);
     if (ctx->aa_first) {
         // plain step: x_1 = G(x_0)
         HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1490,9 +1564,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     // the filled slots are [0, n): slots are taken in order and reused only once all m are filled
     const int n = (int)ctx->aa_order.size();
     const int nq = 2 * n + 1, nblk = 64;
-    hipLaunchKernelGGL((k_aa_dots<256, 4>), dim3(nblk, (n + 3) / 4 + 1), dim3(256), 0, ctx->stream, Z, n,
-                       ctx->aa_dF.as<double>(),
-                       skip ? ctx->aa_f.as<double>() : ctx->aa_dF.as<double>() + (size_t)slot * Z,
+    hipLaunchKernelGGL((k_aa_dots<1024, 8, 2>), dim3(nblk, (n + 7) / 8 + 1), dim3(1024), 0, ctx->stream, Z, n,
+                       ctx->aa_dF.as<float>(), ctx->aa_dF.as<float>() + (size_t)(skip ? 0 : slot) * Z,
                        ctx->aa_f.as<double>(), aa_part_of(ctx));
     hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
                        aa_part_of(ctx), aa_dots_of(ctx), ctx->d_sync + 64);
@@ -1534,7 +1607,7 @@ static int aa_finish(bte_ctx* ctx) {
     }
     // the update and the state's density and boundary intensities in one pass (load_state's work)
     hipLaunchKernelGGL(k_aa_update_state, dim3(gb), dim3(256), 0, ctx->stream, ctx->N, ctx->D, ctx->B * ctx->D,
-                       ctx->aa_nb, nc, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<double>(), ctx->aa_x.as<double>(),
+                       ctx->aa_nb, nc, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<float>(), ctx->aa_x.as<double>(),
                        bands_of(ctx), ctx->dens.as<double>(),
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
@@ -1551,20 +1624,26 @@ static int aa_step(bte_ctx* ctx) {
 
 // ---- learned correction ------------------------------------------------------------------------
 
-// sum over [lo, hi) of (a - b)^2 (b may be null), reduced in a fixed order; synchronises the stream
-static int sq_norm(bte_ctx* ctx, const double* a, const double* b, long long lo, long long hi, double& out) {
-    const int nblk = 64;
+// Slots of h_sync that the learned correction's norms are stored into: the proposal's residual
+// norms {over T, over the whole state} at kSyncLearn, the judge's {residual of the corrected state,
+// squared size of the jump} at kSyncJudge; their device copies are red_out_of(ctx)[0, 2) and [2, 4).
+constexpr int kSyncLearn = 8, kSyncJudge = 10;
+
+// Two squared norms, sum over [0, n1) of (a1 - b1)^2 and over [0, n2) of (a2 - b2)^2 (a null b counts
+// as 0), reduced in a fixed order into red_out_of(ctx)[at, at + 2) and h_sync[kSyncLearn + at, + 2);
+// nothing waits.
+static int launch_norms2(bte_ctx* ctx, int at, long long n1, const double* a1, const double* b1, long long n2,
+                         const double* a2, const double* b2) {
+    const int nblk = 32;  // 2 x 32 partial sums: the reduction slice of the scratch
     if (!ctx->use_scratch) {
-        HIPCHK(ctx->red_part.ensure(sizeof(double) * nblk));
-        HIPCHK(ctx->red_out.ensure(sizeof(double)));
+        HIPCHK(ctx->red_part.ensure(sizeof(double) * 2 * nblk));
+        HIPCHK(ctx->red_out.ensure(sizeof(double) * 4));
     }
-    hipLaunchKernelGGL(k_sqdiff<256>, dim3(nblk), dim3(256), 0, ctx->stream, lo, hi, a, b, red_part_of(ctx));
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(256), 0, ctx->stream, 1, nblk, red_part_of(ctx), red_out_of(ctx),
-                       (double*)nullptr);
+    hipLaunchKernelGGL(k_sqdiff2<1024>, dim3(nblk), dim3(1024), 0, ctx->stream, n1, a1, b1, n2, a2, b2,
+                       red_part_of(ctx));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2), dim3(256), 0, ctx->stream, 2, nblk, red_part_of(ctx),
+                       red_out_of(ctx) + at, ctx->d_sync + kSyncLearn + at);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(ctx->h_margin_pinned + 1, red_out_of(ctx), sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    out = ctx->h_margin_pinned[1];
     return BTE_OK;
 }
 
@@ -1690,28 +1769,26 @@ static int learn_prepare(bte_ctx* ctx) {
     return BTE_OK;
 }
 
-// Features of the current state into X (device, [N][16]); returns the residual norms of the last
-// evaluated state: ffT over the temperature part, ffZ over the whole state.
-static int learn_features(bte_ctx* ctx, float* X, double& ffT, double& ffZ) {
+// Features of the current state into X (device, [N][16]), with the residual norms of the last
+// evaluated state (over the temperature part and over the whole state) formed first on the device:
+// the features scale by the first, and learn_judge reads both.  Nothing waits.
+static int learn_features(bte_ctx* ctx, float* X) {
     int rc;
     if ((rc = learn_prepare(ctx))) return rc;
     const int N = ctx->N;
     const long long Z = ctx->aa_Z;
-    if ((rc = sq_norm(ctx, ctx->aa_f.as<double>(), nullptr, 0, N, ffT))) return rc;
-    if ((rc = sq_norm(ctx, ctx->aa_f.as<double>(), nullptr, 0, Z, ffZ))) return rc;
-    const double s = std::sqrt(ffT / N);
-    const double inv_s = s > 0 ? 1.0 / s : 0.0;
+    if ((rc = launch_norms2(ctx, 0, N, ctx->aa_f.as<double>(), nullptr, Z, ctx->aa_f.as<double>(), nullptr)))
+        return rc;
     const double* dXl = nullptr;
-    const double* dFl = nullptr;
+    const float* dFl = nullptr;
     if (!ctx->aa_plain && !ctx->aa_order.empty()) {
         const int slot = ctx->aa_order.back();
         dXl = ctx->aa_dX.as<double>();  // the step of that slot
-        dFl = ctx->aa_dF.as<double>() + (size_t)slot * Z;
+        dFl = ctx->aa_dF.as<float>() + (size_t)slot * Z;
     }
-    const float log_s = s > 0 ? (float)(std::log10(s / ctx->learn_span) / 4) : -2.f;
     hipLaunchKernelGGL(k_learn_features, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->aa_f.as<double>(),
-                       ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), dXl, dFl, ctx->learn_stat.as<float>(), inv_s,
-                       ctx->learn_Tmid, 1.0 / ctx->learn_span, log_s, X);
+                       ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(), dXl, dFl, ctx->learn_stat.as<float>(),
+                       red_out_of(ctx), ctx->learn_Tmid, ctx->learn_span, X);
     HIPCHK(hipGetLastError());
     return BTE_OK;
 }
@@ -1727,42 +1804,62 @@ static int learn_graph(bte_ctx* ctx) {
 }
 
 // Proposes a correction of the next state: x_next += T_span * GNN(features).  The next outer
-// iteration runs from the corrected state and learn_judge decides whether it stands.
+// iteration runs from the corrected state and learn_judge decides whether it stands.  Nothing
+// waits: a state with no residual to correct (learn_judge finds it from the norms) is left as it is
+// by k_apply_correction, and the proposal then counts as not made.
 static int learn_propose(bte_ctx* ctx) {
     GnnNet* net = ctx->gnn;
     const int N = ctx->N;
-    double ffT = 0, ffZ = 0;
     int rc;
     if ((rc = learn_graph(ctx))) return rc;
-    if ((rc = learn_features(ctx, net->X.as<float>(), ffT, ffZ))) return rc;
-    if (!(ffT > 0) || !std::isfinite(ffZ)) return BTE_OK;  // nothing to correct
+    if ((rc = learn_features(ctx, net->X.as<float>()))) return rc;
     if (gnn_forward_device(net, ctx->stream)) SETERR(BTE_ERR_HIP, "learned correction: network launch failed");
     const long long Z = ctx->aa_Z;
     HIPCHK(ctx->learn_backup.ensure(sizeof(double) * Z));
     HIPCHK(hipMemcpyAsync(ctx->learn_backup.p, ctx->aa_x.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_apply_correction, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->B * ctx->D,
-                       ctx->aa_nb, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span, ctx->aa_x.as<double>());
+                       ctx->aa_nb, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span, red_out_of(ctx),
+                       ctx->aa_x.as<double>());
     HIPCHK(hipGetLastError());
     int rc2;
     if ((rc2 = load_state(ctx))) return rc2;
-    ctx->learn_ref_ff = ffZ;
     ctx->learn_pending = true;
     ctx->run_proposed++;
     return BTE_OK;
 }
 
-// After the outer iteration from a corrected state: accept when its residual |G(x~) - x~| does
-// not exceed the residual of the state evaluated before the correction.
-static int learn_judge(bte_ctx* ctx, bool& accepted) {
+// The judge's inputs for the iteration that ran from a corrected state: g = G(x~), then the squared
+// norms of its residual g - x~ over the state and of the jump x~ - x over the temperature part; nothing
+// waits.
+static int launch_judge(bte_ctx* ctx) {
     const int N = ctx->N, S = ctx->B * ctx->D, nb = ctx->aa_nb;
     const long long Z = ctx->aa_Z;
     hipLaunchKernelGGL(k_state_gather, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
                        ctx->aa_g.as<double>());
     HIPCHK(hipGetLastError());
-    double ff = 0;
+    return launch_norms2(ctx, 2, Z, ctx->aa_g.as<double>(), ctx->aa_x.as<double>(), N, ctx->aa_x.as<double>(),
+                         ctx->learn_backup.as<double>());
+}
+
+// After the outer iteration from a corrected state: accept when its residual |G(x~) - x~| does
+// not exceed the residual of the state evaluated before the correction.
+// void_: the state had no residual to correct, so no correction was applied and the iteration ran
+// from the uncorrected state.  The judge's norms come from launch_judge behind the iteration
+// (judge_spec), or are formed here.
+static int learn_judge(bte_ctx* ctx, bool& accepted, bool& void_) {
     int rc;
-    if ((rc = sq_norm(ctx, ctx->aa_g.as<double>(), ctx->aa_x.as<double>(), 0, Z, ff))) return rc;
+    accepted = void_ = false;
+    if (!ctx->judge_spec && ((rc = launch_judge(ctx)) || (rc = wait_stream(ctx)))) return rc;
+    ctx->judge_spec = false;
+    const double ffT = ctx->h_sync[kSyncLearn], ffZ = ctx->h_sync[kSyncLearn + 1];
+    if (!(ffT > 0) || !std::isfinite(ffZ)) {
+        void_ = true;
+        return BTE_OK;
+    }
+    ctx->learn_ref_ff = ffZ;
+    const double ff = ctx->h_sync[kSyncJudge];
+    ctx->learn_jump = ctx->h_sync[kSyncJudge + 1];
     accepted = std::isfinite(ff) && ff <= ctx->learn_ref_ff;
     ctx->learn_ratio = ctx->learn_ref_ff > 0 ? std::sqrt(ff / ctx->learn_ref_ff) : 0.0;
     return BTE_OK;
@@ -1880,6 +1977,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
         if (ctx->gnn && (rc = learn_graph(ctx))) return rc;
     }
     ctx->run_accepted = ctx->run_proposed = ctx->run_rejected = 0;
+    ctx->judge_spec = false;
     ctx->run_shape_block = shape.block;
     ctx->run_shape_variant = shape.variant;
     ctx->run_graph = use_graph;
@@ -1924,7 +2022,13 @@ static int run_iteration(bte_ctx* ctx, double& m) {
         if (rc) return rc;
         ctx->aa_spec = true;
     }
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    // likewise the judge's norms when the iteration ran from a corrected state
+    ctx->judge_spec = false;
+    if (ctx->learn_pending && ctx->aa_Z > 0) {
+        if (int rc = launch_judge(ctx)) return rc;
+        ctx->judge_spec = true;
+    }
+    if (int rc = wait_stream(ctx)) return rc;
     m = ctx->h_sync[0];  // stored by the iteration's margin kernel
     ctx->run_iter++;
     return BTE_OK;
@@ -1942,19 +2046,18 @@ static int after_judge(bte_ctx* ctx, double m, bool& counted, bool& mix) {
     mix = false;
     if (ctx->learn_pending) {
         ctx->learn_pending = false;
-        bool ok = false;
-        if ((rc = learn_judge(ctx, ok))) return rc;
-        if (o.verbose)
+        bool ok = false, void_ = false;
+        if ((rc = learn_judge(ctx, ok, void_))) return rc;
+        if (o.verbose && !void_)
             std::printf("learned correction %s (residual ratio %.4g)\n", ok ? "accepted" : "rejected", ctx->learn_ratio);
-        if (ok) {
+        if (void_) {
+            ctx->run_proposed--;  // nothing was corrected: the iteration stands as a plain one
+        } else if (ok) {
             ctx->run_accepted++;
             ctx->learn_streak = 0;
             // the jump to the corrected state enters no mixing column: a least-squares step over it
             // would take the correction back (the residual hardly sees smooth errors)
-            double jump = 0;
-            if ((rc = sq_norm(ctx, ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), 0, ctx->N, jump)))
-                return rc;
-            ctx->aa_skip = jump > 0;
+            ctx->aa_skip = ctx->learn_jump > 0;
         } else {
             // run the iteration again from the uncorrected state
             ctx->run_rejected++;
@@ -2042,9 +2145,8 @@ int bte_learn_features(bte_ctx* ctx, float* X, double* x_next, double* span) {
     HIPCHK(hipSetDevice(ctx->device));
     const int N = ctx->N;
     HIPCHK(ctx->learn_X.ensure(sizeof(float) * (size_t)N * kGnnIn));
-    double ffT = 0, ffZ = 0;
     int rc;
-    if ((rc = learn_features(ctx, ctx->learn_X.as<float>(), ffT, ffZ))) return rc;
+    if ((rc = learn_features(ctx, ctx->learn_X.as<float>()))) return rc;
     std::vector<float> hX((size_t)N * kGnnIn);
     std::vector<double> hx(N);
     HIPCHK(hipMemcpyAsync(hX.data(), ctx->learn_X.p, sizeof(float) * hX.size(), hipMemcpyDeviceToHost, ctx->stream));
