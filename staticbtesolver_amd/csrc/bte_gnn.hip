@@ -1,7 +1,9 @@
 // bte_gnn.hip -- mesh graph network for the learned outer-iteration correction: forward, backward
-// and Adam on gfx950.  Dense products run on BF16 MFMA tiles (v_mfma_f32_16x16x16_bf16, FP32
-// accumulation); operands are FP32 in HBM and rounded to BF16 (nearest even) when loaded into the
-// MFMA registers.  Network and parameter layout: oracle/gnn_ref.py restates them on the CPU.
+// and Adam on gfx950.  Dense products run on BF16 MFMA tiles with FP32 accumulation: the training
+// passes on v_mfma_f32_16x16x16_bf16, the inference layers (one fused launch each) on gfx950's
+// v_mfma_f32_16x16x32_bf16.  Operands are FP32 in HBM and rounded to BF16 (nearest even) when
+// loaded into the MFMA registers.  Network and parameter layout: oracle/gnn_ref.py restates them on
+// the CPU.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(256) void k_gnn_enc(int N, const float* __restrict_
             const int row = row0 + 4 * (lane >> 4) + i, col = 16 * t + (lane & 15);
             if (row < N) {
                 const float z = acc[t][i] + b[col];
-                Z[(size_t)row * kGnnHid + col] = z;
+                if (Z) Z[(size_t)row * kGnnHid + col] = z;  // null: inference keeps no pre-activation
                 H[(size_t)row * kGnnHid + col] = fmaxf(z, 0.f);
             }
         }
@@ -270,6 +272,150 @@ __global__ void k_gnn_adam(int P, float* __restrict__ p, const float* __restrict
     }
 }
 
+// ---- inference: fused layers on gfx950's 16x16x32 BF16 MFMA ------------------------------------
+// v_mfma_f32_16x16x32_bf16 operand layout: lane l holds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16] for j = 0..7; the result element acc[i] is C[4 (l / 16) + i][l % 16].
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short short8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 pack_bf16x8(const float (&v)[8]) {
+    short8 s;
+#pragma unroll
+    for (int j = 0; j < 8; j++) s[j] = f2bf(v[j]);
+    return __builtin_bit_cast(bf16x8, s);
+}
+
+// Operand fragments of the message-passing weights, [layer][matrix (W1, W2)][k block][n tile][lane]:
+// lane l of fragment (kb, t) holds W[32 kb + 8 (l / 16) + j][16 t + l % 16], j = 0..7, as BF16.
+__global__ void k_gnn_wfrag(int L, const float* __restrict__ p, bf16x8* __restrict__ frag) {
+    const int total = L * 2 * 2 * 4 * 64;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int lane = e & 63, t = (e >> 6) & 3, kb = (e >> 8) & 1, m = (e >> 9) & 1, l = e >> 10;
+        const float* W = p + gnn_off_layer(l) + m * kGnnHid * kGnnHid;
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = W[(32 * kb + 8 * (lane >> 4) + j) * kGnnHid + 16 * t + (lane & 15)];
+        frag[e] = pack_bf16x8(v);
+    }
+}
+
+// One message-passing layer for inference, H_out = H + relu(H W1 + M W2 + b) with M the mean of H
+// over the interior neighbours (as k_gnn_agg forms it, in FP32 and the same order), without storing
+// M or the pre-activation.  Each wave takes 16-row tiles; its 16 weight fragments stay in
+// registers.  The tile's own rows pass through the wave's LDS tile twice: in the operand layout
+// for the residual term read in the result layout, then H_out in the result layout for stores of
+// whole rows.  Waves take different numbers of tiles, so only wave-scope ordering is used.  DEC: the
+// last layer -- instead of H_out, the decoder y = H_out w_out + b_out.
+template <bool DEC>
+__global__ __launch_bounds__(256) void k_gnn_infer_layer(int N, int F, const int* __restrict__ nbr,
+                                                        const float* __restrict__ H, const bf16x8* __restrict__ wf,
+                                                        const float* __restrict__ b, float* __restrict__ Hn,
+                                                        const float* __restrict__ wo, const float* __restrict__ bo,
+                                                        float* __restrict__ y) {
+    constexpr int TS = kGnnHid + 4;  // LDS row stride (floats): rows stay 16-byte aligned
+    __shared__ float tiles[4][16 * TS];
+    float* tl = tiles[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63, r = lane & 15, q = lane >> 4;
+    const int ntile = (N + 15) / 16;
+    bf16x8 w1[2][4], w2[2][4];
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            w1[kb][t] = wf[((0 * 2 + kb) * 4 + t) * 64 + lane];
+            w2[kb][t] = wf[((1 * 2 + kb) * 4 + t) * 64 + lane];
+        }
+    float bias[4], wout[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        bias[t] = b[16 * t + r];
+        wout[t] = DEC ? wo[16 * t + r] : 0.f;
+    }
+    for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntile; tile += gridDim.x * 4) {
+        // A operands: row tile*16 + r (clamped: rows past N compute discarded outputs), columns
+        // 32 kb + 8 q + j of H and of the neighbour mean
+        const int rr = min(tile * 16 + r, N - 1);
+        float h[2][8], mv[2][8];
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) {
+            const float4* src = (const float4*)(H + (size_t)rr * kGnnHid + 32 * kb + 8 * q);
+            const float4 u0 = src[0], u1 = src[1];
+            h[kb][0] = u0.x, h[kb][1] = u0.y, h[kb][2] = u0.z, h[kb][3] = u0.w;
+            h[kb][4] = u1.x, h[kb][5] = u1.y, h[kb][6] = u1.z, h[kb][7] = u1.w;
+            float4* dst = (float4*)(tl + r * TS + 32 * kb + 8 * q);
+            dst[0] = u0;
+            dst[1] = u1;
+#pragma unroll
+            for (int j = 0; j < 8; j++) mv[kb][j] = 0.f;
+        }
+        int d = 0;
+        for (int f = 0; f < F; f++) {
+            const int nb = nbr[(size_t)rr * F + f];
+            if (nb < 0) continue;
+            d++;
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                const float4* src = (const float4*)(H + (size_t)nb * kGnnHid + 32 * kb + 8 * q);
+                const float4 u0 = src[0], u1 = src[1];
+                mv[kb][0] += u0.x, mv[kb][1] += u0.y, mv[kb][2] += u0.z, mv[kb][3] += u0.w;
+                mv[kb][4] += u1.x, mv[kb][5] += u1.y, mv[kb][6] += u1.z, mv[kb][7] += u1.w;
+            }
+        }
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) mv[kb][j] = d ? mv[kb][j] / (float)d : 0.f;
+        f32x4 acc[4] = {};
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) {
+            const bf16x8 a = pack_bf16x8(h[kb]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w1[kb][t], acc[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) {
+            const bf16x8 a = pack_bf16x8(mv[kb]);
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w2[kb][t], acc[t], 0, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the tile's rows are in LDS
+        // result element (row 4 q + i, column 16 t + r)
+        float hn[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) hn[i][t] = tl[(4 * q + i) * TS + 16 * t + r] + fmaxf(acc[t][i] + bias[t], 0.f);
+        if (DEC) {
+            // y of row 4 q + i: the 64 column terms over the 16 lanes of the group (r = 0..15)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float v = 0.f;
+#pragma unroll
+                for (int t = 0; t < 4; t++) v += hn[i][t] * wout[t];
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+                const int row = tile * 16 + 4 * q + i;
+                if (r == 0 && row < N) y[row] = v + bo[0];
+            }
+        } else {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // every lane has read its residuals
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) tl[(4 * q + i) * TS + 16 * t + r] = hn[i][t];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // the 16 rows are contiguous in H_out: 256 float4, four per lane
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int e = lane + 64 * k, row = e >> 4, c4 = e & 15;
+                if (tile * 16 + row < N)
+                    *(float4*)(Hn + (size_t)(tile * 16 + row) * kGnnHid + 4 * c4) = *(const float4*)(tl + row * TS + 4 * c4);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // LDS reads done before the next tile's writes
+    }
+}
+
 static int grid1(long long n, int b = 256) { return (int)std::max<long long>(1, std::min<long long>((n + b - 1) / b, 65536)); }
 
 int gnn_ensure(GnnNet* net, int n) {
@@ -319,6 +465,37 @@ int gnn_forward_device(GnnNet* net, hipStream_t stream) {
     const float* wo = p + gnn_off_out(L);
     hipLaunchKernelGGL(k_gnn_dec, dim3(grid1(N)), dim3(256), 0, st, N, net->H[L].as<float>(), wo, wo + kGnnHid,
                        net->y.as<float>());
+    return hipGetLastError() == hipSuccess ? 0 : BTE_ERR_HIP;
+}
+
+int gnn_infer_device(GnnNet* net, hipStream_t stream) {
+    const int N = net->N, L = net->L;
+    const float* p = net->params.as<float>();
+    hipStream_t st = stream ? stream : net->stream;
+    const size_t per_layer = 2 * 2 * 4 * 64;
+    if (net->wfrag_ver != net->params_ver) {
+        if (net->wfrag.ensure(sizeof(bf16x8) * per_layer * L) != hipSuccess) return BTE_ERR_HIP;
+        hipLaunchKernelGGL(k_gnn_wfrag, dim3(grid1((long long)per_layer * L)), dim3(256), 0, st, L, p,
+                           net->wfrag.as<bf16x8>());
+        net->wfrag_ver = net->params_ver;
+    }
+    hipLaunchKernelGGL(k_gnn_enc, dim3((N + 63) / 64), dim3(256), 0, st, N, net->X.as<float>(), p,
+                       p + kGnnIn * kGnnHid, (float*)nullptr, net->H[0].as<float>());
+    const int ntile = (N + 15) / 16;
+    const dim3 grid(std::max(1, std::min((ntile + 3) / 4, 65536)));  // a tile per wave
+    const float* wo = p + gnn_off_out(L);
+    for (int l = 0; l < L; l++) {
+        const float* H = net->H[l & 1].as<float>();
+        float* Hn = net->H[(l + 1) & 1].as<float>();
+        const bf16x8* wf = net->wfrag.as<bf16x8>() + per_layer * l;
+        const float* b = p + gnn_off_layer(l) + 2 * kGnnHid * kGnnHid;
+        if (l + 1 < L)
+            hipLaunchKernelGGL(k_gnn_infer_layer<false>, grid, dim3(256), 0, st, N, net->F, net->nbr.as<int>(), H, wf,
+                               b, Hn, wo, wo + kGnnHid, net->y.as<float>());
+        else
+            hipLaunchKernelGGL(k_gnn_infer_layer<true>, grid, dim3(256), 0, st, N, net->F, net->nbr.as<int>(), H, wf,
+                               b, Hn, wo, wo + kGnnHid, net->y.as<float>());
+    }
     return hipGetLastError() == hipSuccess ? 0 : BTE_ERR_HIP;
 }
 
@@ -430,6 +607,7 @@ int bte_gnn_create(int device, int layers, unsigned long long seed, bte_gnn** ou
     }
     fill(gnn_off_out(layers), kGnnHid, 0.01);
     (void)hipMemcpy(g->params.p, p.data(), sizeof(float) * g->P, hipMemcpyHostToDevice);
+    g->params_ver++;
     (void)hipMemset(g->adam_m.p, 0, sizeof(float) * g->P);
     (void)hipMemset(g->adam_v.p, 0, sizeof(float) * g->P);
     (void)hipMemset(g->grads.p, 0, sizeof(float) * g->P);
@@ -462,6 +640,7 @@ int bte_gnn_set_params(bte_gnn* g, const float* params) {
     if (!g || !params) return BTE_ERR_ARG;
     GCHK(hipSetDevice(g->device));
     GCHK(hipMemcpy(g->params.p, params, sizeof(float) * g->P, hipMemcpyHostToDevice));
+    g->params_ver++;
     GCHK(hipMemset(g->adam_m.p, 0, sizeof(float) * g->P));
     GCHK(hipMemset(g->adam_v.p, 0, sizeof(float) * g->P));
     g->adam_t = 0;
@@ -505,7 +684,7 @@ int bte_gnn_forward(bte_gnn* g, const float* x, float* y) {
     if (!g || !x || g->N <= 0) return BTE_ERR_ARG;
     GCHK(hipSetDevice(g->device));
     GCHK(hipMemcpyAsync(g->X.p, x, sizeof(float) * (size_t)g->N * kGnnIn, hipMemcpyHostToDevice, g->stream));
-    if (gnn_forward_device(g)) {
+    if (gnn_infer_device(g)) {
         g->err = "forward launch failed";
         return BTE_ERR_HIP;
     }
@@ -541,6 +720,7 @@ int bte_gnn_adam(bte_gnn* g, float lr, float beta1, float beta2, float eps, floa
     if (!g) return BTE_ERR_ARG;
     GCHK(hipSetDevice(g->device));
     g->adam_t++;
+    g->params_ver++;
     const float c1 = 1.f - std::pow(beta1, (float)g->adam_t), c2 = 1.f - std::pow(beta2, (float)g->adam_t);
     hipLaunchKernelGGL(k_gnn_adam, dim3(grid1(g->P)), dim3(256), 0, g->stream, g->P, g->params.as<float>(),
                        g->grads.as<float>(), g->adam_m.as<float>(), g->adam_v.as<float>(), lr, beta1, beta2, eps,

@@ -78,17 +78,27 @@ struct GnnNet {
     std::vector<DBuf> H, M, Z;          // H[0..L], M[0..L-1], Z[0..L-1]
     DBuf dH, dH2, dZ, G1, G2, part, red;
     size_t part_elems = 0;
+    // inference: the message-passing weights as BF16 MFMA operand fragments (k_gnn_wfrag), built
+    // from the parameters of version wfrag_ver; params_ver counts parameter changes
+    DBuf wfrag;
+    unsigned long long params_ver = 1, wfrag_ver = 0;
 };
 
 // Offsets of the parameter blocks in the flat vector (layout shared with oracle/gnn_ref.py).
-inline int gnn_off_layer(int l) { return kGnnIn * kGnnHid + kGnnHid + l * (2 * kGnnHid * kGnnHid + kGnnHid); }
+__host__ __device__ inline int gnn_off_layer(int l) {
+    return kGnnIn * kGnnHid + kGnnHid + l * (2 * kGnnHid * kGnnHid + kGnnHid);
+}
 inline int gnn_off_out(int L) { return gnn_off_layer(L); }
 inline int gnn_num_params(int L) { return gnn_off_out(L) + kGnnHid + 1; }
 
-// Forward pass on device data already in net->X (N rows); writes net->y.  Launches on `stream`
-// (the network's own stream when null).  Used by the solver's learned correction.  Returns 0 or
-// a negative bte_status.
+// Forward pass on device data already in net->X (N rows); writes net->y, and keeps the activations
+// the backward pass reads.  Launches on `stream` (the network's own stream when null).  Returns 0
+// or a negative bte_status.
 int gnn_forward_device(GnnNet* net, hipStream_t stream = nullptr);
+// The same output y without the backward pass's activations: one fused launch per layer (mean
+// aggregation, both products, residual; the decoder in the last), on BF16 operand fragments of
+// the weights.  Used by the solver's learned correction and bte_gnn_forward.
+int gnn_infer_device(GnnNet* net, hipStream_t stream = nullptr);
 // Ensures activation buffers for n nodes.
 int gnn_ensure(GnnNet* net, int n);
 

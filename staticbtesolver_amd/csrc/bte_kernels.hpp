@@ -1705,62 +1705,39 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 // q = 2 ncols: f . f
 // Grid (nblk, ceil(ncols / G) + 1): work-group (x, y < ceil(ncols / G)) reads columns
 // dF_{Gy .. Gy+G-1} and, once per entry for all of them, dF_new and f; (x, last) forms f . f.  Each
-// thread takes the entries c, c + stride, ... of its work-group's share, U of them per pass with
-// every load of the pass issued first (at clamped, always valid indices; an entry past the end
-// contributes 0), so a pass costs one memory latency.  Each sum runs over its entries in the same
-// fixed order for every call.
-template <int BLOCK, int G, int U>
+// sum runs over the work-group's chunk in the same fixed order for every call.  (Measured on the
+// 50-column history of a 180k-entry state: G = 4 with 256-thread work-groups takes 15 us; wider
+// groups spend more on the work-group's cross-lane sums of 2 G values than they save in reads.)
+template <int BLOCK, int G>
 __global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const float* __restrict__ dF,
                                                   const float* __restrict__ dF_new, const double* __restrict__ f,
                                                   double* __restrict__ partial) {
     __shared__ double lds[2 * G * (BLOCK / 64)];
     const int j0 = blockIdx.y * G, nblk = gridDim.x;
-    const int stride = nblk * BLOCK;
     double s[2 * G];
 #pragma unroll
     for (int k = 0; k < 2 * G; k++) s[k] = 0;
     if (j0 < ncols) {
-        const int ng = min(G, ncols - j0);  // columns of this work-group (uniform)
-        for (int c0 = blockIdx.x * BLOCK + threadIdx.x; c0 < N; c0 += U * stride) {
-            double a[U], b[U];
-            float v[U][G];
+        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
+            const double a = dF_new[c], b = f[c];
+            double v[G];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int c = min(c0 + u * stride, N - 1);
-                a[u] = dF_new[c];
-                b[u] = f[c];
+            for (int g = 0; g < G; g++) v[g] = j0 + g < ncols ? (double)dF[(size_t)(j0 + g) * N + c] : 0.0;
 #pragma unroll
-                for (int g = 0; g < G; g++) v[u][g] = dF[(size_t)(j0 + min(g, ng - 1)) * N + c];
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const bool in = c0 + u * stride < N;
-                const double au = in ? a[u] : 0.0, bu = in ? b[u] : 0.0;
-#pragma unroll
-                for (int g = 0; g < G; g++)
-                    if (g < ng) {
-                        s[g] += (double)v[u][g] * au;
-                        s[G + g] += (double)v[u][g] * bu;
-                    }
+            for (int g = 0; g < G; g++) {
+                s[g] += v[g] * a;
+                s[G + g] += v[g] * b;
             }
         }
         block_sum<BLOCK, 2 * G>(s, lds);
         if (threadIdx.x == 0)
-            for (int g = 0; g < ng; g++) {
+            for (int g = 0; g < G && j0 + g < ncols; g++) {
                 partial[(size_t)(j0 + g) * nblk + blockIdx.x] = s[g];
                 partial[(size_t)(ncols + j0 + g) * nblk + blockIdx.x] = s[G + g];
             }
     } else {
-        for (int c0 = blockIdx.x * BLOCK + threadIdx.x; c0 < N; c0 += U * stride) {
-            double b[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) b[u] = f[min(c0 + u * stride, N - 1)];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const double bu = c0 + u * stride < N ? b[u] : 0.0;
-                s[0] += bu * bu;
-            }
-        }
+#pragma unroll 4
+        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];
         block_sum<BLOCK, 1>(s, lds);
         if (threadIdx.x == 0) partial[(size_t)(2 * ncols) * nblk + blockIdx.x] = s[0];
     }
@@ -1792,15 +1769,19 @@ __global__ void k_signal(long long seq, long long* __restrict__ host) {
 // reflecting (diffuse or specular) boundary face -- the only parts of the previous iterate the next
 // right-hand sides read.  Intensities are carried as temperature equivalents e * Omega / C_b.
 // z = [T (N) ; e_bnd (S x nb)].
-// g = G(x) gathered as k_state_gather does, and in the same pass f = g - x into f_cur and f_prev,
-// x into x_prev and, unless first, the history columns dX = x - x_prev and dF = f - f_prev (from
-// the previous step's values), over the Z = N + S nb state entries.
+// g = G(x) gathered (T, and the boundary intensities at bcells scaled by sc), and in the same pass
+// f = g - x into f_cur and f_prev, x into x_prev and, unless first, the history columns
+// dX = x - x_prev and dF = f - f_prev (from the previous step's values), over the Z = N + S nb
+// state entries.  take: then x = g (a plain
+// step), and with dens the state's loads for the next iteration from it, as k_set_density and
+// k_state_scatter form them (the emission density of the T part, the boundary intensities).
 __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __restrict__ bcells,
                                     const double* __restrict__ sc, const double* __restrict__ T,
-                                    const double* __restrict__ ee, double* __restrict__ z, int first,
-                                    const double* __restrict__ x, double* __restrict__ x_prev,
+                                    double* __restrict__ ee, double* __restrict__ z, int first,
+                                    double* __restrict__ x, double* __restrict__ x_prev,
                                     double* __restrict__ f_prev, double* __restrict__ f_cur, double* __restrict__ dX,
-                                    float* __restrict__ dF, float* __restrict__ dG) {
+                                    float* __restrict__ dF, float* __restrict__ dG, int take, Bands bands,
+                                    double* __restrict__ dens) {
     const long long total = N + (long long)S * nb;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
@@ -1824,20 +1805,18 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
         x_prev[t] = xv;
         f_prev[t] = f;
         f_cur[t] = f;
-    }
-}
-
-__global__ void k_state_gather(int N, int D, int S, int nb, const int* __restrict__ bcells, const double* __restrict__ sc,
-                               const double* __restrict__ T, const double* __restrict__ ee, double* __restrict__ z) {
-    const long long total = N + (long long)S * nb;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
-        if (t < N) {
-            z[t] = T[t];
-        } else {
-            const long long u = t - N;
-            const int s = (int)(u / nb), j = (int)(u % nb);
-            z[t] = ee[(size_t)s * N + bcells[j]] * sc[s / D];
+        if (take) {
+            x[t] = gv;
+            if (dens) {
+                if (t < N) {
+                    for (int b = 0; b < bands.B; b++)
+                        dens[(size_t)t * bands.B + b] = bands.C[b] / bands.solid_angle * (gv - bands.T_ref);
+                } else {
+                    const long long u = t - N;
+                    const int s = (int)(u / nb), j = (int)(u % nb);
+                    ee[(size_t)s * N + bcells[j]] = gv / sc[s / D];
+                }
+            }
         }
     }
 }
@@ -1890,16 +1869,33 @@ __global__ void k_learn_features(int N, const double* __restrict__ f, const doub
 
 // z += s * y on the temperature part and, as a local equilibrium shift, on the reflecting-boundary
 // intensities of each corrected cell (they are carried as temperature equivalents).
-// ff: device address of the proposal's residual norms {over T, over the whole state}; a state with
-// no residual (or a non-finite one) is left as it is.
-__global__ void k_apply_correction(int N, int S, int nb, const int* __restrict__ bcells, const float* __restrict__ y,
-                                   double s, const double* __restrict__ ff, double* __restrict__ z) {
-    if (!(ff[0] > 0) || !isfinite(ff[1])) return;
+// The learned correction z += span y (per cell; a boundary entry takes its cell's), with the state
+// before it kept in backup and the state's loads for the next iteration formed from the corrected
+// z as k_set_density and k_state_scatter form them.  ff: device address of the proposal's residual
+// norms {over T, over the whole state}; a state with no residual (or a non-finite one) is only
+// backed up.
+__global__ void k_apply_correction(int N, int D, int S, int nb, const int* __restrict__ bcells,
+                                   const float* __restrict__ y, double span, const double* __restrict__ ff,
+                                   double* __restrict__ z, double* __restrict__ backup, Bands bands,
+                                   double* __restrict__ dens, const double* __restrict__ sc, double* __restrict__ ee) {
+    const bool apply = ff[0] > 0 && isfinite(ff[1]);
     const long long total = N + (long long)S * nb;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
          t += (long long)gridDim.x * blockDim.x) {
+        const double v0 = z[t];
+        backup[t] = v0;
+        if (!apply) continue;
         const int c = t < N ? (int)t : bcells[(t - N) % nb];
-        z[t] += s * (double)y[c];
+        const double v = v0 + span * (double)y[c];
+        z[t] = v;
+        if (t < N) {
+            for (int b = 0; b < bands.B; b++)
+                dens[(size_t)t * bands.B + b] = bands.C[b] / bands.solid_angle * (v - bands.T_ref);
+        } else {
+            const long long u = t - N;
+            const int s = (int)(u / nb), j = (int)(u % nb);
+            ee[(size_t)s * N + bcells[j]] = v / sc[s / D];
+        }
     }
 }
 
@@ -1942,6 +1938,40 @@ __global__ __launch_bounds__(BLOCK) void k_sqdiff2(long long n1, const double* _
     if (threadIdx.x == 0) {
         partial[blockIdx.x] = s[0];
         partial[gridDim.x + blockIdx.x] = s[1];
+    }
+}
+
+// The judge's two squared norms in one pass, summed as k_sqdiff2 sums them: over the Z = N + S nb
+// state entries of (g - x) with g = G(x~) gathered as k_state_gather_diff does (not stored), and over
+// [0, N) of (x - backup).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_judge_norms(int N, int D, int S, int nb, const int* __restrict__ bcells,
+                                                      const double* __restrict__ sc, const double* __restrict__ T,
+                                                      const double* __restrict__ ee, const double* __restrict__ x,
+                                                      const double* __restrict__ backup, double* __restrict__ partial) {
+    __shared__ double lds[2 * (BLOCK / 64)];
+    double acc[2] = {0, 0};
+    const long long Z = N + (long long)S * nb, st = (long long)gridDim.x * BLOCK;
+    for (long long i = blockIdx.x * (long long)BLOCK + threadIdx.x; i < Z; i += st) {
+        double gv;
+        if (i < N) {
+            gv = T[i];
+        } else {
+            const long long u = i - N;
+            const int s = (int)(u / nb), j = (int)(u % nb);
+            gv = ee[(size_t)s * N + bcells[j]] * sc[s / D];
+        }
+        const double d = gv - x[i];
+        acc[0] += d * d;
+        if (i < N) {
+            const double e = x[i] - backup[i];
+            acc[1] += e * e;
+        }
+    }
+    block_sum<BLOCK, 2>(acc, lds);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = acc[0];
+        partial[gridDim.x + blockIdx.x] = acc[1];
     }
 }
 

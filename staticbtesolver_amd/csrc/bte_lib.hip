@@ -1524,17 +1524,18 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     // g = G(x): the new temperature field and reflecting-boundary intensities, gathered in the same
     // pass as the differences f = g - x (and, for a history step, the columns dF and dG and the
     // latest dX)
-    auto gather_diff = [&](int first, float* dF, float* dG) {
+    // take: a plain step x = g in the same pass (load: with the state's loads for the next iteration)
+    auto gather_diff = [&](int first, float* dF, float* dG, int take = 0, bool load = false) {
         hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
                            ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
                            ctx->aa_g.as<double>(), first, ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(),
-                           ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), dF, dG);
+                           ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), dF, dG, take,
+                           bands_of(ctx), load ? ctx->dens.as<double>() : (double*)nullptr);
     };
     if (ctx->aa_plain) {
         // plain iteration: x_{k+1} = G(x_k); the iteration already left T, density and the boundary
         // intensities of x_{k+1} in place
-        gather_diff(1, ctx->aa_dF.as<float>(), ctx->aa_dG.as<float>());
-        HIPCHK(hipMemcpyAsync(ctx->aa_x.p, ctx->aa_g.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
+        gather_diff(1, ctx->aa_dF.as<float>(), ctx->aa_dG.as<float>(), 1);
         HIPCHK(hipGetLastError());
         return BTE_OK;
     }
@@ -1551,20 +1552,19 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
             ctx->aa_order.erase(ctx->aa_order.begin());
         }
     }
-    const double* g = ctx->aa_g.as<double>();
+    // the first step is plain, x_1 = G(x_0), taken (with the state's loads) in the same pass
     gather_diff((ctx->aa_first || skip) ? 1 : 0, ctx->aa_dF.as<float>() + (size_t)slot * Z,
-                ctx->aa_dG.as<float>() + (size_t)slot * Z);
+                ctx->aa_dG.as<float>() + (size_t)slot * Z, ctx->aa_first ? 1 : 0, ctx->aa_first);
     if (ctx->aa_first) {
-        // plain step: x_1 = G(x_0)
-        HIPCHK(hipMemcpyAsync(ctx->aa_x.p, g, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipGetLastError());
         ctx->aa_first = false;
-        return load_state(ctx);
+        return BTE_OK;
     }
     if (!skip) ctx->aa_order.push_back(slot);
     // the filled slots are [0, n): slots are taken in order and reused only once all m are filled
     const int n = (int)ctx->aa_order.size();
     const int nq = 2 * n + 1, nblk = 64;
-    hipLaunchKernelGGL((k_aa_dots<1024, 8, 2>), dim3(nblk, (n + 7) / 8 + 1), dim3(1024), 0, ctx->stream, Z, n,
+    hipLaunchKernelGGL((k_aa_dots<256, 4>), dim3(nblk, (n + 3) / 4 + 1), dim3(256), 0, ctx->stream, Z, n,
                        ctx->aa_dF.as<float>(), ctx->aa_dF.as<float>() + (size_t)(skip ? 0 : slot) * Z,
                        ctx->aa_f.as<double>(), aa_part_of(ctx));
     hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
@@ -1813,33 +1813,36 @@ static int learn_propose(bte_ctx* ctx) {
     int rc;
     if ((rc = learn_graph(ctx))) return rc;
     if ((rc = learn_features(ctx, net->X.as<float>()))) return rc;
-    if (gnn_forward_device(net, ctx->stream)) SETERR(BTE_ERR_HIP, "learned correction: network launch failed");
+    if (gnn_infer_device(net, ctx->stream)) SETERR(BTE_ERR_HIP, "learned correction: network launch failed");
     const long long Z = ctx->aa_Z;
     HIPCHK(ctx->learn_backup.ensure(sizeof(double) * Z));
-    HIPCHK(hipMemcpyAsync(ctx->learn_backup.p, ctx->aa_x.p, sizeof(double) * Z, hipMemcpyDeviceToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_apply_correction, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->B * ctx->D,
-                       ctx->aa_nb, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span, red_out_of(ctx),
-                       ctx->aa_x.as<double>());
+    // backup, correction and the corrected state's loads (load_state's work) in one pass
+    hipLaunchKernelGGL(k_apply_correction, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->D,
+                       ctx->B * ctx->D, ctx->aa_nb, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span,
+                       red_out_of(ctx), ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), bands_of(ctx),
+                       ctx->dens.as<double>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
-    int rc2;
-    if ((rc2 = load_state(ctx))) return rc2;
     ctx->learn_pending = true;
     ctx->run_proposed++;
     return BTE_OK;
 }
 
-// The judge's inputs for the iteration that ran from a corrected state: g = G(x~), then the squared
-// norms of its residual g - x~ over the state and of the jump x~ - x over the temperature part; nothing
-// waits.
+// The judge's inputs for the iteration that ran from a corrected state: the squared norms of its
+// residual G(x~) - x~ over the state (G gathered in the same pass) and of the jump x~ - x over the
+// temperature part, into red_out_of(ctx)[2, 4) and h_sync[kSyncJudge, + 2); nothing waits.
 static int launch_judge(bte_ctx* ctx) {
-    const int N = ctx->N, S = ctx->B * ctx->D, nb = ctx->aa_nb;
-    const long long Z = ctx->aa_Z;
-    hipLaunchKernelGGL(k_state_gather, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
+    const int N = ctx->N, S = ctx->B * ctx->D, nb = ctx->aa_nb, nblk = 32;
+    if (!ctx->use_scratch) {
+        HIPCHK(ctx->red_part.ensure(sizeof(double) * 2 * nblk));
+        HIPCHK(ctx->red_out.ensure(sizeof(double) * 4));
+    }
+    hipLaunchKernelGGL(k_judge_norms<1024>, dim3(nblk), dim3(1024), 0, ctx->stream, N, ctx->D, S, nb,
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
-                       ctx->aa_g.as<double>());
+                       ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), red_part_of(ctx));
+    hipLaunchKernelGGL(k_reduce_rows, dim3(2), dim3(256), 0, ctx->stream, 2, nblk, red_part_of(ctx),
+                       red_out_of(ctx) + 2, ctx->d_sync + kSyncJudge);
     HIPCHK(hipGetLastError());
-    return launch_norms2(ctx, 2, Z, ctx->aa_g.as<double>(), ctx->aa_x.as<double>(), N, ctx->aa_x.as<double>(),
-                         ctx->learn_backup.as<double>());
+    return BTE_OK;
 }
 
 // After the outer iteration from a corrected state: accept when its residual |G(x~) - x~| does

@@ -1,4 +1,5 @@
-"""GPU: BF16-MFMA graph network kernels against the CPU reference (oracle/gnn_ref.py).
+"""GPU: BF16-MFMA graph network kernels against the CPU reference (oracle/gnn_ref.py): the fused
+inference layers (forward) and the training passes (gradients, which run the training forward).
 
 The reference rounds every GEMM operand to bfloat16 exactly as the kernels do and accumulates in
 FP64; the kernels accumulate in FP32, so outputs and gradients agree to ~1e-4 relative (the
@@ -43,6 +44,24 @@ def test_forward_matches_reference(graph, layers):
     g.set_params(p)
     y = g.forward(X)
     yr, _ = R.forward(p, X, graph, layers, bf=True)
+    assert np.max(np.abs(y - yr)) <= 2e-4 * np.max(np.abs(yr)) + 1e-6
+
+
+def test_forward_matches_reference_on_a_large_mesh(tmp_path):
+    """40k nodes (2,514 row tiles over 629 work-groups); the last tile is partial (the node count is
+    not a multiple of 16)."""
+    from staticbtesolver_amd import meshgen
+
+    say("gnn forward, large mesh")
+    p = meshgen.make_square_case(str(tmp_path), 40000, 7, 5, 7, L=1e-6)
+    nbr = capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=1e-6, Ly=1e-6).geometry()["nbr"]
+    assert nbr.shape[0] % 16 != 0
+    par, X, _ = _setup(nbr, 4, 5)
+    g = Gnn(layers=4)
+    g.set_graph(nbr)
+    g.set_params(par)
+    y = g.forward(X)
+    yr, _ = R.forward(par, X, nbr, 4, bf=True)
     assert np.max(np.abs(y - yr)) <= 2e-4 * np.max(np.abs(yr)) + 1e-6
 
 
