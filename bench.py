@@ -45,6 +45,9 @@ CONFIGS = {
     "2d-20k": ("square", 20000, 10, 2, 8, 8, 1e-7, 2),
     "2d-ref12k": ("file", None, None, 2, 8, 8, 1e-8, None),
     "1d-100": ("1d", 100, 3, 2, 8, 1, 1e-8, 1),
+    # 3D: a seeded jittered Kuhn lattice of 32^3 cubes (196,608 tetrahedra).  Delaunay tetrahedra of
+    # seeded points (meshgen.cube_delaunay_mesh) hold slivers on which the reference's source
+    # iteration diverges (the oracle's too): there is no plain-form baseline on them.
     "3d-200k": ("cube", 32, 20, 3, 4, 4, 1e-7, 3),
 }
 

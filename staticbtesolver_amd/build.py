@@ -6,4 +6,5 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def build(jobs: int = 8) -> None:
-    subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "all"], check=True)
+    """The product (library, C++ facade, BTEcmd), then the profiling probe (staticbtesolver_amd/profile.sh)."""
+    subprocess.run(["make", "-s", "-C", HERE, f"-j{jobs}", "all", "probe"], check=True)

@@ -1,8 +1,7 @@
 // bte_gnn.hip -- mesh graph network for the learned outer-iteration correction: forward, backward
-// and Adam on gfx950.  Dense products run on BF16 MFMA tiles with FP32 accumulation: the training
-// passes on v_mfma_f32_16x16x16_bf16, the inference layers (one fused launch each) on gfx950's
-// v_mfma_f32_16x16x32_bf16.  Operands are FP32 in HBM and rounded to BF16 (nearest even) when
-// loaded into the MFMA registers.  Network and parameter layout: oracle/gnn_ref.py restates them on
+// and Adam on gfx950.  Dense products run on gfx950's BF16 MFMA tiles, v_mfma_f32_16x16x32_bf16,
+// with FP32 accumulation (the inference layers fused into one launch each).  Operands are FP32 in
+// HBM and rounded to BF16 (nearest even) when loaded into the MFMA registers.  Network and parameter layout: oracle/gnn_ref.py restates them on
 // the CPU.
 #include <hip/hip_runtime.h>
 
@@ -10,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bte.h"
@@ -17,7 +17,6 @@
 
 namespace bte {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ short f2bf(float x) {
@@ -26,24 +25,37 @@ __device__ __forceinline__ short f2bf(float x) {
     return (short)(u >> 16);
 }
 
+// gfx950's v_mfma_f32_16x16x32_bf16 operand layout: lane l holds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16] for j = 0..7; the result element acc[i] is C[4 (l / 16) + i][l % 16].
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short short8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 pack_bf16x8(const float (&v)[8]) {
+    short8 s;
+#pragma unroll
+    for (int j = 0; j < 8; j++) s[j] = f2bf(v[j]);
+    return __builtin_bit_cast(bf16x8, s);
+}
+
 // One wave accumulates a 16 x 64 output block: acc[t] is the 16 x 16 tile of columns 16t..16t+15.
-// a_at(r, k): row r (0..15) of the A block, column k; b_at(k, n): B[k][n].  K is a multiple of 16.
-// MFMA 16x16x16 operand layout: lane l holds A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16]
-// for j = 0..3; the result element acc[t][i] is C[4 (l / 16) + i][16 t + l % 16].
+// a_at(r, k): row r (0..15) of the A block, column k; b_at(k, n): B[k][n].  K is a multiple of 16;
+// the k steps are 32 wide, a last half step padded with zeros.
 template <class FA, class FB>
 __device__ __forceinline__ void wave_mm_16x64(const FA& a_at, const FB& b_at, int K, f32x4 (&acc)[4]) {
     const int lane = threadIdx.x & 63;
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bool in = k0 + kq < K;  // a lane's 8 k values are all in range or all past K
+        float av[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) a[j] = f2bf(a_at(r, k0 + kq + j));
+        for (int j = 0; j < 8; j++) av[j] = in ? a_at(r, k0 + kq + j) : 0.f;
+        const bf16x8 a = pack_bf16x8(av);
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            bf16x4 b;
+            float bv[8];
 #pragma unroll
-            for (int j = 0; j < 4; j++) b[j] = f2bf(b_at(k0 + kq + j, 16 * t + r));
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int j = 0; j < 8; j++) bv[j] = in ? b_at(k0 + kq + j, 16 * t + r) : 0.f;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, pack_bf16x8(bv), acc[t], 0, 0, 0);
         }
     }
 }
@@ -129,23 +141,24 @@ __global__ __launch_bounds__(256) void k_gnn_wgrad(int N, int KA, const float* _
     const int n_beg = blockIdx.x * kGnnChunk;
     const int n_end = min(N, n_beg + kGnnChunk);
     f32x4 acc[4] = {};
-    const int r = lane & 15, kq = (lane >> 4) * 4;
-    for (int n0 = n_beg; n0 < n_end; n0 += 16) {
-        bf16x4 a;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    for (int n0 = n_beg; n0 < n_end; n0 += 32) {
+        float av[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < 8; j++) {
             const int n = n0 + kq + j;
-            a[j] = f2bf(n < n_end ? A[(size_t)n * KA + i0 + r] : 0.f);
+            av[j] = n < n_end ? A[(size_t)n * KA + i0 + r] : 0.f;
         }
+        const bf16x8 a = pack_bf16x8(av);
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            bf16x4 b;
+            float bv[8];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < 8; j++) {
                 const int n = n0 + kq + j;
-                b[j] = f2bf(n < n_end ? G[(size_t)n * kGnnHid + 16 * t + r] : 0.f);
+                bv[j] = n < n_end ? G[(size_t)n * kGnnHid + 16 * t + r] : 0.f;
             }
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, pack_bf16x8(bv), acc[t], 0, 0, 0);
         }
     }
     const int nblk = gridDim.x;
@@ -272,18 +285,7 @@ __global__ void k_gnn_adam(int P, float* __restrict__ p, const float* __restrict
     }
 }
 
-// ---- inference: fused layers on gfx950's 16x16x32 BF16 MFMA ------------------------------------
-// v_mfma_f32_16x16x32_bf16 operand layout: lane l holds A[l % 16][8 (l / 16) + j] and
-// B[8 (l / 16) + j][l % 16] for j = 0..7; the result element acc[i] is C[4 (l / 16) + i][l % 16].
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef short short8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ bf16x8 pack_bf16x8(const float (&v)[8]) {
-    short8 s;
-#pragma unroll
-    for (int j = 0; j < 8; j++) s[j] = f2bf(v[j]);
-    return __builtin_bit_cast(bf16x8, s);
-}
+// ---- inference: fused layers --------------------------------------------------------------------
 
 // Operand fragments of the message-passing weights, [layer][matrix (W1, W2)][k block][n tile][lane]:
 // lane l of fragment (kb, t) holds W[32 kb + 8 (l / 16) + j][16 t + l % 16], j = 0..7, as BF16.
@@ -304,9 +306,13 @@ __global__ void k_gnn_wfrag(int L, const float* __restrict__ p, bf16x8* __restri
 // M or the pre-activation.  Each wave takes 16-row tiles; its 16 weight fragments stay in
 // registers.  The tile's own rows pass through the wave's LDS tile twice: in the operand layout
 // for the residual term read in the result layout, then H_out in the result layout for stores of
-// whole rows.  Waves take different numbers of tiles, so only wave-scope ordering is used.  DEC: the
-// last layer -- instead of H_out, the decoder y = H_out w_out + b_out.
-template <bool DEC>
+// whole rows.  Waves take different numbers of tiles, so only wave-scope ordering is used.  Work-group
+// blk runs on XCD blk % 8, and each XCD takes a contiguous range of tiles (the grid is a multiple of
+// 8), so a tile's neighbour rows, mostly in nearby tiles, come from that XCD's L2.  The F <= FM
+// neighbour indices are loaded together, then every neighbour row (an absent one adds 0 in its
+// place in the sum): two memory round trips instead of two per neighbour.  DEC: the last layer --
+// instead of H_out, the decoder y = H_out w_out + b_out.
+template <bool DEC, int FM>
 __global__ __launch_bounds__(256) void k_gnn_infer_layer(int N, int F, const int* __restrict__ nbr,
                                                         const float* __restrict__ H, const bf16x8* __restrict__ wf,
                                                         const float* __restrict__ b, float* __restrict__ Hn,
@@ -331,7 +337,9 @@ __global__ __launch_bounds__(256) void k_gnn_infer_layer(int N, int F, const int
         bias[t] = b[16 * t + r];
         wout[t] = DEC ? wo[16 * t + r] : 0.f;
     }
-    for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntile; tile += gridDim.x * 4) {
+    const int per = gridDim.x / 8;  // work-groups per XCD
+    const int lblk = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    for (int tile = lblk * 4 + (threadIdx.x >> 6); tile < ntile; tile += gridDim.x * 4) {
         // A operands: row tile*16 + r (clamped: rows past N compute discarded outputs), columns
         // 32 kb + 8 q + j of H and of the neighbour mean
         const int rr = min(tile * 16 + r, N - 1);
@@ -348,15 +356,26 @@ __global__ __launch_bounds__(256) void k_gnn_infer_layer(int N, int F, const int
 #pragma unroll
             for (int j = 0; j < 8; j++) mv[kb][j] = 0.f;
         }
+        int nbf[FM];
+#pragma unroll
+        for (int f = 0; f < FM; f++) nbf[f] = f < F ? nbr[(size_t)rr * F + f] : -1;
+        float4 u[FM][2][2];
+#pragma unroll
+        for (int f = 0; f < FM; f++)
+#pragma unroll
+            for (int kb = 0; kb < 2; kb++) {
+                const float4* src = (const float4*)(H + (size_t)max(nbf[f], 0) * kGnnHid + 32 * kb + 8 * q);
+                u[f][kb][0] = src[0];
+                u[f][kb][1] = src[1];
+            }
         int d = 0;
-        for (int f = 0; f < F; f++) {
-            const int nb = nbr[(size_t)rr * F + f];
-            if (nb < 0) continue;
+#pragma unroll
+        for (int f = 0; f < FM; f++) {
+            if (nbf[f] < 0) continue;
             d++;
 #pragma unroll
             for (int kb = 0; kb < 2; kb++) {
-                const float4* src = (const float4*)(H + (size_t)nb * kGnnHid + 32 * kb + 8 * q);
-                const float4 u0 = src[0], u1 = src[1];
+                const float4 u0 = u[f][kb][0], u1 = u[f][kb][1];
                 mv[kb][0] += u0.x, mv[kb][1] += u0.y, mv[kb][2] += u0.z, mv[kb][3] += u0.w;
                 mv[kb][4] += u1.x, mv[kb][5] += u1.y, mv[kb][6] += u1.z, mv[kb][7] += u1.w;
             }
@@ -482,19 +501,31 @@ int gnn_infer_device(GnnNet* net, hipStream_t stream) {
     hipLaunchKernelGGL(k_gnn_enc, dim3((N + 63) / 64), dim3(256), 0, st, N, net->X.as<float>(), p,
                        p + kGnnIn * kGnnHid, (float*)nullptr, net->H[0].as<float>());
     const int ntile = (N + 15) / 16;
-    const dim3 grid(std::max(1, std::min((ntile + 3) / 4, 65536)));  // a tile per wave
+    const dim3 grid(std::min(((ntile + 3) / 4 + 7) / 8 * 8, 65536));  // a tile per wave, a multiple of 8
     const float* wo = p + gnn_off_out(L);
     for (int l = 0; l < L; l++) {
         const float* H = net->H[l & 1].as<float>();
         float* Hn = net->H[(l + 1) & 1].as<float>();
         const bf16x8* wf = net->wfrag.as<bf16x8>() + per_layer * l;
         const float* b = p + gnn_off_layer(l) + 2 * kGnnHid * kGnnHid;
+        // the faces per cell of 1D, 2D and 3D meshes (2, 3, 4) unrolled exactly, others (up to 8) as 8
+        auto go = [&](auto dec) {
+            constexpr bool D = decltype(dec)::value;
+            auto launch = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, grid, dim3(256), 0, st, N, net->F, net->nbr.as<int>(), H, wf, b, Hn, wo,
+                                   wo + kGnnHid, net->y.as<float>());
+            };
+            switch (net->F) {
+                case 2: launch(k_gnn_infer_layer<D, 2>); break;
+                case 3: launch(k_gnn_infer_layer<D, 3>); break;
+                case 4: launch(k_gnn_infer_layer<D, 4>); break;
+                default: launch(k_gnn_infer_layer<D, 8>); break;
+            }
+        };
         if (l + 1 < L)
-            hipLaunchKernelGGL(k_gnn_infer_layer<false>, grid, dim3(256), 0, st, N, net->F, net->nbr.as<int>(), H, wf,
-                               b, Hn, wo, wo + kGnnHid, net->y.as<float>());
+            go(std::false_type{});
         else
-            hipLaunchKernelGGL(k_gnn_infer_layer<true>, grid, dim3(256), 0, st, N, net->F, net->nbr.as<int>(), H, wf,
-                               b, Hn, wo, wo + kGnnHid, net->y.as<float>());
+            go(std::true_type{});
     }
     return hipGetLastError() == hipSuccess ? 0 : BTE_ERR_HIP;
 }
