@@ -1024,12 +1024,13 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p, size_
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(unsigned)bytes, 0x00020000);
 }
 
-template <typename T>
+// AUX: the cache-policy bits of the load (2: non-temporal on gfx950, for data read once).
+template <typename T, int AUX = 0>
 __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     if constexpr (sizeof(T) == 8)
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
     else
-        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
 }
 
 template <typename T>
@@ -1102,7 +1103,7 @@ __device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it
         it.dgc = d1[1];
         it.vw = buf_ld<double>(src.svw, j * 8u, 0);
     }
-    if constexpr (RHS) it.rhs = buf_ld<double>(src.rhs, j * 8u, 0);
+    if constexpr (RHS) it.rhs = buf_ld<double, 2>(src.rhs, j * 8u, 0);
 }
 
 // Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order over the W slots (the
