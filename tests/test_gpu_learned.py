@@ -120,6 +120,29 @@ def test_null_correction_changes_nothing(case3k):
     np.testing.assert_array_equal(r["T"], base["T"])
 
 
+def test_proposals_on_a_fixed_point_are_not_made(tmp_path):
+    """Every wall at T_ref: the initial state is the exact fixed point, every residual is 0, so each
+    proposal (launched before its norms are known) finds no residual to correct, applies nothing and
+    counts as not made; the run equals Anderson mixing alone bit for bit."""
+    say("learned: proposals at a fixed point")
+    p = meshgen.make_square_case(str(tmp_path), 2000, 12, 5, 12, L=L_ND)
+    bc = str(tmp_path / "bc_equal.dat")
+    meshgen.write_bcs(bc, [(0, 1, 300.0), (1, 1, 300.0), (2, 1, 300.0), (3, 1, 300.0)])
+    c = capi.Case(p["mesh"], p["band"], bc, 2, 8, 8, 300.0, Lx=L_ND, Ly=L_ND)
+    g = Gnn(layers=4, seed=0, device=0)
+    g.set_params(np.load(DEFAULT_WEIGHTS))
+    c.attach_gnn(g)
+    opts = dict(accel=capi.ACCEL_ANDERSON, stop_margin=-1.0, max_iter=6)
+    base = c.solve(**opts)
+    r = c.solve(learned=1, learned_every=1, **opts)
+    c.attach_gnn(None)
+    c.close()
+    assert r["learned_proposed"] == 0 and r["learned_accepted"] == 0 and r["iters"] == base["iters"] == 6
+    assert np.all(r["T"] == 300.0)
+    np.testing.assert_array_equal(r["T"], base["T"])
+    np.testing.assert_array_equal(r["margins"], base["margins"])
+
+
 def test_learned_needs_an_attached_network(case3k):
     c = case3k
     c.attach_gnn(None)
