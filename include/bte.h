@@ -82,6 +82,12 @@ typedef struct bte_opts {
                              work-group), 2 or 4; 0 autotuned.  Other forms use 1; default 0 */
     double inner_tol32;   /* BTE_PREC_MIXED: relative residual each FP32 inner solve aims for before
                              the FP64 refinement step (near FP32 rounding it can stall); 1e-5 */
+    int intensity_layout; /* device layout of the intensities: 0 automatic (3D pipelined sweeps with
+                             4 or more bands interleave them), 1 band-major ee[b][d][c] (the
+                             reference's), 2 direction-major with each cell's bands interleaved,
+                             ee[d][c][b], when the pipelined sweep (sweep_variant 4) runs, else
+                             band-major.  Results do not depend on it except the margins' summation
+                             order; bte_get_intensity returns band-major either way; default 0 */
 } bte_opts;
 
 enum bte_accel { BTE_ACCEL_NONE = 0, BTE_ACCEL_ANDERSON = 1 };
@@ -120,6 +126,7 @@ typedef struct bte_stats {
     int block;            /* work-group size used */
     int variant;          /* direct-sweep variant used (1..7), 0 for the Krylov paths */
     int band_group;       /* Krylov paths: bands per work-group used (opts.band_group); 0 direct */
+    int intensity_layout; /* intensity layout of the last solve: 1 band-major, 2 bands interleaved */
 } bte_stats;
 
 void bte_default_opts(bte_opts* opts);

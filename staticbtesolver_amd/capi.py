@@ -46,7 +46,7 @@ class Opts(C.Structure):
                 ("timing", C.c_int), ("learned", C.c_int), ("learned_every", C.c_int),
                 ("direct_sweep", C.c_int), ("sweep_variant", C.c_int), ("accel", C.c_int),
                 ("accel_memory", C.c_int), ("band_group", C.c_int),
-                ("inner_tol32", C.c_double)]
+                ("inner_tol32", C.c_double), ("intensity_layout", C.c_int)]
 
 
 class Result(C.Structure):
@@ -62,7 +62,7 @@ class Stats(C.Structure):
     _fields_ = [("n_kernels", C.c_int), ("names", C.c_char_p * 16), ("ms", C.c_double * 16),
                 ("calls", C.c_longlong * 16), ("spmv_bytes", C.c_double), ("spmv_rows", C.c_longlong),
                 ("ell_width", C.c_int), ("max_levels", C.c_int), ("lagged_edges", C.c_int), ("block", C.c_int),
-                ("variant", C.c_int), ("band_group", C.c_int)]
+                ("variant", C.c_int), ("band_group", C.c_int), ("intensity_layout", C.c_int)]
 
 
 # symbols declared in include/bte.h and include/bte_case.h
@@ -325,7 +325,7 @@ class Case:
         return dict(ms={n: s.ms[i] for i, n in enumerate(names)}, calls={n: s.calls[i] for i, n in enumerate(names)},
                     spmv_rows=s.spmv_rows, ell_width=s.ell_width, max_levels=s.max_levels,
                     lagged_edges=s.lagged_edges, block=s.block, variant=s.variant,
-                    band_group=s.band_group)
+                    band_group=s.band_group, intensity_layout=s.intensity_layout)
 
 
 def ctx_set_bands(ctx, v, tau, Cv, Lr):

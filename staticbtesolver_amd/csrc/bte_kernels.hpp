@@ -7,8 +7,11 @@
 //                ell_col/ell_val[(d*W+k)*N + c]  incoming interior faces, k < W = F-1 slots
 //   sweep        per direction, cells in downwind level order:
 //                lvl_ptr[lvl_off[d] .. + L_d], perm[d*N+i], swp_col/swp_val[(d*W+k)*N+i], swp_dg[d*N+i]
-//   intensities  ee[b][d][c] = ee[(b*D+d)*N + c]  (the reference's ee_curr[b]->data layout)
-//   Krylov work  one slice of N per system and vector, same system order as ee
+//   intensities  ee at b*sb + d*sd + c*sc (EeIdx): band-major ee[b][d][c] (the reference's
+//                ee_curr[b]->data layout), or direction-major with a cell's bands interleaved,
+//                ee[d][c][b] (the pipelined sweep in 3D: the co-scheduled bands of a direction then
+//                write each cell's intensities together)
+//   Krylov work  one slice of N per system and vector, in band-major system order
 //
 // Matrix of system (b, d), StaticBTESolver.cpp:122-160:
 //   diag(c) = v_b tau_b dg[d][c] + w_d V_c        (w_d = omega_d in 2D/3D, 1 in 1D)
@@ -41,11 +44,29 @@ struct Geo {
     const int* nbr;
 };
 
+// Index of intensity (band b, direction d, cell c) in an intensity array: b sb + d sd + c sc.
+// Band-major: sb = D N, sd = N, sc = 1.  Direction-major, bands interleaved: sb = 1, sd = N Bp,
+// sc = Bp, with Bp = B rounded up to a multiple of 4, so that a cell's bands fill whole 32-byte
+// sectors.
+struct EeIdx {
+    long long sb, sd, sc;
+    __host__ __device__ size_t at(int b, int d, int c) const {
+        return (size_t)(b * sb + d * sd + (long long)c * sc);
+    }
+};
+
+__host__ __device__ inline EeIdx ee_band_major(int D, int N) { return EeIdx{(long long)D * N, N, 1}; }
+__host__ __device__ inline int ee_padded_bands(int B) { return (B + 3) & ~3; }
+__host__ __device__ inline EeIdx ee_interleaved(int B, int N) {
+    return EeIdx{1, (long long)N * ee_padded_bands(B), ee_padded_bands(B)};
+}
+
 struct Quad {
     int D;
     const double* omega;
     const double* dir;
     const double* S;
+    EeIdx ee;  // layout of the intensity arrays the kernels read
 };
 
 // ----------------------------------------------------------------------------------------------
@@ -220,7 +241,7 @@ __device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const
             double ev[K];
 #pragma unroll
             for (int k = 0; k < K; k++)
-                ev[k] = d0 + k < q.D ? ee_prev[((size_t)b * q.D + d0 + k) * g.N + c] : 0.0;
+                ev[k] = d0 + k < q.D ? ee_prev[q.ee.at(b, d0 + k, c)] : 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int d2 = d0 + k;
@@ -274,7 +295,7 @@ __device__ __forceinline__ double rhs_at(const Geo& g, const Quad& q, const Bnd&
             } else {
                 const int slot = type - 31;  // 31, 32, 33 validated on the host
                 const int m = bd.mirror[slot * q.D + d];
-                re -= ee_prev[((size_t)b * q.D + m) * g.N + c] * a;
+                re -= ee_prev[q.ee.at(b, m, c)] * a;
             }
         }
     }
@@ -370,6 +391,9 @@ struct Ops {
     // packed level-order records of the pipelined sweeps (k_pack_sweep), 4 entries per (d, i)
     const int* swp_pki;
     const double* swp_pkd;
+    // layout of the intensity arrays (w.x): band-major for every solver form; the pipelined sweep
+    // (variant 4, sweep_ring_system) also writes the interleaved one
+    EeIdx ee;
 };
 
 struct Work {
@@ -1051,7 +1075,8 @@ struct PkItem {
 
 struct PkSrc {
     __amdgpu_buffer_rsrc_t pki, pkd, scol, svw, rhs, x;
-    unsigned n4;  // byte distance between the upwind-cell slots of swp_col: N * 4
+    unsigned n4;   // byte distance between the upwind-cell slots of swp_col: N * 4
+    unsigned xs8;  // byte distance between the intensities of neighbouring cells in x (o.ee.sc * 8)
 };
 
 template <int W>
@@ -1063,8 +1088,10 @@ __device__ __forceinline__ PkSrc pk_src(const Ops& o, const Work& w, int s, int 
     r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
     r.svw = buffer_of(o.swp_vw + d * N, N * 8);
     r.rhs = buffer_of(w.b + (size_t)s * o.Np, N * 8);
-    r.x = buffer_of(w.x + (size_t)s * N, N * 8);
+    const long long sc = o.ee.sc;
+    r.x = buffer_of(w.x + o.ee.at(s / o.D, d, 0), (size_t)((N - 1) * sc + 1) * 8);
     r.n4 = (unsigned)(N * 4);
+    r.xs8 = (unsigned)(sc * 8);
     return r;
 }
 
@@ -1131,7 +1158,7 @@ __device__ __forceinline__ double pk_upwind_sum(const PkItem& it, int lim8, cons
             for (int k = 0; k < W; k++)
                 if (it.p[k] < lim8) {
                     const int c = buf_ld<int>(src.scol, (unsigned)it.i * 4u, k * src.n4);
-                    const double h = xg[c];
+                    const double h = xg[(size_t)c * (src.xs8 >> 3)];
                     asm volatile("v_mov_b64 %0, %1" : "=v"(v[k]) : "v"(h));
                 }
         }
@@ -1170,7 +1197,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
     {
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
-        gmem<double>* __restrict__ xg = to_global(w.x + (size_t)s * N);
+        gmem<double>* __restrict__ xg = to_global(w.x + o.ee.at(b, d, 0));
         const PkSrc src = pk_src<W>(o, w, s, d);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
         const int L = o.lvl_cnt[d];
@@ -1215,7 +1242,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
                 pk_upwind_sum<W, decltype(hb)::value>(it, (end - R) * 8, ring, ring_mask * 8, xg, src);
             const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
-            buf_st<double>(xv, src.x, (unsigned)it.c * 8u, 0);
+            buf_st<double>(xv, src.x, (unsigned)it.c * src.xs8, 0);
         };
         auto solve_level = [&](const PkItem& cur, int beg, int end, auto hb) {
             if (cur.i >= 0) solve_cell(cur, end, hb);
@@ -1638,37 +1665,40 @@ __global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg c
 // threads than a cell loop, so enough loads are in flight to stream both intensity arrays at HBM
 // rate -- fused with the margin partial sums (:311-321).  Work-group blk of nblk strides over the
 // B*N pairs (grids are capped, B*N is not) and writes partial[blk]; each thread adds its pairs in
-// increasing order, so the sums are the same on every call with the same grid.
-template <int BLOCK>
-__device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, const double* __restrict__ omega,
-                                            const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
-                                            double* __restrict__ Tband, double* __restrict__ partial, int blk,
-                                            int nblk) {
+// increasing order, so the sums are the same on every call with the same grid.  The pairs go in
+// (band, cell) order for the band-major intensity layout, in (cell, band) order (CB) for the
+// interleaved one, so that neighbouring threads read neighbouring intensities either way; a
+// band temperature sums its directions in the same order in both.
+template <int BLOCK, bool CB = false>
+__device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, const EeIdx& ei,
+                                            const double* __restrict__ omega, const double* __restrict__ ee_curr,
+                                            const double* __restrict__ ee_prev, double* __restrict__ Tband,
+                                            double* __restrict__ partial, int blk, int nblk) {
     __shared__ double lds[BLOCK / 64];
     double mp[1] = {0};
     const long long total = (long long)bands.B * N;
     for (long long t = blk * (long long)BLOCK + threadIdx.x; t < total; t += (long long)nblk * BLOCK) {
-        const int b = (int)(t / N), c = (int)(t % N);
-        const size_t base = (size_t)b * D * N + c;
+        const int b = CB ? (int)(t % bands.B) : (int)(t / N), c = CB ? (int)(t / bands.B) : (int)(t % N);
+        const size_t base = ei.at(b, 0, c);
         double e0 = 0;
 #pragma unroll 8
         for (int d = 0; d < D; d++) {
-            const double ec = ee_curr[base + (size_t)d * N];
-            const double df = ec - ee_prev[base + (size_t)d * N];
+            const double ec = ee_curr[base + (size_t)d * ei.sd];
+            const double df = ec - ee_prev[base + (size_t)d * ei.sd];
             e0 += ec * omega[d];
             mp[0] += df * df;
         }
-        Tband[t] = e0 / bands.C[b] + bands.T_ref;
+        Tband[(size_t)b * N + c] = e0 / bands.C[b] + bands.T_ref;
     }
     block_sum<BLOCK, 1>(mp, lds);
     if (threadIdx.x == 0) partial[blk] = mp[0];
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, const double* __restrict__ omega,
+template <int BLOCK, bool CB = false>
+__global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, EeIdx ei, const double* __restrict__ omega,
                                                 const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
                                                 double* __restrict__ Tband, double* __restrict__ partial) {
-    tband_block<BLOCK>(N, D, bands, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x, gridDim.x);
+    tband_block<BLOCK, CB>(N, D, bands, ei, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x, gridDim.x);
 }
 
 // Equilibrium temperature and emission density per cell (StaticBTESolver.cpp:106-120).
@@ -1776,7 +1806,7 @@ __global__ void k_signal(long long seq, long long* __restrict__ host) {
 // state entries.  take: then x = g (a plain
 // step), and with dens the state's loads for the next iteration from it, as k_set_density and
 // k_state_scatter form them (the emission density of the T part, the boundary intensities).
-__global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __restrict__ bcells,
+__global__ void k_state_gather_diff(int N, int D, int S, int nb, EeIdx ei, const int* __restrict__ bcells,
                                     const double* __restrict__ sc, const double* __restrict__ T,
                                     double* __restrict__ ee, double* __restrict__ z, int first,
                                     double* __restrict__ x, double* __restrict__ x_prev,
@@ -1792,7 +1822,7 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
         } else {
             const long long u = t - N;
             const int s = (int)(u / nb), j = (int)(u % nb);
-            gv = ee[(size_t)s * N + bcells[j]] * sc[s / D];
+            gv = ee[ei.at(s / D, s % D, bcells[j])] * sc[s / D];
         }
         z[t] = gv;
         const double xv = x[t];
@@ -1815,20 +1845,20 @@ __global__ void k_state_gather_diff(int N, int D, int S, int nb, const int* __re
                 } else {
                     const long long u = t - N;
                     const int s = (int)(u / nb), j = (int)(u % nb);
-                    ee[(size_t)s * N + bcells[j]] = gv / sc[s / D];
+                    ee[ei.at(s / D, s % D, bcells[j])] = gv / sc[s / D];
                 }
             }
         }
     }
 }
 
-__global__ void k_state_scatter(int N, int D, int S, int nb, const int* __restrict__ bcells,
+__global__ void k_state_scatter(int N, int D, int S, int nb, EeIdx ei, const int* __restrict__ bcells,
                                 const double* __restrict__ sc, const double* __restrict__ z, double* __restrict__ ee) {
     const long long total = (long long)S * nb;
     for (long long u = blockIdx.x * (long long)blockDim.x + threadIdx.x; u < total;
          u += (long long)gridDim.x * blockDim.x) {
         const int s = (int)(u / nb), j = (int)(u % nb);
-        ee[(size_t)s * N + bcells[j]] = z[N + u] / sc[s / D];
+        ee[ei.at(s / D, s % D, bcells[j])] = z[N + u] / sc[s / D];
     }
 }
 
@@ -1875,7 +1905,7 @@ __global__ void k_learn_features(int N, const double* __restrict__ f, const doub
 // z as k_set_density and k_state_scatter form them.  ff: device address of the proposal's residual
 // norms {over T, over the whole state}; a state with no residual (or a non-finite one) is only
 // backed up.
-__global__ void k_apply_correction(int N, int D, int S, int nb, const int* __restrict__ bcells,
+__global__ void k_apply_correction(int N, int D, int S, int nb, EeIdx ei, const int* __restrict__ bcells,
                                    const float* __restrict__ y, double span, const double* __restrict__ ff,
                                    double* __restrict__ z, double* __restrict__ backup, Bands bands,
                                    double* __restrict__ dens, const double* __restrict__ sc, double* __restrict__ ee) {
@@ -1895,7 +1925,7 @@ __global__ void k_apply_correction(int N, int D, int S, int nb, const int* __res
         } else {
             const long long u = t - N;
             const int s = (int)(u / nb), j = (int)(u % nb);
-            ee[(size_t)s * N + bcells[j]] = v / sc[s / D];
+            ee[ei.at(s / D, s % D, bcells[j])] = v / sc[s / D];
         }
     }
 }
@@ -1946,7 +1976,8 @@ __global__ __launch_bounds__(BLOCK) void k_sqdiff2(long long n1, const double* _
 // state entries of (g - x) with g = G(x~) gathered as k_state_gather_diff does (not stored), and over
 // [0, N) of (x - backup).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_judge_norms(int N, int D, int S, int nb, const int* __restrict__ bcells,
+__global__ __launch_bounds__(BLOCK) void k_judge_norms(int N, int D, int S, int nb, EeIdx ei,
+                                                      const int* __restrict__ bcells,
                                                       const double* __restrict__ sc, const double* __restrict__ T,
                                                       const double* __restrict__ ee, const double* __restrict__ x,
                                                       const double* __restrict__ backup, double* __restrict__ partial) {
@@ -1960,7 +1991,7 @@ __global__ __launch_bounds__(BLOCK) void k_judge_norms(int N, int D, int S, int 
         } else {
             const long long u = i - N;
             const int s = (int)(u / nb), j = (int)(u % nb);
-            gv = ee[(size_t)s * N + bcells[j]] * sc[s / D];
+            gv = ee[ei.at(s / D, s % D, bcells[j])] * sc[s / D];
         }
         const double d = gv - x[i];
         acc[0] += d * d;
@@ -1991,7 +2022,7 @@ struct AaCoef {
 // columns dG_j = dX_j + dF_j are stored as such), and in the same pass the state's loads for the
 // next iteration from the new x: the emission density of its temperature part (as k_set_density)
 // and its boundary intensities (as k_state_scatter).
-__global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, AaCoef gamma,
+__global__ void k_aa_update_state(int N, int D, int S, int nb, EeIdx ei, int ncols, AaCoef gamma,
                                   const double* __restrict__ g, const float* __restrict__ dG,
                                   double* __restrict__ x, Bands bands,
                                   double* __restrict__ dens, const int* __restrict__ bcells,
@@ -2009,7 +2040,7 @@ __global__ void k_aa_update_state(int N, int D, int S, int nb, int ncols, AaCoef
         } else {
             const long long u = c - N;
             const int s = (int)(u / nb), j = (int)(u % nb);
-            ee[(size_t)s * N + bcells[j]] = v / sc[s / D];
+            ee[ei.at(s / D, s % D, bcells[j])] = v / sc[s / D];
         }
     }
 }
@@ -2119,7 +2150,7 @@ __global__ __launch_bounds__(256) void k_tband_multi(const IterDesc* __restrict_
                                                      int n) {
     const MultiIdx m = multi_index(off, n);
     const IterDesc& D = desc[m.p];
-    tband_block<256>(D.o.N, D.o.D, D.bands, D.omega, D.ee_curr, D.ee_prev, D.Tband, D.partial, m.blk, m.nblk);
+    tband_block<256>(D.o.N, D.o.D, D.bands, D.q.ee, D.omega, D.ee_curr, D.ee_prev, D.Tband, D.partial, m.blk, m.nblk);
 }
 
 __global__ void k_tcell_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
@@ -2148,7 +2179,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Quad q, Bnd bd, Bands ban
         const double A = g.area[(size_t)c * g.F + f];
         double heat = 0;
         for (int d = 0; d < q.D; d++) {
-            const double e = ee[((size_t)b * q.D + d) * g.N + c];
+            const double e = ee[q.ee.at(b, d, c)];
             if (DM == 3 && g.dim == 3)
                 heat += e * dot_nc(q.dir + 3 * d, n) * A * q.omega[d];
             else if (g.dim == 1)

@@ -109,6 +109,10 @@ struct bte_ctx {
     DBuf eeA, eeB, dens, T, Tband, partial, margin, bflux;
     DBuf wb, wr64, wr, wrh, wp, wv, wph, wsh, wt, wy, wf, iters, fails;
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
+    // layout of the intensity arrays (band-major, or interleaved for a pipelined 3D sweep: set by
+    // bte_begin, band-major whenever work buffers are (re)sized)
+    EeIdx eei{};
+    bool ee_inter = false;
     double* h_margin_pinned = nullptr;
     // coherent pinned host memory that kernels store into directly (no copy, read after a
     // synchronisation): [0] the margin of the last outer iteration, [64, 64 + 2 m + 1) the dot
@@ -130,6 +134,7 @@ struct bte_ctx {
     bool graph_valid = false;
     bte_opts graph_opts{};
     int graph_block = 0, graph_variant = 0;
+    bool graph_inter = false;
     // stats
     double ms[KA_COUNT] = {0};
     long long calls[KA_COUNT] = {0};
@@ -209,7 +214,16 @@ Geo geo_of(bte_ctx* c) {
     return Geo{c->dim, c->N, c->F, c->vol.as<double>(), c->area.as<double>(), c->normal.as<double>(),
                c->nbr.as<int>()};
 }
-Quad quad_of(bte_ctx* c) { return Quad{c->D, c->omega.as<double>(), c->dir.as<double>(), c->S.as<double>()}; }
+Quad quad_of(bte_ctx* c) {
+    return Quad{c->D, c->omega.as<double>(), c->dir.as<double>(), c->S.as<double>(), c->eei};
+}
+
+// Intensity layout of the next launches: band-major, or direction-major with each cell's bands
+// interleaved (EeIdx).
+void set_layout(bte_ctx* c, bool interleaved) {
+    c->ee_inter = interleaved;
+    c->eei = interleaved ? ee_interleaved(c->B, c->N) : ee_band_major(c->D, c->N);
+}
 Bands bands_of(bte_ctx* c) {
     return Bands{c->B, c->bv.as<double>(), c->btau.as<double>(), c->bC.as<double>(), c->bLr.as<double>(),
                  c->solid_angle, c->T_ref};
@@ -389,6 +403,7 @@ void bte_default_opts(bte_opts* o) {
     o->accel_memory = 20;
     o->band_group = 0;
     o->inner_tol32 = 1e-5;
+    o->intensity_layout = 0;
 }
 
 int bte_ctx_create(int device, bte_ctx** out) {
@@ -901,11 +916,14 @@ int ensure_fp32(bte_ctx* ctx) {
 int ensure_work(bte_ctx* ctx, int precision) {
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N, BN = (size_t)ctx->B * ctx->N;
     const int S = ctx->B * ctx->D;
+    // intensities: room for either layout (the interleaved one pads the bands to a multiple of 4);
     // Krylov vectors are FP64-sized in both precisions: the mixed form re-solves in FP64 a system
     // whose FP32 inner solve stalls; the right-hand-side buffer also holds the level-order rows of
     // stride lo_stride(N)
+    const size_t EE = std::max(SN, (size_t)ctx->D * ctx->N * ee_padded_bands(ctx->B));
+    set_layout(ctx, false);
     std::vector<std::pair<DBuf*, size_t>> want = {
-        {&ctx->eeA, 8 * SN}, {&ctx->eeB, 8 * SN}, {&ctx->wb, 8 * (size_t)S * lo_stride(ctx->N)},
+        {&ctx->eeA, 8 * EE}, {&ctx->eeB, 8 * EE}, {&ctx->wb, 8 * (size_t)S * lo_stride(ctx->N)},
         {&ctx->wr, 8 * SN}, {&ctx->wrh, 8 * SN},
         {&ctx->wp, 8 * SN}, {&ctx->wv, 8 * SN}, {&ctx->wph, 8 * SN}, {&ctx->wsh, 8 * SN}, {&ctx->wt, 8 * SN},
         {&ctx->iters, sizeof(int) * S}, {&ctx->fails, sizeof(int) * S}, {&ctx->dens, 8 * BN},
@@ -964,6 +982,7 @@ Ops ops_of(bte_ctx* ctx) {
     o.swp_pkd = ctx->swp_pkd.as<double>();
     o.ring_sync = 1;
     (void)ring_size(ctx, &o.ring_sync);
+    o.ee = ctx->eei;
     return o;
 }
 
@@ -1240,8 +1259,14 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
     const int nblk = grid_for((long long)ctx->B * ctx->N, 256);
     {
         Timer tm(ctx, KA_TEMP, timing);
-        hipLaunchKernelGGL(k_tband<256>, dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
-                           ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(), partial_of(ctx));
+        if (ctx->ee_inter)
+            hipLaunchKernelGGL((k_tband<256, true>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
+                               ctx->eei, ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(),
+                               partial_of(ctx));
+        else
+            hipLaunchKernelGGL((k_tband<256, false>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
+                               ctx->eei, ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(),
+                               partial_of(ctx));
         // cell temperatures and densities, and the margin, in one launch (both read the band step)
         const int n_tcell = grid_for(ctx->N, 256);
         hipLaunchKernelGGL(k_tcell_margin, dim3(n_tcell + 1), dim3(256), 0, ctx->stream, ctx->N, bn,
@@ -1504,7 +1529,7 @@ static int load_state(bte_ctx* ctx) {
                        ctx->aa_x.as<double>(), ctx->dens.as<double>());
     if (nb > 0)
         hipLaunchKernelGGL(k_state_scatter, dim3(grid_for((long long)S * nb, 256)), dim3(256), 0, ctx->stream, N,
-                           ctx->D, S, nb, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->aa_x.as<double>(),
+                           ctx->D, S, nb, ctx->eei, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->aa_x.as<double>(),
                            ctx->ee_final);
     HIPCHK(hipGetLastError());
     return BTE_OK;
@@ -1526,7 +1551,7 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     // latest dX)
     // take: a plain step x = g in the same pass (load: with the state's loads for the next iteration)
     auto gather_diff = [&](int first, float* dF, float* dG, int take = 0, bool load = false) {
-        hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb,
+        hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb, ctx->eei,
                            ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
                            ctx->aa_g.as<double>(), first, ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(),
                            ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), dF, dG, take,
@@ -1607,7 +1632,7 @@ static int aa_finish(bte_ctx* ctx) {
     }
     // the update and the state's density and boundary intensities in one pass (load_state's work)
     hipLaunchKernelGGL(k_aa_update_state, dim3(gb), dim3(256), 0, ctx->stream, ctx->N, ctx->D, ctx->B * ctx->D,
-                       ctx->aa_nb, nc, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<float>(), ctx->aa_x.as<double>(),
+                       ctx->aa_nb, ctx->eei, nc, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<float>(), ctx->aa_x.as<double>(),
                        bands_of(ctx), ctx->dens.as<double>(),
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
@@ -1818,7 +1843,7 @@ static int learn_propose(bte_ctx* ctx) {
     HIPCHK(ctx->learn_backup.ensure(sizeof(double) * Z));
     // backup, correction and the corrected state's loads (load_state's work) in one pass
     hipLaunchKernelGGL(k_apply_correction, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->D,
-                       ctx->B * ctx->D, ctx->aa_nb, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span,
+                       ctx->B * ctx->D, ctx->aa_nb, ctx->eei, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span,
                        red_out_of(ctx), ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), bands_of(ctx),
                        ctx->dens.as<double>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
@@ -1836,7 +1861,7 @@ static int launch_judge(bte_ctx* ctx) {
         HIPCHK(ctx->red_part.ensure(sizeof(double) * 2 * nblk));
         HIPCHK(ctx->red_out.ensure(sizeof(double) * 4));
     }
-    hipLaunchKernelGGL(k_judge_norms<1024>, dim3(nblk), dim3(1024), 0, ctx->stream, N, ctx->D, S, nb,
+    hipLaunchKernelGGL(k_judge_norms<1024>, dim3(nblk), dim3(1024), 0, ctx->stream, N, ctx->D, S, nb, ctx->eei,
                        ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
                        ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), red_part_of(ctx));
     hipLaunchKernelGGL(k_reduce_rows, dim3(2), dim3(256), 0, ctx->stream, 2, nblk, red_part_of(ctx),
@@ -1883,6 +1908,7 @@ static int check_opts(bte_ctx* ctx, const bte_opts& o) {
     if (o.precond != BTE_PC_SWEEP && o.precond != BTE_PC_JACOBI) SETERR(BTE_ERR_ARG, "unknown preconditioner");
     if (o.inner_maxit <= 0 || o.inner_tol <= 0) SETERR(BTE_ERR_ARG, "inner_maxit and inner_tol must be positive");
     if (o.accel != BTE_ACCEL_NONE && o.accel != BTE_ACCEL_ANDERSON) SETERR(BTE_ERR_ARG, "unknown outer acceleration");
+    if (o.intensity_layout < 0 || o.intensity_layout > 2) SETERR(BTE_ERR_ARG, "unknown intensity layout");
     if (o.learned) {
         if (o.learned_every < 0) SETERR(BTE_ERR_ARG, "learned_every must not be negative");
         if (o.learned_every > 0 && !ctx->gnn)
@@ -1901,7 +1927,7 @@ static bool same_capture(const bte_ctx* ctx, const bte_opts& o, const Shape& sh)
            ctx->graph_variant == sh.variant && g.precision == o.precision && g.precond == o.precond &&
            (g.direct_sweep != 0) == (o.direct_sweep != 0) && g.inner_maxit == o.inner_maxit &&
            g.inner_tol == o.inner_tol && std::max(1, g.refine_steps) == std::max(1, o.refine_steps) &&
-           g.inner_tol32 == o.inner_tol32;
+           g.inner_tol32 == o.inner_tol32 && ctx->graph_inter == ctx->ee_inter;
 }
 
 int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
@@ -1938,9 +1964,14 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     ctx->group_used = direct_path(ctx, &o) ? 0
                       : group_ok(ctx, &o, shape.variant, shape.block) ? shape.variant
                                                                        : 1;
+    // intensity layout: each cell's bands interleaved for the pipelined sweep when asked, or by
+    // default in 3D with 4 or more bands (the co-scheduled bands of a direction then write a
+    // cell's intensities together, into whole sectors); band-major for every other form
+    set_layout(ctx, direct_path(ctx, &o) && ctx->variant_used == 4 && ctx->B >= 2 &&
+                        (o.intensity_layout == 2 || (o.intensity_layout == 0 && ctx->dim == 3 && ctx->B >= 4)));
     // the zero initial intensities (the first iteration's ee_prev); the buffer it writes is cleared
     // only for the Krylov forms, whose initial guess it is (a sweep writes every cell)
-    HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, ctx->eeA.bytes, ctx->stream));
     if (!direct_path(ctx, &o)) HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->T.p, 0, sizeof(double) * N, ctx->stream));
@@ -1962,6 +1993,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
             HIPCHK(hipGraphDestroy(g));
         }
         ctx->graph_valid = true;
+        ctx->graph_inter = ctx->ee_inter;
         ctx->graph_opts = o;
         ctx->graph_block = shape.block;
         ctx->graph_variant = shape.variant;
@@ -2229,6 +2261,13 @@ int bte_finish(bte_ctx* ctx, bte_result* res) {
     if (nbc > 0 && it > 0)
         HIPCHK(hipMemcpyAsync(hbf.data(), bflux_of(ctx), sizeof(double) * nbc * B, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    // a proposal still awaiting its judgement when the solve stopped: with no residual to correct
+    // (its norms have arrived with the synchronisation) it was not made
+    if (ctx->learn_pending) {
+        ctx->learn_pending = false;
+        if (!(ctx->h_sync[kSyncLearn] > 0) || !std::isfinite(ctx->h_sync[kSyncLearn + 1])) ctx->run_proposed--;
+        res->learned_proposed = ctx->run_proposed;
+    }
     res->inner_iters = 0;
     res->inner_failures = 0;
     for (int s = 0; s < S; s++) {
@@ -2323,9 +2362,17 @@ int bte_get_intensity(bte_ctx* ctx, double* ee) {
     if (!ctx || !ee) return BTE_ERR_ARG;
     if (!ctx->ee_final) SETERR(BTE_ERR_STATE, "no solve has run");
     HIPCHK(hipSetDevice(ctx->device));
-    const size_t S = (size_t)ctx->B * ctx->D;
-    std::vector<double> h(S * ctx->N);
-    HIPCHK(hipMemcpy(h.data(), ctx->ee_final, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    const size_t S = (size_t)ctx->B * ctx->D, N = ctx->N;
+    std::vector<double> h(S * N);
+    if (ctx->ee_inter) {  // to band-major
+        std::vector<double> r((size_t)ctx->D * N * ee_padded_bands(ctx->B));
+        HIPCHK(hipMemcpy(r.data(), ctx->ee_final, sizeof(double) * r.size(), hipMemcpyDeviceToHost));
+        for (int b = 0; b < ctx->B; b++)
+            for (int d = 0; d < ctx->D; d++)
+                for (size_t c = 0; c < N; c++) h[((size_t)b * ctx->D + d) * N + c] = r[ctx->eei.at(b, d, (int)c)];
+    } else {
+        HIPCHK(hipMemcpy(h.data(), ctx->ee_final, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    }
     to_external(ctx, h.data(), ee, 1, S);
     return BTE_OK;
 }
@@ -2346,6 +2393,7 @@ int bte_get_stats(bte_ctx* ctx, bte_stats* st) {
     st->block = ctx->block_used;
     st->variant = ctx->variant_used;
     st->band_group = ctx->group_used;
+    st->intensity_layout = ctx->ee_inter ? 2 : 1;
     return BTE_OK;
 }
 
@@ -2693,6 +2741,7 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
     else
         bte_default_opts(&o);
     o.use_graph = 0;
+    o.intensity_layout = 1;  // the batch's sweep and per-problem kernels index band-major intensities
     b->timing = o.timing != 0;
     b->ms_rhs = b->ms_sweep = 0;
     b->n_rhs = b->n_sweep = 0;

@@ -121,14 +121,16 @@ def test_null_correction_changes_nothing(case3k):
 
 
 def test_proposals_on_a_fixed_point_are_not_made(tmp_path):
-    """Every wall at T_ref: the initial state is the exact fixed point, every residual is 0, so each
-    proposal (launched before its norms are known) finds no residual to correct, applies nothing and
-    counts as not made; the run equals Anderson mixing alone bit for bit."""
+    """Every wall at T_ref = 0 K: the initial state is the exact fixed point and every sum that forms
+    the next one is exactly 0 (at 300 K the band-weighted cell temperature would round away from
+    300 by an ulp), so each proposal (launched before its norms are known) finds no residual to
+    correct, applies nothing and counts as not made; the run equals Anderson mixing alone bit for
+    bit."""
     say("learned: proposals at a fixed point")
     p = meshgen.make_square_case(str(tmp_path), 2000, 12, 5, 12, L=L_ND)
     bc = str(tmp_path / "bc_equal.dat")
-    meshgen.write_bcs(bc, [(0, 1, 300.0), (1, 1, 300.0), (2, 1, 300.0), (3, 1, 300.0)])
-    c = capi.Case(p["mesh"], p["band"], bc, 2, 8, 8, 300.0, Lx=L_ND, Ly=L_ND)
+    meshgen.write_bcs(bc, [(0, 1, 0.0), (1, 1, 0.0), (2, 1, 0.0), (3, 1, 0.0)])
+    c = capi.Case(p["mesh"], p["band"], bc, 2, 8, 8, 0.0, Lx=L_ND, Ly=L_ND)
     g = Gnn(layers=4, seed=0, device=0)
     g.set_params(np.load(DEFAULT_WEIGHTS))
     c.attach_gnn(g)
@@ -138,7 +140,7 @@ def test_proposals_on_a_fixed_point_are_not_made(tmp_path):
     c.attach_gnn(None)
     c.close()
     assert r["learned_proposed"] == 0 and r["learned_accepted"] == 0 and r["iters"] == base["iters"] == 6
-    assert np.all(r["T"] == 300.0)
+    assert np.all(r["T"] == 0.0)
     np.testing.assert_array_equal(r["T"], base["T"])
     np.testing.assert_array_equal(r["margins"], base["margins"])
 
