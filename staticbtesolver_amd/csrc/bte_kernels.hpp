@@ -1733,53 +1733,73 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 // ----------------------------------------------------------------------------------------------
 
 // partial[q * nblk + blk] for q < ncols: dF_q . dF_new, q in [ncols, 2 ncols): dF_{q-ncols} . f,
-// q = 2 ncols: f . f
+// q = 2 ncols: f . f (nblk = kAaDotBlocks)
 // Grid (nblk, ceil(ncols / G) + 1): work-group (x, y < ceil(ncols / G)) reads columns
 // dF_{Gy .. Gy+G-1} and, once per entry for all of them, dF_new and f; (x, last) forms f . f.  Each
 // sum runs over the work-group's chunk in the same fixed order for every call.  (Measured on the
 // 50-column history of a 180k-entry state: G = 4 with 256-thread work-groups takes 15 us; wider
 // groups spend more on the work-group's cross-lane sums of 2 G values than they save in reads.)
+// Arguments of one step's dot products; out and host receive the reduced values (k_reduce_rows)
+struct AaDots {
+    int Z, ncols;
+    const float* dF;      // the history columns, Z apart
+    const float* dF_new;  // the newest column
+    const double* f;
+    double* partial;      // [2 ncols + 1][kAaDotBlocks]
+    double* out;
+    double* host;
+};
+constexpr int kAaDotBlocks = 64;
+
+// Work-group (bx, by) of the (kAaDotBlocks, ceil(ncols / G) + 1) grid.
 template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void k_aa_dots(int N, int ncols, const float* __restrict__ dF,
-                                                  const float* __restrict__ dF_new, const double* __restrict__ f,
-                                                  double* __restrict__ partial) {
+__device__ __forceinline__ void aa_dots_block(const AaDots& a, int bx, int by) {
     __shared__ double lds[2 * G * (BLOCK / 64)];
-    const int j0 = blockIdx.y * G, nblk = gridDim.x;
+    const int N = a.Z, ncols = a.ncols, j0 = by * G, nblk = kAaDotBlocks;
+    const float* __restrict__ dF = a.dF;
+    const float* __restrict__ dF_new = a.dF_new;
+    const double* __restrict__ f = a.f;
     double s[2 * G];
 #pragma unroll
     for (int k = 0; k < 2 * G; k++) s[k] = 0;
     if (j0 < ncols) {
-        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
-            const double a = dF_new[c], b = f[c];
+        for (int c = bx * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
+            const double av = dF_new[c], bv = f[c];
             double v[G];
 #pragma unroll
             for (int g = 0; g < G; g++) v[g] = j0 + g < ncols ? (double)dF[(size_t)(j0 + g) * N + c] : 0.0;
 #pragma unroll
             for (int g = 0; g < G; g++) {
-                s[g] += v[g] * a;
-                s[G + g] += v[g] * b;
+                s[g] += v[g] * av;
+                s[G + g] += v[g] * bv;
             }
         }
         block_sum<BLOCK, 2 * G>(s, lds);
         if (threadIdx.x == 0)
             for (int g = 0; g < G && j0 + g < ncols; g++) {
-                partial[(size_t)(j0 + g) * nblk + blockIdx.x] = s[g];
-                partial[(size_t)(ncols + j0 + g) * nblk + blockIdx.x] = s[G + g];
+                a.partial[(size_t)(j0 + g) * nblk + bx] = s[g];
+                a.partial[(size_t)(ncols + j0 + g) * nblk + bx] = s[G + g];
             }
     } else {
 #pragma unroll 4
-        for (int c = blockIdx.x * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];
+        for (int c = bx * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) s[0] += f[c] * f[c];
         block_sum<BLOCK, 1>(s, lds);
-        if (threadIdx.x == 0) partial[(size_t)(2 * ncols) * nblk + blockIdx.x] = s[0];
+        if (threadIdx.x == 0) a.partial[(size_t)(2 * ncols) * nblk + bx] = s[0];
     }
+}
+
+template <int BLOCK, int G>
+__global__ __launch_bounds__(BLOCK) void k_aa_dots(AaDots a) {
+    aa_dots_block<BLOCK, G>(a, blockIdx.x, blockIdx.y);
 }
 
 // out[q] = sum_b partial[q * nblk + b] in a fixed order; host (or null): the same values also
 // stored into coherent host memory, which the host reads after synchronising (no copy)
-__global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ partial, double* __restrict__ out,
-                              double* __restrict__ host) {
+__device__ __forceinline__ void reduce_rows_range(int nq, int nblk, const double* __restrict__ partial,
+                                                  double* __restrict__ out, double* __restrict__ host, int blk,
+                                                  int ngrid) {
     __shared__ double lds[4];
-    for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    for (int q = blk; q < nq; q += ngrid) {
         double s[1] = {0};
         for (int i = threadIdx.x; i < nblk; i += 256) s[0] += partial[(size_t)q * nblk + i];
         block_sum<256, 1>(s, lds);
@@ -1788,6 +1808,11 @@ __global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ parti
             if (host) host[q] = s[0];
         }
     }
+}
+
+__global__ void k_reduce_rows(int nq, int nblk, const double* __restrict__ partial, double* __restrict__ out,
+                              double* __restrict__ host) {
+    reduce_rows_range(nq, nblk, partial, out, host, blockIdx.x, gridDim.x);
 }
 
 // Stores seq into coherent host memory; launched behind the work a host thread waits for, it runs
@@ -1803,54 +1828,78 @@ __global__ void k_signal(long long seq, long long* __restrict__ host) {
 // g = G(x) gathered (T, and the boundary intensities at bcells scaled by sc), and in the same pass
 // f = g - x into f_cur and f_prev, x into x_prev and, unless first, the history columns
 // dX = x - x_prev and dF = f - f_prev (from the previous step's values), over the Z = N + S nb
-// state entries.  take: then x = g (a plain
-// step), and with dens the state's loads for the next iteration from it, as k_set_density and
-// k_state_scatter form them (the emission density of the T part, the boundary intensities).
-__global__ void k_state_gather_diff(int N, int D, int S, int nb, EeIdx ei, const int* __restrict__ bcells,
-                                    const double* __restrict__ sc, const double* __restrict__ T,
-                                    double* __restrict__ ee, double* __restrict__ z, int first,
-                                    double* __restrict__ x, double* __restrict__ x_prev,
-                                    double* __restrict__ f_prev, double* __restrict__ f_cur, double* __restrict__ dX,
-                                    float* __restrict__ dF, float* __restrict__ dG, int take, Bands bands,
-                                    double* __restrict__ dens) {
-    const long long total = N + (long long)S * nb;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-         t += (long long)gridDim.x * blockDim.x) {
+// state entries.  take: then x = g (a plain step), and with dens the state's loads for the next
+// iteration from it, as k_set_density and k_state_scatter form them (the emission density of the T
+// part, the boundary intensities).
+struct AaGather {
+    int N, D, S, nb;
+    EeIdx ei;
+    const int* bcells;
+    const double* sc;
+    const double* T;
+    double* ee;
+    double* z;
+    int first, take;
+    double* x;
+    double* x_prev;
+    double* f_prev;
+    double* f_cur;
+    double* dX;
+    float* dF;
+    float* dG;
+    Bands bands;
+    double* dens;
+};
+
+// Body over work-group blk of nblk (blockDim.x threads each).  Every load of an entry precedes its
+// stores.
+__device__ __forceinline__ void gather_diff_range(const AaGather& a, int blk, int nblk) {
+    const int N = a.N, D = a.D, nb = a.nb;
+    const long long total = N + (long long)a.S * nb;
+    for (long long t = blk * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)nblk * blockDim.x) {
         double gv;
+        long long at = 0;  // the boundary intensity's index
+        int s = 0;
         if (t < N) {
-            gv = T[t];
+            gv = a.T[t];
         } else {
             const long long u = t - N;
-            const int s = (int)(u / nb), j = (int)(u % nb);
-            gv = ee[ei.at(s / D, s % D, bcells[j])] * sc[s / D];
+            s = (int)(u / nb);
+            const int j = (int)(u % nb);
+            at = (long long)a.ei.at(s / D, s % D, a.bcells[j]);
+            gv = a.ee[at] * a.sc[s / D];
         }
-        z[t] = gv;
-        const double xv = x[t];
+        const double xv = a.x[t];
         const double f = gv - xv;
-        if (!first) {
-            const double dx = xv - x_prev[t], df = f - f_prev[t];
-            dX[t] = dx;
-            dF[t] = (float)df;
-            dG[t] = (float)(dx + df);
+        double dx = 0, df = 0;
+        if (!a.first) {
+            dx = xv - a.x_prev[t];
+            df = f - a.f_prev[t];
         }
-        x_prev[t] = xv;
-        f_prev[t] = f;
-        f_cur[t] = f;
-        if (take) {
-            x[t] = gv;
-            if (dens) {
+        a.z[t] = gv;
+        if (!a.first) {
+            a.dX[t] = dx;
+            a.dF[t] = (float)df;
+            a.dG[t] = (float)(dx + df);
+        }
+        a.x_prev[t] = xv;
+        a.f_prev[t] = f;
+        a.f_cur[t] = f;
+        if (a.take) {
+            a.x[t] = gv;
+            if (a.dens) {
                 if (t < N) {
-                    for (int b = 0; b < bands.B; b++)
-                        dens[(size_t)t * bands.B + b] = bands.C[b] / bands.solid_angle * (gv - bands.T_ref);
+                    for (int b = 0; b < a.bands.B; b++)
+                        a.dens[(size_t)t * a.bands.B + b] = a.bands.C[b] / a.bands.solid_angle * (gv - a.bands.T_ref);
                 } else {
-                    const long long u = t - N;
-                    const int s = (int)(u / nb), j = (int)(u % nb);
-                    ee[ei.at(s / D, s % D, bcells[j])] = gv / sc[s / D];
+                    a.ee[at] = gv / a.sc[s / D];
                 }
             }
         }
     }
 }
+
+__global__ void k_state_gather_diff(AaGather a) { gather_diff_range(a, blockIdx.x, gridDim.x); }
 
 __global__ void k_state_scatter(int N, int D, int S, int nb, EeIdx ei, const int* __restrict__ bcells,
                                 const double* __restrict__ sc, const double* __restrict__ z, double* __restrict__ ee) {
@@ -2011,39 +2060,55 @@ __global__ void k_fill(int n, double v, double* __restrict__ x) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
 }
 
-// Mixing coefficients of one Anderson step, passed by value (kernel arguments): no upload precedes
-// the update.
+// Mixing coefficients of one Anderson step.
 constexpr int kAaMaxCols = 64;
 struct AaCoef {
     double g[kAaMaxCols];
+    __device__ double operator[](int j) const { return g[j]; }
+};
+
+struct AaUpdate {
+    int N, D, S, nb;
+    EeIdx ei;
+    int ncols;
+    const double* g;
+    const float* dG;
+    double* x;
+    Bands bands;
+    double* dens;
+    const int* bcells;
+    const double* sc;
+    double* ee;
+    const double* gamma;  // the coefficients in device memory (the multi-problem launch)
 };
 
 // The mixing update x = g - sum_j gamma_j (dX_j + dF_j) over the Z = N + S nb state entries (the
 // columns dG_j = dX_j + dF_j are stored as such), and in the same pass the state's loads for the
 // next iteration from the new x: the emission density of its temperature part (as k_set_density)
-// and its boundary intensities (as k_state_scatter).
-__global__ void k_aa_update_state(int N, int D, int S, int nb, EeIdx ei, int ncols, AaCoef gamma,
-                                  const double* __restrict__ g, const float* __restrict__ dG,
-                                  double* __restrict__ x, Bands bands,
-                                  double* __restrict__ dens, const int* __restrict__ bcells,
-                                  const double* __restrict__ sc, double* __restrict__ ee) {
-    const long long Z = N + (long long)S * nb;
-    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < Z; c += (long long)gridDim.x * blockDim.x) {
-        double v = g[c];
+// and its boundary intensities (as k_state_scatter).  Body over work-group blk of nblk.
+template <class Coef>
+__device__ __forceinline__ void aa_update_range(const AaUpdate& a, const Coef& gamma, int blk, int nblk) {
+    const long long Z = a.N + (long long)a.S * a.nb;
+    const float* __restrict__ dG = a.dG;
+    for (long long c = blk * (long long)blockDim.x + threadIdx.x; c < Z; c += (long long)nblk * blockDim.x) {
+        double v = a.g[c];
         // unrolled so that several columns' loads are in flight; the updates keep their order
 #pragma unroll 8
-        for (int j = 0; j < ncols; j++) v -= gamma.g[j] * (double)dG[(size_t)j * Z + c];
-        x[c] = v;
-        if (c < N) {
-            for (int b = 0; b < bands.B; b++)
-                dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (v - bands.T_ref);
+        for (int j = 0; j < a.ncols; j++) v -= gamma[j] * (double)dG[(size_t)j * Z + c];
+        a.x[c] = v;
+        if (c < a.N) {
+            for (int b = 0; b < a.bands.B; b++)
+                a.dens[(size_t)c * a.bands.B + b] = a.bands.C[b] / a.bands.solid_angle * (v - a.bands.T_ref);
         } else {
-            const long long u = c - N;
-            const int s = (int)(u / nb), j = (int)(u % nb);
-            ee[ei.at(s / D, s % D, bcells[j])] = v / sc[s / D];
+            const long long u = c - a.N;
+            const int s = (int)(u / a.nb), j = (int)(u % a.nb);
+            a.ee[a.ei.at(s / a.D, s % a.D, a.bcells[j])] = v / a.sc[s / a.D];
         }
     }
 }
+
+// Single problem: the coefficients travel as kernel arguments (no upload precedes the update).
+__global__ void k_aa_update_state(AaUpdate a, AaCoef gamma) { aa_update_range(a, gamma, blockIdx.x, gridDim.x); }
 
 // Emission density of a given temperature field (StaticBTESolver.cpp:115-119), cell-major [N][B].
 __global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens) {
@@ -2163,6 +2228,34 @@ __global__ void k_tcell_multi(const IterDesc* __restrict__ desc, const int* __re
 __global__ __launch_bounds__(256) void k_margin_multi(const IterDesc* __restrict__ desc) {
     const IterDesc& D = desc[blockIdx.x];
     margin_block(D.nblk_tband, D.partial, D.scale, D.margin);
+}
+
+// The mixing steps of the problems, one launch per kernel (AaGather, AaDots, AaUpdate descriptors);
+// each problem's work-groups keep the single-problem launch's partition: grid_for(Z, 256) for the
+// gather and the update, (kAaDotBlocks, ceil(ncols / G) + 1) for the dot products (flattened
+// x-fastest), min(2 ncols + 1, 128) for their reduction.
+__global__ void k_state_gather_diff_multi(const AaGather* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    gather_diff_range(desc[m.p], m.blk, m.nblk);
+}
+
+template <int BLOCK, int G>
+__global__ __launch_bounds__(BLOCK) void k_aa_dots_multi(const AaDots* __restrict__ desc, const int* __restrict__ off,
+                                                        int n) {
+    const MultiIdx m = multi_index(off, n);
+    aa_dots_block<BLOCK, G>(desc[m.p], m.blk % kAaDotBlocks, m.blk / kAaDotBlocks);
+}
+
+__global__ void k_aa_reduce_multi(const AaDots* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    const AaDots& D = desc[m.p];
+    reduce_rows_range(2 * D.ncols + 1, kAaDotBlocks, D.partial, D.out, D.host, m.blk, m.nblk);
+}
+
+__global__ void k_aa_update_multi(const AaUpdate* __restrict__ desc, const int* __restrict__ off, int n) {
+    const MultiIdx m = multi_index(off, n);
+    const AaUpdate& D = desc[m.p];
+    aa_update_range(D, D.gamma, m.blk, m.nblk);
 }
 
 // Heat flux per (BC, band), StaticBTESolver.cpp:323-370.  One work-group per pair.

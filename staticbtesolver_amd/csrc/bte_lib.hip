@@ -113,6 +113,7 @@ struct bte_ctx {
     // bte_begin, band-major whenever work buffers are (re)sized)
     EeIdx eei{};
     bool ee_inter = false;
+    bool run_inter = false;  // the layout of the last solve's intensities (ee_final)
     double* h_margin_pinned = nullptr;
     // coherent pinned host memory that kernels store into directly (no copy, read after a
     // synchronisation): [0] the margin of the last outer iteration, [64, 64 + 2 m + 1) the dot
@@ -1535,15 +1536,27 @@ static int load_state(bte_ctx* ctx) {
     return BTE_OK;
 }
 
+constexpr int kAaDotG = 4;  // history columns per work-group of the dot products
+
+// The launches of one problem's mixing step, recorded instead of issued: the uneven batch issues
+// those of all its problems with one launch per kernel.
+struct AaRec {
+    bool gather = false, dots = false, update = false;
+    AaGather ga{};
+    AaDots da{};
+    AaUpdate ua{};
+    AaCoef coef{};
+};
+
 // One Anderson step after an outer iteration produced g = G(x) in ctx->T: forms the next iterate
 // x in aa_x and the emission density of x for the following iteration.
 // First half of an Anderson step: the state gather, the difference columns and, for a mixing
 // step, the dot products, stored into coherent host memory; nothing waits.  Returns in *finish whether
-// aa_finish must follow (after the stream has been synchronised).
-static int aa_launch(bte_ctx* ctx, bool* finish) {
+// aa_finish must follow (after the stream has been synchronised).  rec: the launches are recorded
+// there instead of issued.
+static int aa_launch(bte_ctx* ctx, bool* finish, AaRec* rec = nullptr) {
     const int N = ctx->N, m = ctx->aa_m, S = ctx->B * ctx->D, nb = ctx->aa_nb;
     const int Z = (int)ctx->aa_Z;
-    const int gb = grid_for(Z, 256);
     *finish = false;
     ctx->aa_steps++;
     // g = G(x): the new temperature field and reflecting-boundary intensities, gathered in the same
@@ -1551,11 +1564,16 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     // latest dX)
     // take: a plain step x = g in the same pass (load: with the state's loads for the next iteration)
     auto gather_diff = [&](int first, float* dF, float* dG, int take = 0, bool load = false) {
-        hipLaunchKernelGGL(k_state_gather_diff, dim3(gb), dim3(256), 0, ctx->stream, N, ctx->D, S, nb, ctx->eei,
-                           ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->T.as<double>(), ctx->ee_final,
-                           ctx->aa_g.as<double>(), first, ctx->aa_x.as<double>(), ctx->aa_xprev.as<double>(),
-                           ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(), ctx->aa_dX.as<double>(), dF, dG, take,
-                           bands_of(ctx), load ? ctx->dens.as<double>() : (double*)nullptr);
+        AaGather a{N, ctx->D, S, nb, ctx->eei, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(),
+                   ctx->T.as<double>(), ctx->ee_final, ctx->aa_g.as<double>(), first, take, ctx->aa_x.as<double>(),
+                   ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(),
+                   ctx->aa_dX.as<double>(), dF, dG, bands_of(ctx), load ? ctx->dens.as<double>() : (double*)nullptr};
+        if (rec) {
+            rec->gather = true;
+            rec->ga = a;
+        } else {
+            hipLaunchKernelGGL(k_state_gather_diff, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, a);
+        }
     };
     if (ctx->aa_plain) {
         // plain iteration: x_{k+1} = G(x_k); the iteration already left T, density and the boundary
@@ -1588,12 +1606,18 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
     if (!skip) ctx->aa_order.push_back(slot);
     // the filled slots are [0, n): slots are taken in order and reused only once all m are filled
     const int n = (int)ctx->aa_order.size();
-    const int nq = 2 * n + 1, nblk = 64;
-    hipLaunchKernelGGL((k_aa_dots<256, 4>), dim3(nblk, (n + 3) / 4 + 1), dim3(256), 0, ctx->stream, Z, n,
-                       ctx->aa_dF.as<float>(), ctx->aa_dF.as<float>() + (size_t)(skip ? 0 : slot) * Z,
-                       ctx->aa_f.as<double>(), aa_part_of(ctx));
-    hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, nblk,
-                       aa_part_of(ctx), aa_dots_of(ctx), ctx->d_sync + 64);
+    const int nq = 2 * n + 1;
+    const AaDots d{Z, n, ctx->aa_dF.as<float>(), ctx->aa_dF.as<float>() + (size_t)(skip ? 0 : slot) * Z,
+                   ctx->aa_f.as<double>(), aa_part_of(ctx), aa_dots_of(ctx), ctx->d_sync + 64};
+    if (rec) {
+        rec->dots = true;
+        rec->da = d;
+    } else {
+        hipLaunchKernelGGL((k_aa_dots<256, kAaDotG>), dim3(kAaDotBlocks, (n + kAaDotG - 1) / kAaDotG + 1), dim3(256), 0,
+                           ctx->stream, d);
+        hipLaunchKernelGGL(k_reduce_rows, dim3(std::min(nq, 128)), dim3(256), 0, ctx->stream, nq, kAaDotBlocks,
+                           d.partial, d.out, d.host);
+    }
     ctx->aa_slot = slot;
     ctx->aa_skip_now = skip;
     ctx->aa_ncols = n;
@@ -1602,11 +1626,11 @@ static int aa_launch(bte_ctx* ctx, bool* finish) {
 }
 
 // Second half, once the dot products have arrived: the m x m least squares on the host, then the
-// update (the coefficients travel as its kernel arguments) and the state.
-static int aa_finish(bte_ctx* ctx) {
+// update (the coefficients travel as its kernel arguments, or, recorded, with its descriptor) and the
+// state.
+static int aa_finish(bte_ctx* ctx, AaRec* rec = nullptr) {
     const int m = ctx->aa_m;
     const int Z = (int)ctx->aa_Z;
-    const int gb = grid_for(Z, 256);
     const int slot = ctx->aa_slot;
     const int nc = ctx->aa_ncols;  // the dots cover the filled columns [0, nc)
     const double* dots = ctx->h_sync + 64;
@@ -1631,10 +1655,16 @@ static int aa_finish(bte_ctx* ctx) {
         for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
     }
     // the update and the state's density and boundary intensities in one pass (load_state's work)
-    hipLaunchKernelGGL(k_aa_update_state, dim3(gb), dim3(256), 0, ctx->stream, ctx->N, ctx->D, ctx->B * ctx->D,
-                       ctx->aa_nb, ctx->eei, nc, coef, ctx->aa_g.as<double>(), ctx->aa_dG.as<float>(), ctx->aa_x.as<double>(),
-                       bands_of(ctx), ctx->dens.as<double>(),
-                       ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final);
+    const AaUpdate u{ctx->N, ctx->D, ctx->B * ctx->D, ctx->aa_nb, ctx->eei, nc, ctx->aa_g.as<double>(),
+                     ctx->aa_dG.as<float>(), ctx->aa_x.as<double>(), bands_of(ctx), ctx->dens.as<double>(),
+                     ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final, nullptr};
+    if (rec) {
+        rec->update = true;
+        rec->ua = u;
+        rec->coef = coef;
+        return BTE_OK;
+    }
+    hipLaunchKernelGGL(k_aa_update_state, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, u, coef);
     HIPCHK(hipGetLastError());
     return BTE_OK;
 }
@@ -1969,6 +1999,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     // cell's intensities together, into whole sectors); band-major for every other form
     set_layout(ctx, direct_path(ctx, &o) && ctx->variant_used == 4 && ctx->B >= 2 &&
                         (o.intensity_layout == 2 || (o.intensity_layout == 0 && ctx->dim == 3 && ctx->B >= 4)));
+    ctx->run_inter = ctx->ee_inter;
     // the zero initial intensities (the first iteration's ee_prev); the buffer it writes is cleared
     // only for the Krylov forms, whose initial guess it is (a sweep writes every cell)
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, ctx->eeA.bytes, ctx->stream));
@@ -2364,12 +2395,13 @@ int bte_get_intensity(bte_ctx* ctx, double* ee) {
     HIPCHK(hipSetDevice(ctx->device));
     const size_t S = (size_t)ctx->B * ctx->D, N = ctx->N;
     std::vector<double> h(S * N);
-    if (ctx->ee_inter) {  // to band-major
+    if (ctx->run_inter) {  // to band-major
+        const EeIdx ei = ee_interleaved(ctx->B, ctx->N);
         std::vector<double> r((size_t)ctx->D * N * ee_padded_bands(ctx->B));
         HIPCHK(hipMemcpy(r.data(), ctx->ee_final, sizeof(double) * r.size(), hipMemcpyDeviceToHost));
         for (int b = 0; b < ctx->B; b++)
             for (int d = 0; d < ctx->D; d++)
-                for (size_t c = 0; c < N; c++) h[((size_t)b * ctx->D + d) * N + c] = r[ctx->eei.at(b, d, (int)c)];
+                for (size_t c = 0; c < N; c++) h[((size_t)b * ctx->D + d) * N + c] = r[ei.at(b, d, (int)c)];
     } else {
         HIPCHK(hipMemcpy(h.data(), ctx->ee_final, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     }
@@ -2393,7 +2425,7 @@ int bte_get_stats(bte_ctx* ctx, bte_stats* st) {
     st->block = ctx->block_used;
     st->variant = ctx->variant_used;
     st->band_group = ctx->group_used;
-    st->intensity_layout = ctx->ee_inter ? 2 : 1;
+    st->intensity_layout = ctx->run_inter ? 2 : 1;
     return BTE_OK;
 }
 
@@ -2430,7 +2462,11 @@ int bte_debug_assemble(bte_ctx* ctx, int band, int dir, double* diag, double* of
     return BTE_OK;
 }
 
+// The debug entry points overwrite the solver's state buffers: a solve in progress cannot continue
+// after them, and its intensities can no longer be read.
 static int upload_state(bte_ctx* ctx, const double* ee_prev, const double* density) {
+    ctx->ee_final = nullptr;
+    ctx->run_active = false;
     const int N = ctx->N, B = ctx->B;
     const size_t S = (size_t)B * ctx->D, SN = S * N;
     std::vector<double> ee(SN);
@@ -2507,6 +2543,11 @@ struct bte_batch {
     DBuf desc, items, mptr, mout;
     DBuf idesc;                  // per iteration: IterDesc of each active problem, then step offsets
     std::vector<unsigned char> h_idesc;
+    // per iteration, the problems' recorded mixing steps and the descriptors of their launches
+    // (host copies kept until the next synchronisation of the batch stream)
+    std::vector<AaRec> aa_rec;
+    DBuf aa_first, aa_second;
+    std::vector<unsigned char> h_aa_first, h_aa_second;
     std::vector<int> items_act;  // the active problems b->items and b->mptr were built for
     double* h_m = nullptr;  // pinned [K] margins
     int ring = 0, block = 256, n_items = 0, max_levels = 0;
@@ -2732,6 +2773,87 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
     return BTE_OK;
 }
 
+}  // extern "C"
+
+// Appends n records of T to a host staging buffer at an 8-byte aligned offset; returns the offset.
+template <class T>
+static size_t stage(std::vector<unsigned char>& h, const T* v, size_t n) {
+    const size_t at = (h.size() + 7) & ~(size_t)7;
+    h.resize(at + sizeof(T) * n);
+    if (n) std::memcpy(h.data() + at, v, sizeof(T) * n);
+    return at;
+}
+
+// The recorded first halves of the problems' mixing steps: the gathers, then the dot products and
+// their reductions, one launch each for all problems.
+static int batch_aa_launch(bte_batch* b) {
+    std::vector<AaGather> g;
+    std::vector<AaDots> d;
+    std::vector<int> og{0}, od{0}, orr{0};
+    for (const AaRec& r : b->aa_rec) {
+        if (r.gather) {
+            g.push_back(r.ga);
+            og.push_back(og.back() + grid_for(r.ga.N + (long long)r.ga.S * r.ga.nb, 256));
+        }
+        if (r.dots) {
+            d.push_back(r.da);
+            od.push_back(od.back() + kAaDotBlocks * ((r.da.ncols + kAaDotG - 1) / kAaDotG + 1));
+            orr.push_back(orr.back() + std::min(2 * r.da.ncols + 1, 128));
+        }
+    }
+    if (g.empty()) return BTE_OK;
+    std::vector<unsigned char>& h = b->h_aa_first;
+    h.clear();
+    const size_t at_g = stage(h, g.data(), g.size()), at_d = stage(h, d.data(), d.size());
+    const size_t at_og = stage(h, og.data(), og.size()), at_od = stage(h, od.data(), od.size());
+    const size_t at_or = stage(h, orr.data(), orr.size());
+    BCHK(b->aa_first.ensure(h.size()));
+    BCHK(hipMemcpyAsync(b->aa_first.p, h.data(), h.size(), hipMemcpyHostToDevice, b->stream));
+    const char* base = (const char*)b->aa_first.p;
+    hipLaunchKernelGGL(k_state_gather_diff_multi, dim3(og.back()), dim3(256), 0, b->stream,
+                       (const AaGather*)(base + at_g), (const int*)(base + at_og), (int)g.size());
+    if (!d.empty()) {
+        hipLaunchKernelGGL((k_aa_dots_multi<256, kAaDotG>), dim3(od.back()), dim3(256), 0, b->stream,
+                           (const AaDots*)(base + at_d), (const int*)(base + at_od), (int)d.size());
+        hipLaunchKernelGGL(k_aa_reduce_multi, dim3(orr.back()), dim3(256), 0, b->stream, (const AaDots*)(base + at_d),
+                           (const int*)(base + at_or), (int)d.size());
+    }
+    BCHK(hipGetLastError());
+    return BTE_OK;
+}
+
+// The recorded updates, one launch for all problems; each problem's coefficients travel in the same
+// upload as the descriptors.
+static int batch_aa_update(bte_batch* b) {
+    std::vector<AaUpdate> u;
+    std::vector<double> gam;
+    std::vector<size_t> gat;
+    std::vector<int> ou{0};
+    for (const AaRec& r : b->aa_rec)
+        if (r.update) {
+            u.push_back(r.ua);
+            gat.push_back(gam.size());
+            gam.insert(gam.end(), r.coef.g, r.coef.g + std::max(r.ua.ncols, 1));
+            ou.push_back(ou.back() + grid_for(r.ua.N + (long long)r.ua.S * r.ua.nb, 256));
+        }
+    if (u.empty()) return BTE_OK;
+    std::vector<unsigned char>& h = b->h_aa_second;
+    h.clear();
+    const size_t at_u = stage(h, u.data(), u.size()), at_c = stage(h, gam.data(), gam.size());
+    const size_t at_o = stage(h, ou.data(), ou.size());
+    BCHK(b->aa_second.ensure(h.size()));
+    const char* base = (const char*)b->aa_second.p;
+    AaUpdate* hu = (AaUpdate*)(h.data() + at_u);
+    for (size_t k = 0; k < u.size(); k++) hu[k].gamma = (const double*)(base + at_c) + gat[k];
+    BCHK(hipMemcpyAsync(b->aa_second.p, h.data(), h.size(), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_aa_update_multi, dim3(ou.back()), dim3(256), 0, b->stream, (const AaUpdate*)(base + at_u),
+                       (const int*)(base + at_o), (int)u.size());
+    BCHK(hipGetLastError());
+    return BTE_OK;
+}
+
+extern "C" {
+
 int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
     if (!b) return BTE_ERR_ARG;
     BCHK(hipSetDevice(b->device));
@@ -2781,16 +2903,14 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
         int rc = batch_iteration(b, act, m);
         if (rc) return rc;
         std::vector<int> next;
-        // the bookkeeping of every problem, then the mixing steps of all of them with one
-        // synchronisation for their dot products.  These small per-problem kernels run on each
-        // problem's own stream, so that those of different problems overlap; batch_iteration left
-        // the batch stream idle, and the device is synchronised before it is used again.  Not with
-        // attached networks: problems may share one, and with it its work buffers.
-        bool own = true;
-        for (int k : act) own = own && b->ctx[k]->gnn == nullptr;
-        if (own)
-            for (int k : act) b->ctx[k]->stream = streams.saved[k];
-        std::vector<int> fin;
+        // the bookkeeping of every problem, with its mixing step recorded; then the steps of all of
+        // them, one launch per kernel, with one synchronisation for their dot products
+        b->aa_rec.assign(act.size(), AaRec{});
+        auto fail = [&](int a) {
+            b->err = "problem " + std::to_string(act[a]) + ": " + b->ctx[act[a]]->err;
+            return rc;
+        };
+        std::vector<int> fin, plain;
         for (size_t a = 0; a < act.size(); a++) {
             bte_ctx* ctx = b->ctx[act[a]];
             const int par = ctx->run_iter & 1;
@@ -2798,28 +2918,23 @@ int bte_batch_solve(bte_batch* b, const bte_opts* opts, bte_result* results) {
             ctx->run_iter++;
             bool counted = false, mix = false, finish = false;
             rc = after_judge(ctx, m[a], counted, mix);
-            if (!rc && mix) rc = aa_launch(ctx, &finish);
-            if (rc) {
-                b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
-                return rc;
-            }
-            if (finish) fin.push_back((int)a);
-            if (mix && !finish && (rc = after_mix(ctx))) {
-                b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
-                return rc;
-            }
+            if (!rc && mix) rc = aa_launch(ctx, &finish, &b->aa_rec[a]);
+            if (rc) return fail((int)a);
+            if (finish)
+                fin.push_back((int)a);
+            else if (mix)
+                plain.push_back((int)a);
         }
-        if (!fin.empty()) BCHK(own ? hipDeviceSynchronize() : hipStreamSynchronize(b->stream));
-        for (int a : fin) {
-            bte_ctx* ctx = b->ctx[act[a]];
-            if ((rc = aa_finish(ctx)) || (rc = after_mix(ctx))) {
-                b->err = "problem " + std::to_string(act[a]) + ": " + ctx->err;
-                return rc;
-            }
-        }
-        if (own) {
-            BCHK(hipDeviceSynchronize());
-            for (int k : act) b->ctx[k]->stream = b->stream;
+        if ((rc = batch_aa_launch(b))) return rc;
+        for (int a : plain)
+            if ((rc = after_mix(b->ctx[act[a]]))) return fail(a);
+        if (!fin.empty()) {
+            BCHK(hipStreamSynchronize(b->stream));
+            for (int a : fin)
+                if ((rc = aa_finish(b->ctx[act[a]], &b->aa_rec[a]))) return fail(a);
+            if ((rc = batch_aa_update(b))) return rc;
+            for (int a : fin)
+                if ((rc = after_mix(b->ctx[act[a]]))) return fail(a);
         }
         for (size_t a = 0; a < act.size(); a++) {
             bte_ctx* ctx = b->ctx[act[a]];
