@@ -441,7 +441,7 @@ def main():
                     help="only time K outer iterations of the form's inner solve and print its roofline")
     ap.add_argument("--block", type=int, default=0, choices=[0, 64, 128, 256, 512, 1024],
                     help="work-group size of the inner solve (0: autotuned)")
-    ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2, 3, 4, 5, 6, 7],
+    ap.add_argument("--variant", type=int, default=0, choices=[0, 1, 2, 3, 4, 5, 6, 7, 8],
                     help="sweep variant of the direct form (0: autotuned)")
     ap.add_argument("--band-group", type=int, default=0, choices=[0, 1, 2, 4],
                     help="jacobi: bands per work-group of the batched BiCGSTAB (0: autotuned)")
@@ -574,7 +574,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": prof["bytes"] if prof else None,
                      "kernel": {4: "k_sweep_ring", 5: "k_sweep_wave", 6: "k_sweep_ring_bands",
-                                7: "k_sweep_ring_bands"}.get(st["variant"], "inner solve"),
+                                7: "k_sweep_ring_bands", 8: "k_sweep_ring"}.get(st["variant"], "inner solve"),
                      "kernel_ms": round(solve_ms, 5), "rhs_ms": round(rhs_ms, 5), "alg_bytes_per_launch": b_alg,
                      "alg_bytes_formula": "B_shared: each direction's operator once + 2 w n per system",
                      "achieved_canonical": round(b_canon / (solve_ms * 1e-3) / 1e9, 2),

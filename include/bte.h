@@ -70,8 +70,10 @@ typedef struct bte_opts {
                              (needs n_cell * 8 bytes of LDS), 4 pipelined (operands loaded three
                              levels ahead, recent intensities in an LDS ring), 5 as 4 with one
                              wave per system (no work-group barriers), 6 / 7 as 4 with 2 / 4 bands
-                             of a direction per work-group sharing each operator load; a form the
-                             mesh cannot use falls back to 2; default 0 */
+                             of a direction per work-group sharing each operator load, 8 as 4 with
+                             a ring of 16384 values (128 KB of LDS: one work-group per compute unit,
+                             fewer upwind values read back from HBM); a form the mesh cannot use
+                             falls back to 2; default 0 */
     int accel;            /* outer iteration: BTE_ACCEL_NONE (the reference's source iteration) or
                              BTE_ACCEL_ANDERSON (Anderson mixing of the temperature field over the
                              last accel_memory iterates); default BTE_ACCEL_NONE */
@@ -85,7 +87,7 @@ typedef struct bte_opts {
     int intensity_layout; /* device layout of the intensities: 0 automatic (3D pipelined sweeps with
                              4 or more bands interleave them), 1 band-major ee[b][d][c] (the
                              reference's), 2 direction-major with each cell's bands interleaved,
-                             ee[d][c][b], when the pipelined sweep (sweep_variant 4) runs, else
+                             ee[d][c][b], when the pipelined sweep (sweep_variant 4 or 8) runs, else
                              band-major.  Results do not depend on it except the margins' summation
                              order; bte_get_intensity returns band-major either way; default 0 */
 } bte_opts;
@@ -124,7 +126,7 @@ typedef struct bte_stats {
     int max_levels;       /* deepest sweep schedule over directions */
     int lagged_edges;     /* upwind edges dropped from the sweep to break cycles */
     int block;            /* work-group size used */
-    int variant;          /* direct-sweep variant used (1..7), 0 for the Krylov paths */
+    int variant;          /* direct-sweep variant used (1..8), 0 for the Krylov paths */
     int band_group;       /* Krylov paths: bands per work-group used (opts.band_group); 0 direct */
     int intensity_layout; /* intensity layout of the last solve: 1 band-major, 2 bands interleaved */
 } bte_stats;

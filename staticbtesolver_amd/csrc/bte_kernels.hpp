@@ -218,7 +218,8 @@ struct Bands {
 // Diffuse (type 2) re-emitted intensity of each boundary face, StaticBTESolver.cpp:180-204.  It
 // does not depend on the incoming direction, so it is formed once per (band, face) and iteration.
 // Body over work-group blk of nblk (blockDim.x threads each); k_diffuse runs it over the grid.
-__device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const Bnd& bd, int B,
+template <class BndT>
+__device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const BndT& bd, int B,
                                               const double* __restrict__ ee_prev, double* __restrict__ ediff, int blk,
                                               int nblk) {
     const int total = B * bd.n_bf;
@@ -273,7 +274,8 @@ __global__ void k_diffuse(Geo g, Quad q, Bnd bd, int B, const double* __restrict
 }
 
 // Right-hand side of system (b, d) at cell c, StaticBTESolver.cpp:162-269.
-__device__ __forceinline__ double rhs_at(const Geo& g, const Quad& q, const Bnd& bd, const Bands& bands, int b, int d,
+template <class BndT>
+__device__ __forceinline__ double rhs_at(const Geo& g, const Quad& q, const BndT& bd, const Bands& bands, int b, int d,
                                          int c, const double* __restrict__ ee_prev, const double* __restrict__ dens) {
     double re = 0;
     const double* dv = q.dir + 3 * d;
@@ -384,10 +386,15 @@ struct Ops {
     // pipelined sweeps with a ring of ring_r values: lvl_sync[lvl_off[d] + l] bit 1 when the sweep
     // of direction d must make its HBM writes visible after level l (a later level reads, from HBM, a
     // value older than the ring written at or before l), bit 0 when level l reads such values;
-    // dir_fb[d] = 1 when direction d has such reads at all
+    // dir_fb[d] bit 0 when direction d has such reads at all.  Bits 3, 2 and dir_fb bit 1: the same
+    // for the wide ring of ring_r2 values (0: none), which a launch selects with ring_sel = 1.
     const unsigned char* lvl_sync;
     const int* dir_fb;
     int ring_r;
+    int ring_r2;
+    int ring_sel;
+    // smallest ring the selected flags hold for
+    __host__ __device__ int flag_ring() const { return ring_sel ? ring_r2 : ring_r; }
     // packed level-order records of the pipelined sweeps (k_pack_sweep), 4 entries per (d, i)
     const int* swp_pki;
     const double* swp_pkd;
@@ -1007,7 +1014,8 @@ __global__ void k_diffuse_rhs_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const 
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
 // their face loops run in work-groups of their own instead of stalling the interior waves; one
 // thread per (list entry, band).
-__device__ __forceinline__ void rhs_bnd_range(const Geo& g, const Quad& q, const Bnd& bd, const Bands& bands,
+template <class BndT>
+__device__ __forceinline__ void rhs_bnd_range(const Geo& g, const Quad& q, const BndT& bd, const Bands& bands,
                                               const Ops& o, int S, const int* __restrict__ list, int n_list,
                                               const double* __restrict__ ee_prev, const double* __restrict__ dens,
                                               double* __restrict__ rhs_lo, int blk, int nblk) {
@@ -1036,6 +1044,38 @@ using gmem = __attribute__((address_space(1))) T;
 template <class T>
 __device__ __forceinline__ gmem<T>* to_global(T* p) {
     return (gmem<T>*)p;
+}
+
+// Descriptors read through the constant address space: a pointer loaded from there is taken to be a
+// global one (only the host fills such memory), so the loads and stores through it are global
+// ones, not flat ones (which also count as LDS traffic and serialise the waits).
+template <class T>
+using cmem = __attribute__((address_space(4))) const T;
+template <class T>
+__device__ __forceinline__ T cload(cmem<T>* p) {
+    T v;
+    __builtin_memcpy(&v, (const T*)p, sizeof(T));
+    return v;
+}
+
+// Boundary data of a descriptor in constant memory, its per-condition tables read in place (a copy
+// of the tables, indexed at run time, would live in scratch memory).
+struct BndView {
+    int nbc;
+    cmem<int>* bci;
+    cmem<int>* bct;
+    cmem<double>* bcT;
+    int n_bf;
+    const int* bf_cell;
+    const int* bf_face;
+    const int* bfid;
+    const int* mirror;
+    const double* ediff;
+    int has_diffuse;
+};
+__device__ __forceinline__ BndView bnd_view(cmem<Bnd>* p) {
+    return BndView{p->nbc, p->bci, p->bct, p->bcT, p->n_bf, p->bf_cell, p->bf_face, p->bfid, p->mirror, p->ediff,
+                   p->has_diffuse};
 }
 
 // Raw buffer loads and stores: a buffer resource (four scalar registers) holds an array's base and
@@ -1187,10 +1227,10 @@ __host__ __device__ inline size_t ring_lds_bytes(int ring, int max_levels) {
 // synchronisation lies between the write of every value read this way and its read.  All other
 // levels synchronise on the LDS only, leaving the prefetches in flight.  FB = false: no direction
 // of the launch has such reads (no HBM read-back path, no full synchronisation); with FB = true a
-// direction without them (o.dir_fb[d] = 0) still runs the FB = false code.
+// direction without them (its o.dir_fb bit clear) still runs the FB = false code.  o.ring_sel picks
+// the flags computed for the launch's ring (the regular or the wide one).
 template <int BLOCK, int W, bool FB, int PD = 3>
 __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
-    const int N = o.N;
     const int tid = threadIdx.x;
     const int R = ring_mask + 1;
     int* lps = (int*)(ring + R);
@@ -1205,7 +1245,8 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         // 30 (it reads values older than the ring)
         if constexpr (FB) {
             const unsigned char* __restrict__ fl = o.lvl_sync + o.lvl_off[d];
-            for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t] | (t < L ? (int)fl[t] << 30 : 0);
+            const int sh = 2 * o.ring_sel;
+            for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t] | (t < L ? (int)((fl[t] >> sh) & 3) << 30 : 0);
         } else {
             for (int t = tid; t <= L; t += BLOCK) lps[t] = lp[t];
         }
@@ -1217,9 +1258,9 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         const int lane = tid & 63;
         int wb = 0, win = 0;
         unsigned long long syncs = 0, olds = 0;
-        // the flags hold for rings of at least o.ring_r values; a smaller ring treats every level as
-        // flagged
-        const bool every = R < o.ring_r;
+        // the flags hold for rings of at least o.flag_ring() values; a smaller ring treats every level
+        // as flagged
+        const bool every = R < o.flag_ring();
         auto load_window = [&](int from) {
             wb = from;
             const int v = lps[min(from + lane, L)];
@@ -1291,7 +1332,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
 // FB = true: the HBM read-back path only for a direction that has such reads
 template <int BLOCK, int W, bool FB, int PD = 3>
 __device__ __forceinline__ void sweep_ring_dispatch(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
-    if (FB && (o.dir_fb[s % o.D] || ring_mask + 1 < o.ring_r))
+    if (FB && (((o.dir_fb[s % o.D] >> o.ring_sel) & 1) || ring_mask + 1 < o.flag_ring()))
         sweep_ring_system<BLOCK, W, true, PD>(o, w, s, ring, ring_mask);
     else
         sweep_ring_system<BLOCK, W, false, PD>(o, w, s, ring, ring_mask);
@@ -1435,7 +1476,7 @@ __device__ __forceinline__ void sweep_wave_system(const Ops& o, const Work& w, i
     // level pointers and synchronisation flags as in sweep_ring_system
     if constexpr (FB) {
         const unsigned char* __restrict__ fl = o.lvl_sync + o.lvl_off[d];
-        for (int t = lane; t <= L; t += 64) lps[t] = lp[t] | (t < L ? (int)fl[t] << 30 : 0);
+        for (int t = lane; t <= L; t += 64) lps[t] = lp[t] | (t < L ? (int)(fl[t] & 3) << 30 : 0);
     } else {
         for (int t = lane; t <= L; t += 64) lps[t] = lp[t];
     }
@@ -1506,7 +1547,7 @@ template <int W, bool FB>
 __global__ __launch_bounds__(64) void k_sweep_wave(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
     for (int s = blockIdx.x; s < S; s += gridDim.x) {
-        if (FB && (o.dir_fb[s % o.D] || ring_mask + 1 < o.ring_r))
+        if (FB && ((o.dir_fb[s % o.D] & 1) || ring_mask + 1 < o.ring_r))
             sweep_wave_system<W, 2, true>(o, w, s, ring, ring_mask);
         else
             sweep_wave_system<W, 2, false>(o, w, s, ring, ring_mask);
@@ -2192,41 +2233,47 @@ __device__ __forceinline__ MultiIdx multi_index(const int* __restrict__ off, int
     return MultiIdx{lo, t - off[lo], off[lo + 1] - off[lo]};
 }
 
-__global__ void k_diffuse_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_diffuse_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    const IterDesc& D = desc[m.p];
-    diffuse_range(D.g, D.q, D.bd, D.bands.B, D.ee_prev, D.ediff, m.blk, m.nblk);
+    cmem<IterDesc>* P = desc + m.p;
+    const Geo g = cload(&P->g);
+    const Quad q = cload(&P->q);
+    diffuse_range(g, q, bnd_view(&P->bd), P->bands.B, P->ee_prev, P->ediff, m.blk, m.nblk);
 }
 
-__global__ void k_rhs_levels_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_rhs_levels_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    const IterDesc& D = desc[m.p];
+    const IterDesc D = cload(desc + m.p);
     rhs_levels_range(D.o, D.bands.B, D.dens, D.rhs, m.blk, m.nblk);
 }
 
-__global__ void k_rhs_bnd_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_rhs_bnd_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    const IterDesc& D = desc[m.p];
-    rhs_bnd_range(D.g, D.q, D.bd, D.bands, D.o, D.bands.B * D.o.D, D.bnd_list, D.n_bnd_list, D.ee_prev, D.dens,
-                  D.rhs, m.blk, m.nblk);
+    cmem<IterDesc>* P = desc + m.p;
+    const Geo g = cload(&P->g);
+    const Quad q = cload(&P->q);
+    const Bands bands = cload(&P->bands);
+    const Ops o = cload(&P->o);
+    rhs_bnd_range(g, q, bnd_view(&P->bd), bands, o, bands.B * o.D, P->bnd_list, P->n_bnd_list, P->ee_prev, P->dens,
+                  P->rhs, m.blk, m.nblk);
 }
 
-__global__ __launch_bounds__(256) void k_tband_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off,
+__global__ __launch_bounds__(256) void k_tband_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off,
                                                      int n) {
     const MultiIdx m = multi_index(off, n);
-    const IterDesc& D = desc[m.p];
+    const IterDesc D = cload(desc + m.p);
     tband_block<256>(D.o.N, D.o.D, D.bands, D.q.ee, D.omega, D.ee_curr, D.ee_prev, D.Tband, D.partial, m.blk, m.nblk);
 }
 
-__global__ void k_tcell_multi(const IterDesc* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_tcell_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    const IterDesc& D = desc[m.p];
+    const IterDesc D = cload(desc + m.p);
     tcell_range(D.o.N, D.bands, D.Tband, D.T, D.dens, m.blk, m.nblk);
 }
 
 // one work-group per problem
-__global__ __launch_bounds__(256) void k_margin_multi(const IterDesc* __restrict__ desc) {
-    const IterDesc& D = desc[blockIdx.x];
+__global__ __launch_bounds__(256) void k_margin_multi(cmem<IterDesc>* __restrict__ desc) {
+    const IterDesc D = cload(desc + blockIdx.x);
     margin_block(D.nblk_tband, D.partial, D.scale, D.margin);
 }
 
@@ -2234,27 +2281,29 @@ __global__ __launch_bounds__(256) void k_margin_multi(const IterDesc* __restrict
 // each problem's work-groups keep the single-problem launch's partition: grid_for(Z, 256) for the
 // gather and the update, (kAaDotBlocks, ceil(ncols / G) + 1) for the dot products (flattened
 // x-fastest), min(2 ncols + 1, 128) for their reduction.
-__global__ void k_state_gather_diff_multi(const AaGather* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_state_gather_diff_multi(cmem<AaGather>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    gather_diff_range(desc[m.p], m.blk, m.nblk);
+    const AaGather D = cload(desc + m.p);
+    gather_diff_range(D, m.blk, m.nblk);
 }
 
 template <int BLOCK, int G>
-__global__ __launch_bounds__(BLOCK) void k_aa_dots_multi(const AaDots* __restrict__ desc, const int* __restrict__ off,
+__global__ __launch_bounds__(BLOCK) void k_aa_dots_multi(cmem<AaDots>* __restrict__ desc, const int* __restrict__ off,
                                                         int n) {
     const MultiIdx m = multi_index(off, n);
-    aa_dots_block<BLOCK, G>(desc[m.p], m.blk % kAaDotBlocks, m.blk / kAaDotBlocks);
+    const AaDots D = cload(desc + m.p);
+    aa_dots_block<BLOCK, G>(D, m.blk % kAaDotBlocks, m.blk / kAaDotBlocks);
 }
 
-__global__ void k_aa_reduce_multi(const AaDots* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_aa_reduce_multi(cmem<AaDots>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    const AaDots& D = desc[m.p];
+    const AaDots D = cload(desc + m.p);
     reduce_rows_range(2 * D.ncols + 1, kAaDotBlocks, D.partial, D.out, D.host, m.blk, m.nblk);
 }
 
-__global__ void k_aa_update_multi(const AaUpdate* __restrict__ desc, const int* __restrict__ off, int n) {
+__global__ void k_aa_update_multi(cmem<AaUpdate>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
-    const AaUpdate& D = desc[m.p];
+    const AaUpdate D = cload(desc + m.p);
     aa_update_range(D, D.gamma, m.blk, m.nblk);
 }
 

@@ -97,6 +97,7 @@ struct bte_ctx {
     DBuf lvl_sync, dir_fb;             // pipelined sweep: synchronisation levels, directions with HBM reads
     DBuf swp_pki, swp_pkd;             // pipelined sweep: packed level-order operand records
     int ring_r = 0;                    // ring size those were computed for (ring_size(ctx), 0: none)
+    int ring_r2 = 0;                   // the wide ring's size (ring_wide(ctx), 0: none), flags in bits 2, 3
     int max_levels = 0, lagged = 0;
     int ring_span[7] = {};             // max positions spanned by 64, 32, ..., 2, 1 consecutive sweep levels
     int max_back = 0;                  // max over cells of (end of its level) - (upwind position)
@@ -551,6 +552,7 @@ namespace {
 
 // LDS ring size of the pipelined sweeps (G bands per work-group), 0 when they cannot run
 int ring_size(const bte_ctx* ctx, int* sync_every = nullptr, int G = 1);
+int ring_wide(const bte_ctx* ctx);
 
 // Kahn level of every cell in the upwind graph of direction v (cell c waits for its neighbour
 // across each face with v.n < 0); a cycle is broken at the lowest unplaced cell.
@@ -810,9 +812,12 @@ int prepare(bte_ctx* ctx, bool timing) {
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) == hipSuccess && lds > 0)
             ctx->max_lds = lds;
         ctx->ring_r = ring_size(ctx);
+        ctx->ring_r2 = ring_wide(ctx);
         std::vector<unsigned char> sync_all(lp_all.size(), 0);
         std::vector<int> fb_dir(D, 0);
-        if (ctx->ring_r > 0) {
+        // the flags of a ring of R values into bits sel (reads) and sel + 1 (synchronise after), and
+        // bit sel / 2 of fb_dir
+        auto flag_levels = [&](int R, int sel) {
             std::vector<int> lv_of(N);
             std::vector<std::pair<int, int>> iv;  // (last, first) level end that may synchronise
             for (int d = 0; d < D; d++) {
@@ -825,22 +830,24 @@ int prepare(bte_ctx* ctx, bool timing) {
                     for (int i = P[l]; i < P[l + 1]; i++)
                         for (int k = 0; k < W; k++) {
                             const int p = spos_all[((size_t)d * W + k) * N + i];
-                            if (p >= 0 && p < P[l + 1] - ctx->ring_r) {
+                            if (p >= 0 && p < P[l + 1] - R) {
                                 iv.push_back({l - 1, lv_of[p]});
-                                sync_all[off[d] + l] |= 1;
+                                sync_all[off[d] + l] |= (unsigned char)(1 << sel);
                             }
                         }
                 if (iv.empty()) continue;
-                fb_dir[d] = 1;
+                fb_dir[d] |= 1 << (sel / 2);
                 std::sort(iv.begin(), iv.end());
                 int last = -1;
                 for (const auto& e : iv)
                     if (last < e.second) {
                         last = e.first;
-                        sync_all[off[d] + last] |= 2;
+                        sync_all[off[d] + last] |= (unsigned char)(2 << sel);
                     }
             }
-        }
+        };
+        if (ctx->ring_r > 0) flag_levels(ctx->ring_r, 0);
+        if (ctx->ring_r2 > 0) flag_levels(ctx->ring_r2, 2);
         HIPCHK(upload(ctx->lvl_sync, sync_all, ctx->stream));
         HIPCHK(upload(ctx->dir_fb, fb_dir, ctx->stream));
         ctx->h_lvl_cnt = cnt;
@@ -979,6 +986,8 @@ Ops ops_of(bte_ctx* ctx) {
     o.lvl_sync = ctx->lvl_sync.as<unsigned char>();
     o.dir_fb = ctx->dir_fb.as<int>();
     o.ring_r = ctx->ring_r;
+    o.ring_r2 = ctx->ring_r2;
+    o.ring_sel = 0;
     o.swp_pki = ctx->swp_pki.as<int>();
     o.swp_pkd = ctx->swp_pkd.as<double>();
     o.ring_sync = 1;
@@ -1008,7 +1017,8 @@ Work work_of(bte_ctx* ctx, double* ee_curr) {
 
 // Launch shape of the inner-solve kernel: work-group size and, for the direct sweep, its variant
 // (1 basic, 2 level-ordered operands, 3 level-ordered operands + intensity vector in LDS,
-// 4 level-ordered operands + LDS ring of recent intensities + pipelined operand loads).
+// 4 level-ordered operands + LDS ring of recent intensities + pipelined operand loads, 5 as 4 with
+// one wave per system, 6 / 7 as 4 with 2 / 4 bands per work-group, 8 as 4 with the wide ring).
 struct Shape {
     int block = 256;
     int variant = 2;  // direct form: sweep variant; Krylov forms: bands per work-group (1, 2, 4)
@@ -1056,6 +1066,16 @@ int ring_size(const bte_ctx* ctx, int* sync_every, int G) {
     return 0;
 }
 
+// The wide ring of the pipelined sweep (variant 8): twice the largest ring, 16,384 values (128 KB of
+// LDS, so one work-group per compute unit), where it fits; 0 otherwise.  Fewer upwind values are
+// then older than the ring and read back from HBM, which pays where there are many more systems
+// than compute units.
+int ring_wide(const bte_ctx* ctx) {
+    const int r = ring_size(ctx);
+    if (r != 8192 || ring_lds_bytes(2 * r, ctx->max_levels) > (size_t)ctx->max_lds) return 0;
+    return 2 * r;
+}
+
 // Right-hand sides of all systems in level order (interior cells, then the boundary-cell list);
 // with_diffuse: the diffuse re-emission of ee_prev first, in the interior pass's launch.
 void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out,
@@ -1097,7 +1117,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     dim3 grid(S), blk(BLOCK);
     // diffuse_first: the diffuse re-emission of ee_prev is still due; the pipelined sweep's
     // right-hand-side pass takes it into its launch, every other form runs it first
-    const bool v4 = direct_path(ctx, o) && variant == 4 && ring_size(ctx) > 0;
+    const bool v4 = direct_path(ctx, o) && ((variant == 4 && ring_size(ctx) > 0) || (variant == 8 && ctx->ring_r2 > 0));
     mark(marks, 0, ctx->stream);
     if (diffuse_first && !v4)
         hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
@@ -1153,17 +1173,22 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                 by_w(Unroll<2>{});
             else
                 by_w(Unroll<4>{});
-        } else if (variant == 4 && ring > 0) {
-            ctx->variant_used = 4;
+        } else if ((variant == 4 && ring > 0) || (variant == 8 && ctx->ring_r2 > 0)) {
+            // variant 8: the wide ring, with the level flags computed for it
+            const bool wide = variant == 8;
+            const int R = wide ? ctx->ring_r2 : ring;
+            ctx->variant_used = variant;
+            Ops opr = op;
+            opr.ring_sel = wide ? 1 : 0;
             // right-hand sides in level order into w.b, then the pipelined sweep
             enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b, diffuse_first);
             mark(marks, 1, ctx->stream);
-            const size_t lds = ring_lds_bytes(ring, ctx->max_levels);
+            const size_t lds = ring_lds_bytes(R, ctx->max_levels);
             auto go = [&](auto kernel) {
                 (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-                hipLaunchKernelGGL(kernel, grid, blk, lds, ctx->stream, op, w, S, ring - 1);
+                hipLaunchKernelGGL(kernel, grid, blk, lds, ctx->stream, opr, w, S, R - 1);
             };
-            const bool fb = ctx->max_back > ring;
+            const bool fb = ctx->max_back > R;
             if (ctx->W == 1)
                 fb ? go(k_sweep_ring<BLOCK, 1, true>) : go(k_sweep_ring<BLOCK, 1, false>);
             else if (ctx->W == 2)
@@ -1289,6 +1314,7 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
             cand.push_back({blk, 2});
             if (lds_fits(ctx)) cand.push_back({blk, 3});
             if (ring_size(ctx) > 0) cand.push_back({blk, 4});
+            if (blk >= 128 && ctx->ring_r2 > 0) cand.push_back({blk, 8});
             if (blk == 64 && ring_size(ctx) > 0) cand.push_back({64, 5});  // one wave per system
             if (blk >= 128 && ctx->B >= 2 && ring_size(ctx, nullptr, 2) > 0) cand.push_back({blk, 6});
             if (blk >= 128 && ctx->B >= 4 && ring_size(ctx, nullptr, 4) > 0) cand.push_back({blk, 7});
@@ -1988,6 +2014,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
                         : (shape.variant == 7 && ctx->B >= 4 && ring_size(ctx, nullptr, 4) > 0) ? 7
                         : (shape.variant == 5 && ring_size(ctx) > 0)      ? 5
                         : (shape.variant == 4 && ring_size(ctx) > 0)      ? 4
+                        : (shape.variant == 8 && ctx->ring_r2 > 0)        ? 8
                         : (shape.variant == 3 && lds_fits(ctx))           ? 3
                         : (shape.variant == 1)                            ? 1
                                                                           : 2;
@@ -1997,7 +2024,7 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     // intensity layout: each cell's bands interleaved for the pipelined sweep when asked, or by
     // default in 3D with 4 or more bands (the co-scheduled bands of a direction then write a
     // cell's intensities together, into whole sectors); band-major for every other form
-    set_layout(ctx, direct_path(ctx, &o) && ctx->variant_used == 4 && ctx->B >= 2 &&
+    set_layout(ctx, direct_path(ctx, &o) && (ctx->variant_used == 4 || ctx->variant_used == 8) && ctx->B >= 2 &&
                         (o.intensity_layout == 2 || (o.intensity_layout == 0 && ctx->dim == 3 && ctx->B >= 4)));
     ctx->run_inter = ctx->ee_inter;
     // the zero initial intensities (the first iteration's ee_prev); the buffer it writes is cleared
@@ -2723,7 +2750,7 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
     }
     BCHK(b->idesc.ensure(b->h_idesc.size()));
     BCHK(hipMemcpyAsync(b->idesc.p, b->h_idesc.data(), b->h_idesc.size(), hipMemcpyHostToDevice, b->stream));
-    const IterDesc* did = b->idesc.as<IterDesc>();
+    cmem<IterDesc>* did = (cmem<IterDesc>*)b->idesc.p;
     const int* doff = (const int*)((const char*)b->idesc.p + sizeof(IterDesc) * P);
     auto multi = [&](auto kernel, int step, int block) {
         const int total = offs[step * (P + 1) + P];
@@ -2811,11 +2838,11 @@ static int batch_aa_launch(bte_batch* b) {
     BCHK(hipMemcpyAsync(b->aa_first.p, h.data(), h.size(), hipMemcpyHostToDevice, b->stream));
     const char* base = (const char*)b->aa_first.p;
     hipLaunchKernelGGL(k_state_gather_diff_multi, dim3(og.back()), dim3(256), 0, b->stream,
-                       (const AaGather*)(base + at_g), (const int*)(base + at_og), (int)g.size());
+                       (cmem<AaGather>*)(base + at_g), (const int*)(base + at_og), (int)g.size());
     if (!d.empty()) {
         hipLaunchKernelGGL((k_aa_dots_multi<256, kAaDotG>), dim3(od.back()), dim3(256), 0, b->stream,
-                           (const AaDots*)(base + at_d), (const int*)(base + at_od), (int)d.size());
-        hipLaunchKernelGGL(k_aa_reduce_multi, dim3(orr.back()), dim3(256), 0, b->stream, (const AaDots*)(base + at_d),
+                           (cmem<AaDots>*)(base + at_d), (const int*)(base + at_od), (int)d.size());
+        hipLaunchKernelGGL(k_aa_reduce_multi, dim3(orr.back()), dim3(256), 0, b->stream, (cmem<AaDots>*)(base + at_d),
                            (const int*)(base + at_or), (int)d.size());
     }
     BCHK(hipGetLastError());
@@ -2846,7 +2873,7 @@ static int batch_aa_update(bte_batch* b) {
     AaUpdate* hu = (AaUpdate*)(h.data() + at_u);
     for (size_t k = 0; k < u.size(); k++) hu[k].gamma = (const double*)(base + at_c) + gat[k];
     BCHK(hipMemcpyAsync(b->aa_second.p, h.data(), h.size(), hipMemcpyHostToDevice, b->stream));
-    hipLaunchKernelGGL(k_aa_update_multi, dim3(ou.back()), dim3(256), 0, b->stream, (const AaUpdate*)(base + at_u),
+    hipLaunchKernelGGL(k_aa_update_multi, dim3(ou.back()), dim3(256), 0, b->stream, (cmem<AaUpdate>*)(base + at_u),
                        (const int*)(base + at_o), (int)u.size());
     BCHK(hipGetLastError());
     return BTE_OK;

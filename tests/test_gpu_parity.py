@@ -123,7 +123,7 @@ def test_sweep_variants_and_block_sizes_agree(name):
     say("variants", name)
     c = capi.Case(**case_kwargs(name))
     base = c.solve(sweep_variant=1, block=256, stop_margin=1e-4)
-    for variant in (2, 3, 4, 5, 6, 7):
+    for variant in (2, 3, 4, 5, 6, 7, 8):
         for block in (64, 128, 256, 512):
             r = c.solve(sweep_variant=variant, block=block, stop_margin=1e-4)
             st = c.stats()
@@ -133,19 +133,20 @@ def test_sweep_variants_and_block_sizes_agree(name):
             np.testing.assert_allclose(r["flux"], base["flux"], rtol=1e-9, atol=1e-12 * np.abs(base["flux"]).max())
     r = c.solve(stop_margin=1e-4)  # autotuned
     st = c.stats()
-    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3, 4, 5, 6, 7)
+    assert st["block"] in (64, 128, 256, 512) and st["variant"] in (2, 3, 4, 5, 6, 7, 8)
     np.testing.assert_allclose(r["T"], base["T"], rtol=1e-13)
 
 
 @pytest.mark.parametrize("variant,block,n_cells", [(4, 64, 20000), (4, 256, 20000), (5, 64, 20000),
                                                    (4, 256, 50000), (5, 64, 50000), (6, 128, 20000),
-                                                   (6, 256, 50000), (7, 256, 20000), (7, 512, 50000)])
+                                                   (6, 256, 50000), (7, 256, 20000), (7, 512, 50000),
+                                                   (8, 128, 20000), (8, 256, 50000)])
 def test_ring_sweep_matches_level_sweep(tmp_path, variant, block, n_cells):
     """The pipelined LDS-ring sweeps (variant 4: work-group per system; 5: one wave per system;
-    6 / 7: 2 / 4 bands of a direction per work-group, 10 bands so the last group of 4 is short)
-    perform the level-ordered sweep's arithmetic in the same order: identical iterates, including
-    levels wider than the pipelined lanes (finished unpipelined) and, at 50k cells, upwind values
-    older than the ring (read back from HBM)."""
+    6 / 7: 2 / 4 bands of a direction per work-group, 10 bands so the last group of 4 is short;
+    8: the wide ring of 16384 values) perform the level-ordered sweep's arithmetic in the same
+    order: identical iterates, including levels wider than the pipelined lanes (finished
+    unpipelined) and, at 50k cells, upwind values older than the ring (read back from HBM)."""
     say("ring sweep", variant, block, n_cells)
     p = meshgen.make_square_case(str(tmp_path), n_cells, 2, 10, 2, L=1e-7)
     c = capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=1e-7, Ly=1e-7)
@@ -155,6 +156,30 @@ def test_ring_sweep_matches_level_sweep(tmp_path, variant, block, n_cells):
     assert a["iters"] == b["iters"]
     np.testing.assert_array_equal(a["T"], b["T"])
     np.testing.assert_array_equal(a["margins"], b["margins"])
+
+
+def test_wide_ring_sweep_matches_level_sweep_in_3d(tmp_path):
+    """Variant 8 (the ring of 16384 values) on a 24,576-tet cube with 4 bands: a few upwind values
+    are older than even the wide ring, so its own level flags and HBM read-back run.  Band-major,
+    identical iterates to the level sweep and to variant 4; with the interleaved layout (the 3D
+    default) the same up to the margins' summation order."""
+    say("wide ring 3D")
+    p = meshgen.make_cube_case(str(tmp_path), 16, 3, 4, 3, L=1e-7)
+    c = capi.Case(p["mesh"], p["band"], p["bc"], 3, 4, 4, 300.0, Lx=1e-7, Ly=1e-7, Lz=1e-7)
+    kw = dict(block=256, max_iter=25, intensity_layout=1)
+    a = c.solve(sweep_variant=2, **kw)
+    for v in (4, 8):
+        b = c.solve(sweep_variant=v, **kw)
+        assert c.stats()["variant"] == v
+        assert a["iters"] == b["iters"]
+        np.testing.assert_array_equal(a["T"], b["T"])
+        np.testing.assert_array_equal(a["margins"], b["margins"])
+        np.testing.assert_array_equal(a["flux"], b["flux"])
+    i = c.solve(sweep_variant=8, block=512, max_iter=25)
+    assert c.stats()["variant"] == 8 and c.stats()["intensity_layout"] == 2
+    np.testing.assert_array_equal(a["T"], i["T"])
+    np.testing.assert_allclose(i["margins"], a["margins"], rtol=1e-12, atol=0)
+    c.close()
 
 
 def test_default_stop_and_repeat_determinism():
