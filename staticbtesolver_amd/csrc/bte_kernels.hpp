@@ -217,55 +217,80 @@ struct Bands {
 
 // Diffuse (type 2) re-emitted intensity of each boundary face, StaticBTESolver.cpp:180-204.  It
 // does not depend on the incoming direction, so it is formed once per (band, face) and iteration.
+// With the bands of a cell interleaved in the intensity arrays (ee.sb == 1): one thread per (face,
+// group of 4 bands), the groups of a face in neighbouring lanes, so that a group's four
+// intensities of a direction are one 32-byte sector (the face's groups together read the cell's
+// whole record) and the outgoing test and the weights are formed once for the group.  Band-major:
+// one thread per (band, face), faces in neighbouring lanes (more threads for the scattered loads).
+// Each band sums its outgoing terms in direction order either way.
 // Body over work-group blk of nblk (blockDim.x threads each); k_diffuse runs it over the grid.
-template <class BndT>
-__device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const BndT& bd, int B,
-                                              const double* __restrict__ ee_prev, double* __restrict__ ediff, int blk,
-                                              int nblk) {
-    const int total = B * bd.n_bf;
+template <int G, class BndT>
+__device__ __forceinline__ void diffuse_range_g(const Geo& g, const Quad& q, const BndT& bd, int B,
+                                                const double* __restrict__ ee_prev, double* __restrict__ ediff,
+                                                int blk, int nblk) {
+    const int ng = (B + G - 1) / G;
+    const int total = ng * bd.n_bf;
     for (int t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
-        const int b = t / bd.n_bf, bf = t % bd.n_bf;
+        const int bf = G == 1 ? t % bd.n_bf : t / ng;
+        const int b0 = G == 1 ? t / bd.n_bf : (t % ng) * G, nbg = min(G, B - b0);
         const int c = bd.bf_cell[bf], f = bd.bf_face[bf];
         const int nb = g.nbr[(size_t)c * g.F + f];
         bool diffuse = false;
         for (int k = 0; k < bd.nbc; k++) diffuse |= (bd.bci[k] == nb && bd.bct[k] == 2);
         if (!diffuse) {
-            ediff[t] = 0;
+            for (int j = 0; j < nbg; j++) ediff[(size_t)(b0 + j) * bd.n_bf + bf] = 0;
             continue;
         }
         const double* n = g.normal + 3 * ((size_t)c * g.F + f);
-        double es = 0, tw = 0;
+        double es[G] = {}, tw = 0;
         // the intensities of 8 directions are loaded together (a load under the outgoing test
         // would make each wait for the last); the sums take the outgoing terms in direction order
         constexpr int K = 8;
         for (int d0 = 0; d0 < q.D; d0 += K) {
-            double ev[K];
+            double ev[K][G];
 #pragma unroll
             for (int k = 0; k < K; k++)
-                ev[k] = d0 + k < q.D ? ee_prev[q.ee.at(b, d0 + k, c)] : 0.0;
+#pragma unroll
+                for (int j = 0; j < G; j++)
+                    ev[k][j] = (d0 + k < q.D && j < nbg) ? ee_prev[q.ee.at(b0 + j, d0 + k, c)] : 0.0;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 const int d2 = d0 + k;
                 if (d2 >= q.D) break;
                 const double dn = dot_nc(q.dir + 3 * d2, n);
                 if (dn > 0) {
-                    const double e = ev[k];
                     if (g.dim == 3) {
                         const double sn = dot_nc(q.S + 3 * d2, n);
-                        es += e * sn * q.omega[d2];
+#pragma unroll
+                        for (int j = 0; j < G; j++) es[j] += ev[k][j] * sn * q.omega[d2];
                         tw += sn * q.omega[d2];
                     } else if (g.dim == 2) {
                         const double sn = dot_nc(q.S + 3 * d2, n);
-                        es += e * sn;
+#pragma unroll
+                        for (int j = 0; j < G; j++) es[j] += ev[k][j] * sn;
                         tw += sn;
                     } else {
-                        es += e * dn * q.omega[d2];
+#pragma unroll
+                        for (int j = 0; j < G; j++) es[j] += ev[k][j] * dn * q.omega[d2];
                     }
                 }
             }
         }
-        ediff[t] = (g.dim != 1) ? es / tw : es / kPi;
+        for (int j = 0; j < nbg; j++)
+            ediff[(size_t)(b0 + j) * bd.n_bf + bf] = (g.dim != 1) ? es[j] / tw : es[j] / kPi;
     }
+}
+
+__host__ __device__ inline int diffuse_group(const EeIdx& ei) { return ei.sb == 1 ? 4 : 1; }
+
+template <class BndT>
+__device__ __forceinline__ void diffuse_range(const Geo& g, const Quad& q, const BndT& bd, int B,
+                                              const double* __restrict__ ee_prev, double* __restrict__ ediff, int blk,
+                                              int nblk) {
+    if (diffuse_group(q.ee) == 4)
+        diffuse_range_g<4>(g, q, bd, B, ee_prev, ediff, blk, nblk);
+    else
+        diffuse_range_g<1>(g, q, bd, B, ee_prev, ediff, blk, nblk);
 }
 
 __global__ void k_diffuse(Geo g, Quad q, Bnd bd, int B, const double* __restrict__ ee_prev,
@@ -363,6 +388,8 @@ struct Ops {
     const double* vol;
     const double* omega;  // w_d (already 1 in 1D)
     const double* vt;     // [B] v_b * tau_b
+    const double* band_C;  // [B] C_b: emission density C_b / solid_angle * (T - T_ref)
+    double solid_angle, T_ref;
     const double* dg;     // [D*N]
     const int* ell_col;   // [(d*W+k)*N + c]
     const double* ell_val;
@@ -406,6 +433,7 @@ struct Ops {
 struct Work {
     double* x;     // ee_curr [S*N]
     double* b;     // rhs [S*N]
+    const double* tlo;  // [D][Np] source temperature of the cell at each level-order position
     double* r64;   // FP64 residual (mixed precision)
     void* r;       // Krylov vectors of type R [S*N] each
     void* rh;
@@ -999,6 +1027,33 @@ __global__ void k_rhs_levels(Ops o, int B, const double* __restrict__ dens, doub
     rhs_levels_range(o, B, dens, rhs_lo, blockIdx.x, gridDim.x);
 }
 
+// Source temperatures in level order, tlo[d * Np + i] = tsrc[cell at position i of direction d],
+// for the sweeps that form the interior right-hand sides themselves: 8 bytes per position shared by
+// all bands, against the B right-hand sides k_rhs_levels stores.
+__device__ __forceinline__ void tsrc_levels_range(const Ops& o, const double* __restrict__ tsrc,
+                                                  double* __restrict__ tlo, int blk, int nblk) {
+    const int N = o.N, Np = o.Np;
+    const long long total = (long long)o.D * Np;
+    for (long long t = blk * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)nblk * blockDim.x) {
+        const int d = (int)(t / Np), i = (int)(t % Np);
+        if (i < N) tlo[t] = tsrc[o.perm[(size_t)d * N + i]];
+    }
+}
+
+__global__ void k_tsrc_levels(Ops o, const double* __restrict__ tsrc, double* __restrict__ tlo) {
+    tsrc_levels_range(o, tsrc, tlo, blockIdx.x, gridDim.x);
+}
+
+// The diffuse re-emission (work-groups [0, n_diff)) and the level-order source temperatures.
+__global__ void k_diffuse_tsrc_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const double* __restrict__ ee_prev,
+                                      double* __restrict__ ediff, const double* __restrict__ tsrc,
+                                      double* __restrict__ tlo, int n_diff) {
+    if ((int)blockIdx.x < n_diff)
+        diffuse_range(g, q, bd, B, ee_prev, ediff, blockIdx.x, n_diff);
+    else
+        tsrc_levels_range(o, tsrc, tlo, blockIdx.x - n_diff, gridDim.x - n_diff);
+}
+
 // The diffuse re-emission (work-groups [0, n_diff)) and the interior level-order right-hand sides
 // (the rest) in one launch: the two are independent, so the small diffuse pass runs beside the
 // larger one.
@@ -1013,16 +1068,27 @@ __global__ void k_diffuse_rhs_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const 
 
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
 // their face loops run in work-groups of their own instead of stalling the interior waves; one
-// thread per (list entry, band).
+// thread per (list entry, band).  A work-group takes blockDim.x consecutive entries of one band (its
+// stores coalesced); the B work-groups of the same entries follow one another on one XCD (virtual
+// work-group v runs on XCD v % 8 when the grid is a multiple of 8), so the cells' geometry,
+// densities and incoming intensities they all read come from that XCD's L2 after the first band.
+__host__ __device__ inline long long rhs_bnd_units(int n_list, int B, int E) {
+    const long long chunks = (n_list + E - 1) / E;
+    return 8 * (long long)B * ((chunks + 7) / 8);
+}
+
 template <class BndT>
 __device__ __forceinline__ void rhs_bnd_range(const Geo& g, const Quad& q, const BndT& bd, const Bands& bands,
                                               const Ops& o, int S, const int* __restrict__ list, int n_list,
                                               const double* __restrict__ ee_prev, const double* __restrict__ dens,
                                               double* __restrict__ rhs_lo, int blk, int nblk) {
-    const int N = o.N, D = o.D, B = S / D;
-    const long long total = (long long)n_list * B;
-    for (long long u = blk * (long long)blockDim.x + threadIdx.x; u < total; u += (long long)nblk * blockDim.x) {
-        const int b = (int)(u / n_list), k = (int)(u % n_list);
+    const int N = o.N, D = o.D, B = S / D, E = blockDim.x;
+    const long long V = rhs_bnd_units(n_list, B, E);
+    for (long long v = blk; v < V; v += nblk) {
+        const long long r = v / 8;
+        const int b = (int)(r % B);
+        const long long k = ((r / B) * 8 + v % 8) * E + threadIdx.x;
+        if (k >= n_list) continue;
         const int t = list[k];
         const int d = t / N, i = t % N;
         rhs_lo[((size_t)b * D + d) * o.Np + i] = rhs_at(g, q, bd, bands, b, d, o.perm[t], ee_prev, dens);
@@ -1108,13 +1174,15 @@ __device__ __forceinline__ void buf_st(T v, __amdgpu_buffer_rsrc_t r, unsigned v
 // Operands of one cell of the pipelined sweeps from the packed records (k_pack_sweep).
 struct PkItem {
     int i, c;     // level-order position (-1: no cell for this thread) and cell
+    bool bnd;     // the cell has an incoming boundary face (loaded by the TS sweeps only)
     int p[3];     // 8 x level-order positions of the upwind neighbours (absent: INT_MAX)
     double a[3];  // their coefficients (absent: 0)
     double dgc, vw, rhs;
+    double ts;    // source temperature (sweeps that form interior right-hand sides themselves)
 };
 
 struct PkSrc {
-    __amdgpu_buffer_rsrc_t pki, pkd, scol, svw, rhs, x;
+    __amdgpu_buffer_rsrc_t pki, pkd, scol, svw, rhs, x, tlo, sbnd;
     unsigned n4;   // byte distance between the upwind-cell slots of swp_col: N * 4
     unsigned xs8;  // byte distance between the intensities of neighbouring cells in x (o.ee.sc * 8)
 };
@@ -1128,6 +1196,8 @@ __device__ __forceinline__ PkSrc pk_src(const Ops& o, const Work& w, int s, int 
     r.scol = buffer_of(o.swp_col + d * W * N, W * N * 4);
     r.svw = buffer_of(o.swp_vw + d * N, N * 8);
     r.rhs = buffer_of(w.b + (size_t)s * o.Np, N * 8);
+    r.tlo = buffer_of(w.tlo ? w.tlo + d * (size_t)o.Np : w.b, N * 8);
+    r.sbnd = buffer_of(o.swp_bnd + d * N, N);
     const long long sc = o.ee.sc;
     r.x = buffer_of(w.x + o.ee.at(s / o.D, d, 0), (size_t)((N - 1) * sc + 1) * 8);
     r.n4 = (unsigned)(N * 4);
@@ -1139,8 +1209,10 @@ typedef unsigned int bte_u32x4 __attribute__((ext_vector_type(4)));
 typedef double bte_f64x2 __attribute__((ext_vector_type(2)));
 
 // Three 16-byte loads (four with W = 3) and the right-hand side, all at the lane's position; an
-// absent item (i < 0) loads position 0 and is ignored.
-template <int W, bool RHS = true>
+// absent item (i < 0) loads position 0 and is ignored.  TS: also the source temperature (shared
+// by the bands of the direction, so a plain cached load) and whether the cell has an incoming
+// boundary face (swp_bnd).
+template <int W, bool RHS = true, bool TS = false>
 __device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it) {
     it.i = i;
     const unsigned j = (unsigned)(i < 0 ? 0 : i);
@@ -1171,6 +1243,10 @@ __device__ __forceinline__ void pk_item_load(int i, const PkSrc& src, PkItem& it
         it.vw = buf_ld<double>(src.svw, j * 8u, 0);
     }
     if constexpr (RHS) it.rhs = buf_ld<double, 2>(src.rhs, j * 8u, 0);
+    if constexpr (TS) {
+        it.ts = buf_ld<double>(src.tlo, j * 8u, 0);
+        it.bnd = __builtin_amdgcn_raw_buffer_load_b8(src.sbnd, j, 0, 0) != 0;
+    }
 }
 
 // Upwind sum of a pipelined-sweep cell, sum_k a_k x(p_k) in slot order over the W slots (the
@@ -1228,8 +1304,11 @@ __host__ __device__ inline size_t ring_lds_bytes(int ring, int max_levels) {
 // levels synchronise on the LDS only, leaving the prefetches in flight.  FB = false: no direction
 // of the launch has such reads (no HBM read-back path, no full synchronisation); with FB = true a
 // direction without them (its o.dir_fb bit clear) still runs the FB = false code.  o.ring_sel picks
-// the flags computed for the launch's ring (the regular or the wide one).
-template <int BLOCK, int W, bool FB, int PD = 3>
+// the flags computed for the launch's ring (the regular or the wide one).  TS: w.b holds the
+// right-hand sides of the cells with an incoming boundary face only; an interior cell's is formed
+// here from its source temperature (w.tlo), (C_b / solid_angle * (T - T_ref)) * w_d V_c -- the
+// emission density times the volume weight, rounded as k_rhs_levels forms it.
+template <int BLOCK, int W, bool FB, int PD = 3, bool TS = false>
 __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
     const int tid = threadIdx.x;
     const int R = ring_mask + 1;
@@ -1237,6 +1316,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
     {
         const int b = s / o.D, d = s % o.D;
         const double vt = o.vt[b];
+        const double cb = TS ? o.band_C[b] / o.solid_angle : 0.0, t_ref = o.T_ref;
         gmem<double>* __restrict__ xg = to_global(w.x + o.ee.at(b, d, 0));
         const PkSrc src = pk_src<W>(o, w, s, d);
         const int* __restrict__ lp = o.lvl_ptr + o.lvl_off[d];
@@ -1281,7 +1361,9 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         auto solve_cell = [&](const PkItem& it, int end, auto hb) {
             const double acc =
                 pk_upwind_sum<W, decltype(hb)::value>(it, (end - R) * 8, ring, ring_mask * 8, xg, src);
-            const double xv = (it.rhs - vt * acc) / (vt * it.dgc + it.vw);
+            double rhs = it.rhs;
+            if constexpr (TS) rhs = it.bnd ? it.rhs : cb * (it.ts - t_ref) * it.vw;
+            const double xv = (rhs - vt * acc) / (vt * it.dgc + it.vw);
             ring[it.i & ring_mask] = xv;
             buf_st<double>(xv, src.x, (unsigned)it.c * src.xs8, 0);
         };
@@ -1289,7 +1371,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
             if (cur.i >= 0) solve_cell(cur, end, hb);
             for (int i = beg + BLOCK + tid; i < end; i += BLOCK) {
                 PkItem E;
-                pk_item_load<W>(i, src, E);
+                pk_item_load<W, true, TS>(i, src, E);
                 solve_cell(E, end, hb);
             }
         };
@@ -1299,7 +1381,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         auto level = [&](int l, PkItem& cur, PkItem& ahead) {
             if (l + PD + 1 - wb > 63) load_window(l);
             const int beg = lp_at(l), end = lp_at(l + 1);
-            pk_item_load<W>(first(l + PD), src, ahead);
+            pk_item_load<W, true, TS>(first(l + PD), src, ahead);
             if (FB && ((olds >> (l - wb)) & 1ull))
                 solve_level(cur, beg, end, BoolC<true>{});
             else
@@ -1311,7 +1393,7 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
         };
         PkItem X[PD + 1];
 #pragma unroll
-        for (int k = 0; k < PD; k++) pk_item_load<W>(first(k), src, X[k]);
+        for (int k = 0; k < PD; k++) pk_item_load<W, true, TS>(first(k), src, X[k]);
         X[PD].i = -1;
         // Steady state: every copy of the unrolled body runs, so on every path into a level the
         // compiler sees its item loaded PD levels earlier and waits for those loads only (a copy
@@ -1330,12 +1412,12 @@ __device__ __forceinline__ void sweep_ring_system(const Ops& o, const Work& w, i
 }
 
 // FB = true: the HBM read-back path only for a direction that has such reads
-template <int BLOCK, int W, bool FB, int PD = 3>
+template <int BLOCK, int W, bool FB, int PD = 3, bool TS = false>
 __device__ __forceinline__ void sweep_ring_dispatch(const Ops& o, const Work& w, int s, double* ring, int ring_mask) {
     if (FB && (((o.dir_fb[s % o.D] >> o.ring_sel) & 1) || ring_mask + 1 < o.flag_ring()))
-        sweep_ring_system<BLOCK, W, true, PD>(o, w, s, ring, ring_mask);
+        sweep_ring_system<BLOCK, W, true, PD, TS>(o, w, s, ring, ring_mask);
     else
-        sweep_ring_system<BLOCK, W, false, PD>(o, w, s, ring, ring_mask);
+        sweep_ring_system<BLOCK, W, false, PD, TS>(o, w, s, ring, ring_mask);
 }
 
 // Accelerator complex dies (XCDs) of the MI355X; work-group blk of a launch runs on XCD blk % 8, each
@@ -1353,12 +1435,13 @@ __device__ __forceinline__ int system_of_block(int blk, int B, int D) {
     return (r % B) * D + (r / B) * kXcds + x;
 }
 
-// W = incoming slots per cell (F - 1: 1 in 1D, 2 in 2D, 3 in 3D)
-template <int BLOCK, int W, bool FB, int PD = 3>
+// W = incoming slots per cell (F - 1: 1 in 1D, 2 in 2D, 3 in 3D); TS: interior right-hand sides
+// from the source temperatures (sweep_ring_system)
+template <int BLOCK, int W, bool FB, bool TS = false, int PD = 3>
 __global__ __launch_bounds__(BLOCK) void k_sweep_ring(Ops o, Work w, int S, int ring_mask) {
     extern __shared__ double ring[];
     for (int blk = blockIdx.x; blk < S; blk += gridDim.x)
-        sweep_ring_dispatch<BLOCK, W, FB, PD>(o, w, system_of_block(blk, S / o.D, o.D), ring, ring_mask);
+        sweep_ring_dispatch<BLOCK, W, FB, PD, TS>(o, w, system_of_block(blk, S / o.D, o.D), ring, ring_mask);
 }
 
 // Band-batched pipelined sweep (variants 6 and 7): one work-group sweeps G bands of one direction.
@@ -1743,8 +1826,10 @@ __global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, EeId
 }
 
 // Equilibrium temperature and emission density per cell (StaticBTESolver.cpp:106-120).
+// tsrc: the temperature each cell's emission density was formed from (the source temperature)
 __device__ __forceinline__ void tcell_range(int N, const Bands& bands, const double* __restrict__ Tband,
-                                            double* __restrict__ T, double* __restrict__ dens, int blk, int nblk) {
+                                            double* __restrict__ T, double* __restrict__ dens,
+                                            double* __restrict__ tsrc, int blk, int nblk) {
     for (int c = blk * blockDim.x + threadIdx.x; c < N; c += nblk * blockDim.x) {
         double rrn = 0, rnn = 0;
         for (int b = 0; b < bands.B; b++) {
@@ -1753,13 +1838,14 @@ __device__ __forceinline__ void tcell_range(int N, const Bands& bands, const dou
         }
         const double tc = rrn / rnn;
         T[c] = tc;
+        tsrc[c] = tc;
         for (int b = 0; b < bands.B; b++) dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (tc - bands.T_ref);
     }
 }
 
 __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, double* __restrict__ T,
-                        double* __restrict__ dens) {
-    tcell_range(N, bands, Tband, T, dens, blockIdx.x, gridDim.x);
+                        double* __restrict__ dens, double* __restrict__ tsrc) {
+    tcell_range(N, bands, Tband, T, dens, tsrc, blockIdx.x, gridDim.x);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1890,6 +1976,7 @@ struct AaGather {
     float* dG;
     Bands bands;
     double* dens;
+    double* tsrc;
 };
 
 // Body over work-group blk of nblk (blockDim.x threads each).  Every load of an entry precedes its
@@ -1930,6 +2017,7 @@ __device__ __forceinline__ void gather_diff_range(const AaGather& a, int blk, in
             a.x[t] = gv;
             if (a.dens) {
                 if (t < N) {
+                    a.tsrc[t] = gv;
                     for (int b = 0; b < a.bands.B; b++)
                         a.dens[(size_t)t * a.bands.B + b] = a.bands.C[b] / a.bands.solid_angle * (gv - a.bands.T_ref);
                 } else {
@@ -1998,7 +2086,8 @@ __global__ void k_learn_features(int N, const double* __restrict__ f, const doub
 __global__ void k_apply_correction(int N, int D, int S, int nb, EeIdx ei, const int* __restrict__ bcells,
                                    const float* __restrict__ y, double span, const double* __restrict__ ff,
                                    double* __restrict__ z, double* __restrict__ backup, Bands bands,
-                                   double* __restrict__ dens, const double* __restrict__ sc, double* __restrict__ ee) {
+                                   double* __restrict__ dens, double* __restrict__ tsrc, const double* __restrict__ sc,
+                                   double* __restrict__ ee) {
     const bool apply = ff[0] > 0 && isfinite(ff[1]);
     const long long total = N + (long long)S * nb;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
@@ -2010,6 +2099,7 @@ __global__ void k_apply_correction(int N, int D, int S, int nb, EeIdx ei, const 
         const double v = v0 + span * (double)y[c];
         z[t] = v;
         if (t < N) {
+            tsrc[t] = v;
             for (int b = 0; b < bands.B; b++)
                 dens[(size_t)t * bands.B + b] = bands.C[b] / bands.solid_angle * (v - bands.T_ref);
         } else {
@@ -2121,6 +2211,7 @@ struct AaUpdate {
     const double* sc;
     double* ee;
     const double* gamma;  // the coefficients in device memory (the multi-problem launch)
+    double* tsrc;
 };
 
 // The mixing update x = g - sum_j gamma_j (dX_j + dF_j) over the Z = N + S nb state entries (the
@@ -2138,6 +2229,7 @@ __device__ __forceinline__ void aa_update_range(const AaUpdate& a, const Coef& g
         for (int j = 0; j < a.ncols; j++) v -= gamma[j] * (double)dG[(size_t)j * Z + c];
         a.x[c] = v;
         if (c < a.N) {
+            a.tsrc[c] = v;
             for (int b = 0; b < a.bands.B; b++)
                 a.dens[(size_t)c * a.bands.B + b] = a.bands.C[b] / a.bands.solid_angle * (v - a.bands.T_ref);
         } else {
@@ -2152,10 +2244,13 @@ __device__ __forceinline__ void aa_update_range(const AaUpdate& a, const Coef& g
 __global__ void k_aa_update_state(AaUpdate a, AaCoef gamma) { aa_update_range(a, gamma, blockIdx.x, gridDim.x); }
 
 // Emission density of a given temperature field (StaticBTESolver.cpp:115-119), cell-major [N][B].
-__global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x)
+__global__ void k_set_density(int N, Bands bands, const double* __restrict__ T, double* __restrict__ dens,
+                              double* __restrict__ tsrc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+        tsrc[c] = T[c];
         for (int b = 0; b < bands.B; b++)
             dens[(size_t)c * bands.B + b] = bands.C[b] / bands.solid_angle * (T[c] - bands.T_ref);
+    }
 }
 
 // Sum of the block partials in a fixed order; margin = sum / (N_band * N_cell).  host (or null):
@@ -2179,11 +2274,12 @@ __global__ void k_margin(int nparts, const double* __restrict__ partial, double 
 // k_tcell over work-groups [0, n_tcell) and k_margin as work-group n_tcell, in one launch: both
 // read only the band-temperature step's results.
 __global__ __launch_bounds__(256) void k_tcell_margin(int N, Bands bands, const double* __restrict__ Tband,
-                                                      double* __restrict__ T, double* __restrict__ dens, int n_tcell,
-                                                      int nparts, const double* __restrict__ partial, double scale,
+                                                      double* __restrict__ T, double* __restrict__ dens,
+                                                      double* __restrict__ tsrc, int n_tcell, int nparts,
+                                                      const double* __restrict__ partial, double scale,
                                                       double* __restrict__ out, double* __restrict__ host) {
     if ((int)blockIdx.x < n_tcell)
-        tcell_range(N, bands, Tband, T, dens, blockIdx.x, n_tcell);
+        tcell_range(N, bands, Tband, T, dens, tsrc, blockIdx.x, n_tcell);
     else
         margin_block(nparts, partial, scale, out, host);
 }
@@ -2209,6 +2305,7 @@ struct IterDesc {
     double* Tband;
     double* T;
     double* dens;
+    double* tsrc;
     double* partial;
     double* margin;
     const double* omega;
@@ -2268,7 +2365,7 @@ __global__ __launch_bounds__(256) void k_tband_multi(cmem<IterDesc>* __restrict_
 __global__ void k_tcell_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
     const IterDesc D = cload(desc + m.p);
-    tcell_range(D.o.N, D.bands, D.Tband, D.T, D.dens, m.blk, m.nblk);
+    tcell_range(D.o.N, D.bands, D.Tband, D.T, D.dens, D.tsrc, m.blk, m.nblk);
 }
 
 // one work-group per problem

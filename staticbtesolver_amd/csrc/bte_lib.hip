@@ -108,6 +108,11 @@ struct bte_ctx {
     bool have_sweep = false, have_fp32 = false;
     // state + work
     DBuf eeA, eeB, dens, T, Tband, partial, margin, bflux;
+    // source temperatures: tsrc[c] is the temperature the emission density of cell c was formed
+    // from (every writer of dens stores it), tlo the same in level order per direction; tsrc_ok
+    // while that holds (a density uploaded as such breaks it until the next bte_begin)
+    DBuf tsrc, tlo;
+    bool tsrc_ok = false;
     DBuf wb, wr64, wr, wrh, wp, wv, wph, wsh, wt, wy, wf, iters, fails;
     double* ee_final = nullptr;  // which of eeA/eeB holds the last iterate
     // layout of the intensity arrays (band-major, or interleaved for a pipelined 3D sweep: set by
@@ -210,6 +215,19 @@ hipError_t upload(DBuf& d, const std::vector<T>& h, hipStream_t s) {
 int grid_for(long long n, int block) {
     long long g = (n + block - 1) / block;
     return (int)std::max<long long>(1, std::min<long long>(g, 65536));
+}
+
+// Work-groups of the diffuse re-emission pass (one thread per face and group of diffuse_group bands
+// of the intensity layout in use).
+int diffuse_grid(const bte_ctx* c) {
+    const int G = diffuse_group(c->eei);
+    return grid_for((long long)((c->B + G - 1) / G) * c->n_bf, 256);
+}
+
+// Work-groups (64 threads) of the boundary right-hand-side pass: one per chunk of 64 list entries
+// and band, a multiple of 8 up to the grid cap.
+int rhs_bnd_grid(const bte_ctx* c) {
+    return (int)std::min<long long>(rhs_bnd_units(c->n_bnd_list, c->B, 64), 65536);
 }
 
 Geo geo_of(bte_ctx* c) {
@@ -935,7 +953,8 @@ int ensure_work(bte_ctx* ctx, int precision) {
         {&ctx->wr, 8 * SN}, {&ctx->wrh, 8 * SN},
         {&ctx->wp, 8 * SN}, {&ctx->wv, 8 * SN}, {&ctx->wph, 8 * SN}, {&ctx->wsh, 8 * SN}, {&ctx->wt, 8 * SN},
         {&ctx->iters, sizeof(int) * S}, {&ctx->fails, sizeof(int) * S}, {&ctx->dens, 8 * BN},
-        {&ctx->T, 8 * (size_t)ctx->N}, {&ctx->Tband, 8 * BN},
+        {&ctx->T, 8 * (size_t)ctx->N}, {&ctx->Tband, 8 * BN}, {&ctx->tsrc, 8 * (size_t)ctx->N},
+        {&ctx->tlo, 8 * (size_t)ctx->D * lo_stride(ctx->N)},
         {&ctx->ediff, 8 * std::max<size_t>(1, (size_t)ctx->B * ctx->n_bf)}};
     if (precision == BTE_PREC_MIXED) {
         want.push_back({&ctx->wr64, 8 * SN});
@@ -966,6 +985,9 @@ Ops ops_of(bte_ctx* ctx) {
     o.vol = ctx->vol.as<double>();
     o.omega = ctx->w_eff.as<double>();
     o.vt = ctx->vt.as<double>();
+    o.band_C = ctx->bC.as<double>();
+    o.solid_angle = ctx->solid_angle;
+    o.T_ref = ctx->T_ref;
     o.dg = ctx->dg.as<double>();
     o.ell_col = ctx->ell_col.as<int>();
     o.ell_val = ctx->ell_val.as<double>();
@@ -1000,6 +1022,7 @@ Work work_of(bte_ctx* ctx, double* ee_curr) {
     Work w{};
     w.x = ee_curr;
     w.b = ctx->wb.as<double>();
+    w.tlo = ctx->tlo.as<double>();
     w.r64 = ctx->wr64.as<double>();
     w.r = ctx->wr.p;
     w.rh = ctx->wrh.p;
@@ -1083,7 +1106,7 @@ void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, con
     const int S = ctx->B * ctx->D;
     const int n_rhs = grid_for((long long)ctx->D * lo_stride(ctx->N), 256);
     if (with_diffuse) {
-        const int n_diff = grid_for((long long)ctx->B * ctx->n_bf, 256);
+        const int n_diff = diffuse_grid(ctx);
         hipLaunchKernelGGL(k_diffuse_rhs_levels, dim3((unsigned)(n_diff + n_rhs)), dim3(256), 0, st, geo_of(ctx),
                            quad_of(ctx), bnd_of(ctx), ops_of(ctx), ctx->B, ee_prev, ctx->ediff.as<double>(), dens,
                            out, n_diff);
@@ -1091,10 +1114,32 @@ void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, con
         hipLaunchKernelGGL(k_rhs_levels, dim3(n_rhs), dim3(256), 0, st, ops_of(ctx), ctx->B, dens, out);
     }
     if (ctx->n_bnd_list > 0)
-        hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(grid_for((long long)ctx->n_bnd_list * ctx->B, 64)), dim3(64), 0, st,
+        hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(rhs_bnd_grid(ctx)), dim3(64), 0, st,
                            geo_of(ctx),
                            quad_of(ctx), bnd_of(ctx), bands_of(ctx), ops_of(ctx), S, ctx->bnd_list.as<int>(),
                            ctx->n_bnd_list, ee_prev, dens, out);
+}
+
+// For the sweeps that form the interior right-hand sides themselves: the level-order source
+// temperatures (with the diffuse re-emission in the same launch when with_diffuse), then the
+// right-hand sides of the cells with an incoming boundary face into out, as enqueue_rhs_levels.
+void enqueue_tsrc_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, const double* dens, double* out,
+                         bool with_diffuse = false) {
+    const int S = ctx->B * ctx->D;
+    const int n_lo = grid_for((long long)ctx->D * lo_stride(ctx->N), 256);
+    if (with_diffuse) {
+        const int n_diff = diffuse_grid(ctx);
+        hipLaunchKernelGGL(k_diffuse_tsrc_levels, dim3((unsigned)(n_diff + n_lo)), dim3(256), 0, st, geo_of(ctx),
+                           quad_of(ctx), bnd_of(ctx), ops_of(ctx), ctx->B, ee_prev, ctx->ediff.as<double>(),
+                           ctx->tsrc.as<double>(), ctx->tlo.as<double>(), n_diff);
+    } else {
+        hipLaunchKernelGGL(k_tsrc_levels, dim3(n_lo), dim3(256), 0, st, ops_of(ctx), ctx->tsrc.as<double>(),
+                           ctx->tlo.as<double>());
+    }
+    if (ctx->n_bnd_list > 0)
+        hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(rhs_bnd_grid(ctx)), dim3(64), 0, st, geo_of(ctx), quad_of(ctx),
+                           bnd_of(ctx), bands_of(ctx), ops_of(ctx), S, ctx->bnd_list.as<int>(), ctx->n_bnd_list,
+                           ee_prev, dens, out);
 }
 
 // marks (null, or three events): recorded before the right-hand sides, between them and the inner
@@ -1120,7 +1165,7 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
     const bool v4 = direct_path(ctx, o) && ((variant == 4 && ring_size(ctx) > 0) || (variant == 8 && ctx->ring_r2 > 0));
     mark(marks, 0, ctx->stream);
     if (diffuse_first && !v4)
-        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diffuse, dim3(diffuse_grid(ctx)), dim3(256), 0, ctx->stream,
                            g, q, bd, ctx->B, ee_prev, ctx->ediff.as<double>());
     if (direct_path(ctx, o)) {
         int sync_every = 1;
@@ -1180,8 +1225,15 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
             ctx->variant_used = variant;
             Ops opr = op;
             opr.ring_sel = wide ? 1 : 0;
-            // right-hand sides in level order into w.b, then the pipelined sweep
-            enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b, diffuse_first);
+            // right-hand sides in level order into w.b, then the pipelined sweep.  Variant 8, while
+            // the source temperatures hold: only the boundary cells' ones, the sweep forming the
+            // interior ones (a few more instructions per cell, which the issue-bound sweep of a
+            // launch with few systems per compute unit, variant 4's case, does not recover)
+            const bool ts = wide && ctx->tsrc_ok && dens == ctx->dens.as<double>();
+            if (ts)
+                enqueue_tsrc_levels(ctx, ctx->stream, ee_prev, dens, w.b, diffuse_first);
+            else
+                enqueue_rhs_levels(ctx, ctx->stream, ee_prev, dens, w.b, diffuse_first);
             mark(marks, 1, ctx->stream);
             const size_t lds = ring_lds_bytes(R, ctx->max_levels);
             auto go = [&](auto kernel) {
@@ -1189,12 +1241,19 @@ void launch_solve_b(bte_ctx* ctx, const bte_opts* o, int variant, int S, const d
                 hipLaunchKernelGGL(kernel, grid, blk, lds, ctx->stream, opr, w, S, R - 1);
             };
             const bool fb = ctx->max_back > R;
-            if (ctx->W == 1)
-                fb ? go(k_sweep_ring<BLOCK, 1, true>) : go(k_sweep_ring<BLOCK, 1, false>);
-            else if (ctx->W == 2)
-                fb ? go(k_sweep_ring<BLOCK, 2, true>) : go(k_sweep_ring<BLOCK, 2, false>);
+            auto by_w = [&](auto tsc) {
+                constexpr bool TS = decltype(tsc)::value;
+                if (ctx->W == 1)
+                    fb ? go(k_sweep_ring<BLOCK, 1, true, TS>) : go(k_sweep_ring<BLOCK, 1, false, TS>);
+                else if (ctx->W == 2)
+                    fb ? go(k_sweep_ring<BLOCK, 2, true, TS>) : go(k_sweep_ring<BLOCK, 2, false, TS>);
+                else
+                    fb ? go(k_sweep_ring<BLOCK, 3, true, TS>) : go(k_sweep_ring<BLOCK, 3, false, TS>);
+            };
+            if (ts)
+                by_w(BoolC<true>{});
             else
-                fb ? go(k_sweep_ring<BLOCK, 3, true>) : go(k_sweep_ring<BLOCK, 3, false>);
+                by_w(BoolC<false>{});
         } else if (variant == 3 && lds_fits(ctx)) {
             ctx->variant_used = 3;
             mark(marks, 1, ctx->stream);  // these sweeps form their right-hand sides themselves
@@ -1296,7 +1355,8 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
         // cell temperatures and densities, and the margin, in one launch (both read the band step)
         const int n_tcell = grid_for(ctx->N, 256);
         hipLaunchKernelGGL(k_tcell_margin, dim3(n_tcell + 1), dim3(256), 0, ctx->stream, ctx->N, bn,
-                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>(), n_tcell, nblk,
+                           ctx->Tband.as<double>(), ctx->T.as<double>(), ctx->dens.as<double>(),
+                           ctx->tsrc.as<double>(), n_tcell, nblk,
                            partial_of(ctx), 1.0 / ctx->B / ctx->N, margin_of(ctx), ctx->d_sync);
     }
     HIPCHK(hipGetLastError());
@@ -1334,8 +1394,11 @@ int autotune(bte_ctx* ctx, const bte_opts* o, Shape& best) {
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)ctx->B * ctx->N, ctx->stream));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, ctx->T_ref,
+                       ctx->tsrc.as<double>());  // the zero density's source temperature
+    ctx->tsrc_ok = true;
     if (ctx->has_diffuse)
-        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diffuse, dim3(diffuse_grid(ctx)), dim3(256), 0, ctx->stream,
                            geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeA.as<double>(), ctx->ediff.as<double>());
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
@@ -1553,7 +1616,7 @@ static int aa_reset(bte_ctx* ctx) {
 static int load_state(bte_ctx* ctx) {
     const int N = ctx->N, S = ctx->B * ctx->D, nb = ctx->aa_nb;
     hipLaunchKernelGGL(k_set_density, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, bands_of(ctx),
-                       ctx->aa_x.as<double>(), ctx->dens.as<double>());
+                       ctx->aa_x.as<double>(), ctx->dens.as<double>(), ctx->tsrc.as<double>());
     if (nb > 0)
         hipLaunchKernelGGL(k_state_scatter, dim3(grid_for((long long)S * nb, 256)), dim3(256), 0, ctx->stream, N,
                            ctx->D, S, nb, ctx->eei, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->aa_x.as<double>(),
@@ -1593,7 +1656,8 @@ static int aa_launch(bte_ctx* ctx, bool* finish, AaRec* rec = nullptr) {
         AaGather a{N, ctx->D, S, nb, ctx->eei, ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(),
                    ctx->T.as<double>(), ctx->ee_final, ctx->aa_g.as<double>(), first, take, ctx->aa_x.as<double>(),
                    ctx->aa_xprev.as<double>(), ctx->aa_fprev.as<double>(), ctx->aa_f.as<double>(),
-                   ctx->aa_dX.as<double>(), dF, dG, bands_of(ctx), load ? ctx->dens.as<double>() : (double*)nullptr};
+                   ctx->aa_dX.as<double>(), dF, dG, bands_of(ctx), load ? ctx->dens.as<double>() : (double*)nullptr,
+                   ctx->tsrc.as<double>()};
         if (rec) {
             rec->gather = true;
             rec->ga = a;
@@ -1683,7 +1747,8 @@ static int aa_finish(bte_ctx* ctx, AaRec* rec = nullptr) {
     // the update and the state's density and boundary intensities in one pass (load_state's work)
     const AaUpdate u{ctx->N, ctx->D, ctx->B * ctx->D, ctx->aa_nb, ctx->eei, nc, ctx->aa_g.as<double>(),
                      ctx->aa_dG.as<float>(), ctx->aa_x.as<double>(), bands_of(ctx), ctx->dens.as<double>(),
-                     ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final, nullptr};
+                     ctx->aa_bcells.as<int>(), ctx->aa_sc.as<double>(), ctx->ee_final, nullptr,
+                     ctx->tsrc.as<double>()};
     if (rec) {
         rec->update = true;
         rec->ua = u;
@@ -1901,7 +1966,7 @@ static int learn_propose(bte_ctx* ctx) {
     hipLaunchKernelGGL(k_apply_correction, dim3(grid_for(Z, 256)), dim3(256), 0, ctx->stream, N, ctx->D,
                        ctx->B * ctx->D, ctx->aa_nb, ctx->eei, ctx->aa_bcells.as<int>(), net->y.as<float>(), ctx->learn_span,
                        red_out_of(ctx), ctx->aa_x.as<double>(), ctx->learn_backup.as<double>(), bands_of(ctx),
-                       ctx->dens.as<double>(), ctx->aa_sc.as<double>(), ctx->ee_final);
+                       ctx->dens.as<double>(), ctx->tsrc.as<double>(), ctx->aa_sc.as<double>(), ctx->ee_final);
     HIPCHK(hipGetLastError());
     ctx->learn_pending = true;
     ctx->run_proposed++;
@@ -2032,6 +2097,9 @@ int bte_begin(bte_ctx* ctx, const bte_opts* opts) {
     HIPCHK(hipMemsetAsync(ctx->eeA.p, 0, ctx->eeA.bytes, ctx->stream));
     if (!direct_path(ctx, &o)) HIPCHK(hipMemsetAsync(ctx->eeB.p, 0, sizeof(double) * SN, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->dens.p, 0, sizeof(double) * (size_t)B * N, ctx->stream));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(N, 256)), dim3(256), 0, ctx->stream, N, ctx->T_ref,
+                       ctx->tsrc.as<double>());  // the zero density's source temperature
+    ctx->tsrc_ok = true;
     HIPCHK(hipMemsetAsync(ctx->T.p, 0, sizeof(double) * N, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->iters.p, 0, sizeof(int) * S, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
@@ -2281,7 +2349,7 @@ int bte_set_temperature(bte_ctx* ctx, const double* T) {
     to_internal(ctx, T, hT.data());
     HIPCHK(hipMemcpyAsync(ctx->T.p, hT.data(), sizeof(double) * ctx->N, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(k_set_density, dim3(grid_for(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, bands_of(ctx),
-                       ctx->T.as<double>(), ctx->dens.as<double>());
+                       ctx->T.as<double>(), ctx->dens.as<double>(), ctx->tsrc.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));  // hT is a host temporary
     return BTE_OK;
@@ -2494,6 +2562,7 @@ int bte_debug_assemble(bte_ctx* ctx, int band, int dir, double* diag, double* of
 static int upload_state(bte_ctx* ctx, const double* ee_prev, const double* density) {
     ctx->ee_final = nullptr;
     ctx->run_active = false;
+    ctx->tsrc_ok = false;  // a density uploaded as such has no source temperature
     const int N = ctx->N, B = ctx->B;
     const size_t S = (size_t)B * ctx->D, SN = S * N;
     std::vector<double> ee(SN);
@@ -2515,7 +2584,7 @@ int bte_debug_rhs(bte_ctx* ctx, const double* ee_prev, const double* density, do
     if ((rc = upload_state(ctx, ee_prev, density))) return rc;
     const size_t SN = (size_t)ctx->B * ctx->D * ctx->N;
     if (ctx->has_diffuse)
-        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diffuse, dim3(diffuse_grid(ctx)), dim3(256), 0, ctx->stream,
                            geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeB.as<double>(), ctx->ediff.as<double>());
     hipLaunchKernelGGL(k_rhs, dim3(grid_for((long long)SN, 256)), dim3(256), 0, ctx->stream, geo_of(ctx), quad_of(ctx),
                        bnd_of(ctx), bands_of(ctx), ctx->eeB.as<double>(), ctx->dens.as<double>(), ctx->wb.as<double>());
@@ -2546,7 +2615,7 @@ int bte_debug_inner(bte_ctx* ctx, const bte_opts* opts, const double* ee_prev, c
     HIPCHK(hipMemsetAsync(ctx->iters.p, 0, sizeof(int) * S, ctx->stream));
     HIPCHK(hipMemsetAsync(ctx->fails.p, 0, sizeof(int) * S, ctx->stream));
     if (ctx->has_diffuse)
-        hipLaunchKernelGGL(k_diffuse, dim3(grid_for((long long)ctx->B * ctx->n_bf, 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diffuse, dim3(diffuse_grid(ctx)), dim3(256), 0, ctx->stream,
                            geo_of(ctx), quad_of(ctx), bnd_of(ctx), ctx->B, ctx->eeB.as<double>(), ctx->ediff.as<double>());
     Shape shape;
     if ((rc = pick_shape(ctx, &o, shape, false))) return rc;
@@ -2706,16 +2775,17 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         e.Tband = ctx->Tband.as<double>();
         e.T = ctx->T.as<double>();
         e.dens = ctx->dens.as<double>();
+        e.tsrc = ctx->tsrc.as<double>();
         e.partial = ctx->partial.as<double>();
         e.margin = ctx->margin.as<double>();
         e.omega = ctx->omega.as<double>();
         e.nblk_tband = grid_for((long long)ctx->B * ctx->N, 256);
         e.scale = 1.0 / ctx->B / ctx->N;
         // work-groups per step, as the single-problem launches partition it
-        const int nb[ST_N] = {ctx->has_diffuse ? grid_for((long long)ctx->B * ctx->n_bf, 256) : 0,
+        const int nb[ST_N] = {ctx->has_diffuse ? diffuse_grid(ctx) : 0,
                               b->ring > 0 ? grid_for((long long)ctx->D * lo_stride(ctx->N), 256) : 0,
                               (b->ring > 0 && ctx->n_bnd_list > 0)
-                                  ? grid_for((long long)ctx->n_bnd_list * ctx->B, 64) : 0,
+                                  ? rhs_bnd_grid(ctx) : 0,
                               e.nblk_tband, grid_for(ctx->N, 256)};
         for (int k = 0; k < ST_N; k++) offs[k * (P + 1) + a + 1] = offs[k * (P + 1) + a] + nb[k];
         SweepDesc& d = hd[a];  // per iteration: the intensity buffers alternate

@@ -65,6 +65,34 @@ def test_square_layouts_agree_with_padding_and_learned_corrections(tmp_path):
     c.close()
 
 
+@pytest.mark.parametrize("layout", [1, 2])
+def test_wide_ring_with_mixing_and_corrections_matches_variant_4(tmp_path, layout):
+    """Variant 8 forms the interior right-hand sides inside the sweep from the source temperature
+    that every writer of the emission density records (cell temperatures, Anderson updates and
+    gathers, network corrections and their undo); variant 4 reads them from the separate pass.
+    With Anderson mixing and network corrections: identical runs in either layout."""
+    say("wide ring, learned, layout", layout)
+    p = meshgen.make_square_case(str(tmp_path), 3000, 21, 10, 21, L=1e-5)
+    c = capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=1e-5, Ly=1e-5)
+    g = Gnn(layers=4, seed=0, device=0)
+    g.set_params(np.load(DEFAULT_WEIGHTS))
+    c.attach_gnn(g)
+    kw = dict(block=128, accel=capi.ACCEL_ANDERSON, learned=1, learned_every=4, max_iter=20000,
+              intensity_layout=layout)
+    a = c.solve(sweep_variant=4, **kw)
+    assert c.stats()["variant"] == 4
+    b = c.solve(sweep_variant=8, **kw)
+    assert c.stats()["variant"] == 8 and c.stats()["intensity_layout"] == layout
+    assert a["learned_proposed"] > 0
+    assert (a["learned_proposed"], a["learned_accepted"]) == (b["learned_proposed"], b["learned_accepted"])
+    assert a["iters"] == b["iters"]
+    np.testing.assert_array_equal(a["T"], b["T"])
+    np.testing.assert_array_equal(a["margins"], b["margins"])
+    np.testing.assert_array_equal(a["flux"], b["flux"])
+    c.attach_gnn(None)
+    c.close()
+
+
 def test_other_forms_stay_band_major(tmp_path):
     """The interleaved layout is only for the pipelined sweep: asked for with another form, the
     solve runs band-major."""
