@@ -1863,9 +1863,10 @@ __global__ void k_tcell(int N, Bands bands, const double* __restrict__ Tband, do
 // q = 2 ncols: f . f (nblk = kAaDotBlocks)
 // Grid (nblk, ceil(ncols / G) + 1): work-group (x, y < ceil(ncols / G)) reads columns
 // dF_{Gy .. Gy+G-1} and, once per entry for all of them, dF_new and f; (x, last) forms f . f.  Each
-// sum runs over the work-group's chunk in the same fixed order for every call.  (Measured on the
-// 50-column history of a 180k-entry state: G = 4 with 256-thread work-groups takes 15 us; wider
-// groups spend more on the work-group's cross-lane sums of 2 G values than they save in reads.)
+// sum runs over the work-group's chunk in the same fixed order for every call, whatever G.
+// (Measured on the 50-column history of a 180k-entry state, columns not in cache: 256-thread
+// work-groups take 16 us with G = 2, 25 us with G = 4 and 32 us with G = 8 -- the work-group's
+// cross-lane sums of 2 G values and the fewer work-groups cost more than the shared reads save.)
 // Arguments of one step's dot products; out and host receive the reduced values (k_reduce_rows)
 struct AaDots {
     int Z, ncols;
@@ -1890,6 +1891,9 @@ __device__ __forceinline__ void aa_dots_block(const AaDots& a, int bx, int by) {
 #pragma unroll
     for (int k = 0; k < 2 * G; k++) s[k] = 0;
     if (j0 < ncols) {
+        // unrolled so that several entries' loads are in flight (a thread's entries are a
+        // dependent chain of sums otherwise); the sums keep their order
+#pragma unroll 4
         for (int c = bx * BLOCK + threadIdx.x; c < N; c += nblk * BLOCK) {
             const double av = dF_new[c], bv = f[c];
             double v[G];
@@ -2225,7 +2229,7 @@ __device__ __forceinline__ void aa_update_range(const AaUpdate& a, const Coef& g
     for (long long c = blk * (long long)blockDim.x + threadIdx.x; c < Z; c += (long long)nblk * blockDim.x) {
         double v = a.g[c];
         // unrolled so that several columns' loads are in flight; the updates keep their order
-#pragma unroll 8
+#pragma unroll 16
         for (int j = 0; j < a.ncols; j++) v -= gamma[j] * (double)dG[(size_t)j * Z + c];
         a.x[c] = v;
         if (c < a.N) {

@@ -67,6 +67,15 @@ std::atomic<unsigned long long> g_geo_tags{0};
 
 }  // namespace
 
+// Cholesky factor R (upper, R^T R = H + lam I) of the Gram matrix H of the Anderson history
+// columns, in the order the history keeps them (oldest first), carried from step to step.
+struct AaChol {
+    static constexpr int M = bte::kAaMaxCols;
+    int n = -1;  // columns factored; -1: none valid
+    double lam = 0;
+    double R[M * M];  // R[i * M + j], j >= i
+};
+
 struct bte_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -164,6 +173,8 @@ struct bte_ctx {
     bool aa_first = true;
     std::vector<int> aa_order;  // active slots, oldest first
     std::vector<double> aa_H;   // Gram matrix of the dF columns by slot [m*m]
+    AaChol aa_chol;             // its factor in aa_order's order
+    bool aa_popped = false;     // the last aa_launch dropped the oldest column
     double aa_best = 0;
     double aa_last_ff = 0;      // squared residual norm of the last evaluated Anderson state
     int aa_steps = 0;
@@ -1489,46 +1500,99 @@ static void small_solve_ge(std::vector<double> H, std::vector<double> b, int n, 
     }
 }
 
-// The same system by a Cholesky factorisation H + lambda I = U^T U (a third of the elimination's
-// work, no pivot search), right-looking on the upper triangle so that every inner loop runs along
-// a row (independent updates the compiler vectorises); the upper triangle of H is read.  A pivot
-// that is not positive (H + lambda I numerically indefinite) hands the system to small_solve_ge.
-static void small_solve(const std::vector<double>& H, const std::vector<double>& b, int n, std::vector<double>& y) {
-    constexpr int kMax = 64;
-    if (n > kMax) return small_solve_ge(H, b, n, y);
-    double U[kMax * kMax], z[kMax];
-    double dmax = 0;
-    for (int i = 0; i < n; i++) dmax = std::max(dmax, H[(size_t)i * n + i]);
-    const double lambda = 1e-12 * dmax + 1e-300;
+// Factors H + lam I = R^T R (n x n, row-major, upper triangle read) into c, right-looking on the
+// upper triangle so that every inner loop runs along a row (independent updates the compiler
+// vectorises); false on a pivot that is not positive (H + lam I numerically indefinite).
+static bool chol_factor(AaChol& c, const std::vector<double>& H, int n, double lam) {
+    constexpr int M = AaChol::M;
+    double* U = c.R;
+    c.n = -1;
+    if (n > M) return false;
     for (int i = 0; i < n; i++) {
-        for (int k = i; k < n; k++) U[i * kMax + k] = H[(size_t)i * n + k];
-        U[i * kMax + i] += lambda;
+        for (int k = i; k < n; k++) U[i * M + k] = H[(size_t)i * n + k];
+        U[i * M + i] += lam;
     }
     for (int j = 0; j < n; j++) {
-        double* Uj = U + j * kMax;
+        double* Uj = U + j * M;
         const double d = Uj[j];
-        if (!(d > 0) || !std::isfinite(d)) return small_solve_ge(H, b, n, y);
+        if (!(d > 0) || !std::isfinite(d)) return false;
         const double r = std::sqrt(d), inv = 1.0 / r;
         Uj[j] = r;
         for (int i = j + 1; i < n; i++) Uj[i] *= inv;
         for (int k = j + 1; k < n; k++) {
             const double u = Uj[k];
-            double* Uk = U + k * kMax;
+            double* Uk = U + k * M;
             for (int i = k; i < n; i++) Uk[i] -= u * Uj[i];
         }
     }
-    for (int i = 0; i < n; i++) z[i] = b[i];
-    for (int k = 0; k < n; k++) {  // U^T z = b, column by column
-        const double* Uk = U + k * kMax;
-        z[k] /= Uk[k];
-        for (int i = k + 1; i < n; i++) z[i] -= Uk[i] * z[k];
+    c.n = n;
+    c.lam = lam;
+    return true;
+}
+
+// Drops the first (oldest) column: R without its first column is upper Hessenberg, and Givens
+// rotations of neighbouring rows restore the triangle of the remaining columns' factor.
+static void chol_drop_first(AaChol& c) {
+    constexpr int M = AaChol::M;
+    const int n = c.n;
+    double* R = c.R;
+    for (int i = 0; i < n; i++)
+        for (int j = std::max(0, i - 1); j < n - 1; j++) R[i * M + j] = R[i * M + j + 1];
+    for (int k = 0; k + 1 < n; k++) {
+        double* a = R + k * M;
+        double* b = R + (k + 1) * M;
+        const double x = a[k], y = b[k], r = std::sqrt(x * x + y * y);  // entries are norms: no overflow
+        if (r > 0) {
+            const double cs = x / r, sn = y / r;
+            for (int j = k; j < n - 1; j++) {
+                const double t1 = cs * a[j] + sn * b[j], t2 = cs * b[j] - sn * a[j];
+                a[j] = t1;
+                b[j] = t2;
+            }
+        }
+        b[k] = 0;
+    }
+    c.n = n - 1;
+}
+
+// Appends a column with inner products h[0, n) with the factored ones and hnn with itself: one
+// triangular solve.  False when the new pivot is not safely positive (a column nearly in the span
+// of the others), for a new factorisation to decide.
+static bool chol_append(AaChol& c, const double* h, double hnn) {
+    constexpr int M = AaChol::M;
+    const int n = c.n;
+    if (n < 0 || n >= M) return false;
+    double* R = c.R;
+    double ss = 0;
+    for (int i = 0; i < n; i++) {
+        double s = h[i];
+        for (int k = 0; k < i; k++) s -= R[k * M + i] * R[k * M + n];
+        const double r = s / R[i * M + i];
+        R[i * M + n] = r;
+        ss += r * r;
+    }
+    const double d = hnn + c.lam - ss;
+    if (!(d > 1e-8 * (hnn + c.lam)) || !std::isfinite(d)) return false;
+    R[n * M + n] = std::sqrt(d);
+    c.n = n + 1;
+    return true;
+}
+
+// R^T z = b, R y = z.
+static void chol_solve(const AaChol& c, const std::vector<double>& b, std::vector<double>& y) {
+    constexpr int M = AaChol::M;
+    const int n = c.n;
+    const double* R = c.R;
+    std::vector<double> z(b);
+    for (int k = 0; k < n; k++) {
+        z[k] /= R[k * M + k];
+        for (int i = k + 1; i < n; i++) z[i] -= R[k * M + i] * z[k];
     }
     y.assign(n, 0.0);
-    for (int i = n - 1; i >= 0; i--) {  // U y = z
-        const double* Ui = U + i * kMax;
+    for (int i = n - 1; i >= 0; i--) {
         double s = z[i];
-        for (int k = i + 1; k < n; k++) s -= Ui[k] * y[k];
-        y[i] = s / Ui[i];
+        for (int k = i + 1; k < n; k++) s -= R[i * M + k] * y[k];
+        y[i] = s / R[i * M + i];
     }
 }
 
@@ -1607,6 +1671,7 @@ static int aa_reset(bte_ctx* ctx) {
     ctx->learn_streak = 0;
     ctx->aa_order.clear();
     ctx->aa_H.assign((size_t)m * m, 0.0);
+    ctx->aa_chol.n = -1;
     ctx->aa_best = 0;
     return BTE_OK;
 }
@@ -1625,7 +1690,7 @@ static int load_state(bte_ctx* ctx) {
     return BTE_OK;
 }
 
-constexpr int kAaDotG = 4;  // history columns per work-group of the dot products
+constexpr int kAaDotG = 2;  // history columns per work-group of the dot products
 
 // The launches of one problem's mixing step, recorded instead of issued: the uneven batch issues
 // those of all its problems with one launch per kernel.
@@ -1677,12 +1742,14 @@ static int aa_launch(bte_ctx* ctx, bool* finish, AaRec* rec = nullptr) {
     const bool skip = ctx->aa_skip;
     ctx->aa_skip = false;
     int slot = 0;
+    ctx->aa_popped = false;
     if (!ctx->aa_first && !skip) {
         if ((int)ctx->aa_order.size() < m) {
             slot = (int)ctx->aa_order.size();
         } else {
             slot = ctx->aa_order.front();
             ctx->aa_order.erase(ctx->aa_order.begin());
+            ctx->aa_popped = true;
         }
     }
     // the first step is plain, x_1 = G(x_0), taken (with the state's loads) in the same pass
@@ -1715,6 +1782,42 @@ static int aa_launch(bte_ctx* ctx, bool* finish, AaRec* rec = nullptr) {
     return BTE_OK;
 }
 
+// The least squares of a mixing step, (H + lam I) y = b over the ordered history with
+// lam = 1e-12 max diag.  The kept factor follows the step's change of columns (the oldest dropped,
+// the newest appended: O(n^2)); it is factorised anew (O(n^3)) when it is not valid, when a new
+// pivot is not safely positive, or when lam has moved by more than a factor 100 from the one it
+// holds.  A factorisation that fails hands the system to elimination (small_solve_ge).
+static void aa_least_squares(bte_ctx* ctx, const std::vector<double>& b, std::vector<double>& y) {
+    const int m = ctx->aa_m, n = (int)ctx->aa_order.size();
+    AaChol& C = ctx->aa_chol;
+    auto Hij = [&](int i, int j) { return ctx->aa_H[(size_t)ctx->aa_order[i] * m + ctx->aa_order[j]]; };
+    double dmax = 0;
+    for (int i = 0; i < n; i++) dmax = std::max(dmax, Hij(i, i));
+    const double lam = 1e-12 * dmax + 1e-300;
+    bool ok = C.n >= 0;
+    if (ok && !ctx->aa_skip_now) {
+        if (ctx->aa_popped && C.n == n) chol_drop_first(C);
+        if (C.n == n - 1) {
+            double h[AaChol::M];
+            for (int i = 0; i + 1 < n; i++) h[i] = Hij(i, n - 1);
+            ok = chol_append(C, h, Hij(n - 1, n - 1));
+        } else {
+            ok = false;
+        }
+    }
+    ok = ok && C.n == n && lam <= 100 * C.lam && 100 * lam >= C.lam;
+    if (!ok) {
+        std::vector<double> H((size_t)n * n);
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) H[(size_t)i * n + j] = Hij(i, j);
+        if (!chol_factor(C, H, n, lam)) {
+            small_solve_ge(H, b, n, y);
+            return;
+        }
+    }
+    chol_solve(C, b, y);
+}
+
 // Second half, once the dot products have arrived: the m x m least squares on the host, then the
 // update (the coefficients travel as its kernel arguments, or, recorded, with its descriptor) and the
 // state.
@@ -1734,14 +1837,12 @@ static int aa_finish(bte_ctx* ctx, AaRec* rec = nullptr) {
     double* gamma = coef.g;
     if (ff > 1e6 * ctx->aa_best) {
         ctx->aa_order.clear();
+        ctx->aa_chol.n = -1;
     } else {
         const int n = (int)ctx->aa_order.size();
-        std::vector<double> H((size_t)n * n), b(n), y;
-        for (int i = 0; i < n; i++) {
-            b[i] = dots[nc + ctx->aa_order[i]];
-            for (int j = 0; j < n; j++) H[(size_t)i * n + j] = ctx->aa_H[(size_t)ctx->aa_order[i] * m + ctx->aa_order[j]];
-        }
-        small_solve(H, b, n, y);
+        std::vector<double> b(n), y;
+        for (int i = 0; i < n; i++) b[i] = dots[nc + ctx->aa_order[i]];
+        aa_least_squares(ctx, b, y);
         for (int i = 0; i < n; i++) gamma[ctx->aa_order[i]] = std::isfinite(y[i]) ? y[i] : 0.0;
     }
     // the update and the state's density and boundary intensities in one pass (load_state's work)
