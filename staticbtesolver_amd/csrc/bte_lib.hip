@@ -1865,7 +1865,7 @@ static int aa_step(bte_ctx* ctx) {
     bool finish = false;
     int rc = aa_launch(ctx, &finish);
     if (rc || !finish) return rc;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = wait_stream(ctx))) return rc;  // the dot products, in coherent host memory
     return aa_finish(ctx);
 }
 

@@ -51,6 +51,24 @@ def test_batch_with_anderson_on_uneven_meshes(tmp_path):
         same(s, b)
 
 
+def test_large_batch_matches_single_solves(tmp_path):
+    """Eight squares of 10 bands (2,560 systems, more work-groups than the device holds at once)
+    with Anderson mixing: every problem matches its single solve bit for bit."""
+    say("batch: large")
+    cases = []
+    for k in range(8):
+        n, L = 1200 + 400 * k, 10 ** (-5 - 0.3 * k)
+        p = meshgen.make_square_case(str(tmp_path / f"m{k}"), n, 40 + k, 10, 40 + k, L=L)
+        cases.append(capi.Case(p["mesh"], p["band"], p["bc"], 2, 8, 8, 300.0, Lx=L, Ly=L))
+    assert sum(c.N_band * c.N_dir for c in cases) >= 2048
+    opts = dict(accel=capi.ACCEL_ANDERSON)
+    single = [c.solve(sweep_variant=2, block=256, **opts) for c in cases]
+    assert len({s["iters"] for s in single}) > 1
+    out = capi.Batch(cases).solve(**opts)
+    for s, b in zip(single, out):
+        same(s, b)
+
+
 def test_batch_rejects_krylov_forms():
     c = capi.Case(**case_kwargs("2DM2DG"))
     with pytest.raises(capi.BTEError):
