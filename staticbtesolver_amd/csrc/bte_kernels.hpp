@@ -991,8 +991,20 @@ __device__ __forceinline__ void rhs_band_chunk(const double* __restrict__ dc, do
     for (int k = 0; k < NB; k++) out[(size_t)k * stride] = v[k] * vw;
 }
 
+// The same from the source temperature t of the cell: C_b / solid_angle * (t - T_ref) is the
+// emission density as its writers form it, so the products are the same.
+template <int NB>
+__device__ __forceinline__ void rhs_band_chunk_t(const double* __restrict__ C, double solid_angle, double tm, double vw,
+                                                 double* __restrict__ out, size_t stride) {
+#pragma unroll
+    for (int k = 0; k < NB; k++) out[(size_t)k * stride] = C[k] / solid_angle * tm * vw;
+}
+
+// tsrc (or null): the source temperatures, whose 8 bytes per cell the pass gathers instead of the
+// cell's B densities (the gather by cell is the pass's cost; the temperatures stay in L2).
 __device__ __forceinline__ void rhs_levels_range(const Ops& o, int B, const double* __restrict__ dens,
-                                                 double* __restrict__ rhs_lo, int blk, int nblk) {
+                                                 const double* __restrict__ tsrc, double* __restrict__ rhs_lo,
+                                                 int blk, int nblk) {
     const int N = o.N, D = o.D, Np = o.Np;
     const long long total = (long long)D * Np;
     const size_t stride = (size_t)D * Np;  // system (b, d) -> (b + 1, d)
@@ -1000,8 +1012,30 @@ __device__ __forceinline__ void rhs_levels_range(const Ops& o, int B, const doub
         const int d = (int)(t / Np), i = (int)(t % Np);
         if (i >= N) continue;                // padding of the row
         const size_t u = (size_t)d * N + i;  // the direction's level-order operands are unpadded
-        const double* __restrict__ dc = dens + (size_t)o.perm[u] * B;
         const double vw = o.swp_vw[u];
+        if (tsrc) {
+            const double tm = tsrc[o.perm[u]] - o.T_ref;
+            for (int b0 = 0; b0 < B; b0 += 12) {
+                const double* c0 = o.band_C + b0;
+                double* o0 = rhs_lo + t + (size_t)b0 * stride;
+                switch (min(12, B - b0)) {
+                    case 1: rhs_band_chunk_t<1>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 2: rhs_band_chunk_t<2>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 3: rhs_band_chunk_t<3>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 4: rhs_band_chunk_t<4>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 5: rhs_band_chunk_t<5>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 6: rhs_band_chunk_t<6>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 7: rhs_band_chunk_t<7>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 8: rhs_band_chunk_t<8>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 9: rhs_band_chunk_t<9>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 10: rhs_band_chunk_t<10>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    case 11: rhs_band_chunk_t<11>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                    default: rhs_band_chunk_t<12>(c0, o.solid_angle, tm, vw, o0, stride); break;
+                }
+            }
+            continue;
+        }
+        const double* __restrict__ dc = dens + (size_t)o.perm[u] * B;
         for (int b0 = 0; b0 < B; b0 += 12) {
             const double* c0 = dc + b0;
             double* o0 = rhs_lo + t + (size_t)b0 * stride;
@@ -1023,8 +1057,9 @@ __device__ __forceinline__ void rhs_levels_range(const Ops& o, int B, const doub
     }
 }
 
-__global__ void k_rhs_levels(Ops o, int B, const double* __restrict__ dens, double* __restrict__ rhs_lo) {
-    rhs_levels_range(o, B, dens, rhs_lo, blockIdx.x, gridDim.x);
+__global__ void k_rhs_levels(Ops o, int B, const double* __restrict__ dens, const double* __restrict__ tsrc,
+                             double* __restrict__ rhs_lo) {
+    rhs_levels_range(o, B, dens, tsrc, rhs_lo, blockIdx.x, gridDim.x);
 }
 
 // Source temperatures in level order, tlo[d * Np + i] = tsrc[cell at position i of direction d],
@@ -1059,11 +1094,11 @@ __global__ void k_diffuse_tsrc_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const
 // larger one.
 __global__ void k_diffuse_rhs_levels(Geo g, Quad q, Bnd bd, Ops o, int B, const double* __restrict__ ee_prev,
                                      double* __restrict__ ediff, const double* __restrict__ dens,
-                                     double* __restrict__ rhs_lo, int n_diff) {
+                                     const double* __restrict__ tsrc, double* __restrict__ rhs_lo, int n_diff) {
     if ((int)blockIdx.x < n_diff)
         diffuse_range(g, q, bd, B, ee_prev, ediff, blockIdx.x, n_diff);
     else
-        rhs_levels_range(o, B, dens, rhs_lo, blockIdx.x - n_diff, gridDim.x - n_diff);
+        rhs_levels_range(o, B, dens, tsrc, rhs_lo, blockIdx.x - n_diff, gridDim.x - n_diff);
 }
 
 // The boundary cells of k_rhs_levels from a compact list of level-order indices t = d*N + i, so
@@ -2310,6 +2345,7 @@ struct IterDesc {
     double* T;
     double* dens;
     double* tsrc;
+    const double* tsrc_rhs;  // tsrc when it holds (the right-hand sides are formed from it), else null
     double* partial;
     double* margin;
     const double* omega;
@@ -2345,7 +2381,7 @@ __global__ void k_diffuse_multi(cmem<IterDesc>* __restrict__ desc, const int* __
 __global__ void k_rhs_levels_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {
     const MultiIdx m = multi_index(off, n);
     const IterDesc D = cload(desc + m.p);
-    rhs_levels_range(D.o, D.bands.B, D.dens, D.rhs, m.blk, m.nblk);
+    rhs_levels_range(D.o, D.bands.B, D.dens, D.tsrc_rhs, D.rhs, m.blk, m.nblk);
 }
 
 __global__ void k_rhs_bnd_multi(cmem<IterDesc>* __restrict__ desc, const int* __restrict__ off, int n) {

@@ -1116,13 +1116,15 @@ void enqueue_rhs_levels(bte_ctx* ctx, hipStream_t st, const double* ee_prev, con
                         bool with_diffuse = false) {
     const int S = ctx->B * ctx->D;
     const int n_rhs = grid_for((long long)ctx->D * lo_stride(ctx->N), 256);
+    // the interior cells' densities from their source temperatures while those hold
+    const double* ts = (ctx->tsrc_ok && dens == ctx->dens.as<double>()) ? ctx->tsrc.as<double>() : nullptr;
     if (with_diffuse) {
         const int n_diff = diffuse_grid(ctx);
         hipLaunchKernelGGL(k_diffuse_rhs_levels, dim3((unsigned)(n_diff + n_rhs)), dim3(256), 0, st, geo_of(ctx),
                            quad_of(ctx), bnd_of(ctx), ops_of(ctx), ctx->B, ee_prev, ctx->ediff.as<double>(), dens,
-                           out, n_diff);
+                           ts, out, n_diff);
     } else {
-        hipLaunchKernelGGL(k_rhs_levels, dim3(n_rhs), dim3(256), 0, st, ops_of(ctx), ctx->B, dens, out);
+        hipLaunchKernelGGL(k_rhs_levels, dim3(n_rhs), dim3(256), 0, st, ops_of(ctx), ctx->B, dens, ts, out);
     }
     if (ctx->n_bnd_list > 0)
         hipLaunchKernelGGL(k_rhs_levels_bnd, dim3(rhs_bnd_grid(ctx)), dim3(64), 0, st,
@@ -2877,6 +2879,7 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
         e.T = ctx->T.as<double>();
         e.dens = ctx->dens.as<double>();
         e.tsrc = ctx->tsrc.as<double>();
+        e.tsrc_rhs = ctx->tsrc_ok ? ctx->tsrc.as<double>() : nullptr;
         e.partial = ctx->partial.as<double>();
         e.margin = ctx->margin.as<double>();
         e.omega = ctx->omega.as<double>();
