@@ -14,9 +14,13 @@ ARGS=${*:-"--profile-run"}
 cd /tmp
 echo "== kernel trace: bench.py $ARGS"
 timeout -k 10 300 rocprofv3 --kernel-trace --stats -d "$OUT/trace" -o bench -- python3 "$ROOT/bench.py" $ARGS
+# with PROF_KERNEL, counters are collected for that kernel only (collecting them serialises every
+# dispatch, which makes a run with thousands of small launches very slow)
+FILTER=()
+[ -n "${PROF_KERNEL:-}" ] && FILTER=(--kernel-include-regex "$PROF_KERNEL")
 for C in FETCH_SIZE WRITE_SIZE; do
   echo "== pmc $C: bench.py $ARGS"
-  timeout -s KILL 200 rocprofv3 --pmc $C -d "$OUT/pmc_$C" -o bench -- python3 "$ROOT/bench.py" $ARGS
+  timeout -s KILL 200 rocprofv3 --pmc $C "${FILTER[@]}" -d "$OUT/pmc_$C" -o bench -- python3 "$ROOT/bench.py" $ARGS
   echo "== pmc $C: hbm_copy 1024 MiB"
   timeout -s KILL 60 rocprofv3 --pmc $C -d "$OUT/pmc_copy_$C" -o copy -- "$ROOT/staticbtesolver_amd/bin/hbm_copy" 1024 3
 done
