@@ -1694,6 +1694,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep_batch(const SweepDesc* __restri
     extern __shared__ double lds_x[];
     for (int t = blockIdx.x; t < n_items; t += gridDim.x) {
         const int2 it = items[t];
+        if (it.x < 0) continue;  // padding of an XCD's sequence
         const SweepDesc& D = desc[it.x];
         if constexpr (RING)
             switch (D.o.W) {

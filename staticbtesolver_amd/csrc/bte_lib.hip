@@ -2906,15 +2906,29 @@ static int batch_iteration(bte_batch* b, const std::vector<int>& act, std::vecto
     // the work items change only when a problem leaves the batch
     std::vector<int2> items;
     if (act != b->items_act) {
+        // One group per (problem, direction): its bands share the direction's operator records.
+        // Work-group t runs on XCD t % 8, so each XCD gets whole groups in consecutive slots of its
+        // own (t = 8 j + x): a direction's bands sweep at the same time on one XCD and read the
+        // records from its L2.  Groups go longest level chain first, each to the XCD with the
+        // fewest items so far; the XCDs' shorter sequences are padded with empty items (x = -1).
         for (size_t a = 0; a < act.size(); a++) {
             const bte_ctx* ctx = b->ctx[act[a]];
-            const int S = ctx->B * ctx->D;
-            for (int s = 0; s < S; s++) order.push_back({ctx->h_lvl_cnt[s % ctx->D], make_int2((int)a, s)});
+            for (int d = 0; d < ctx->D; d++) order.push_back({ctx->h_lvl_cnt[d], make_int2((int)a, d)});
         }
-        // longest level chains first, so the last work-groups to start are the short ones
         std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-        items.resize(order.size());
-        for (size_t t = 0; t < order.size(); t++) items[t] = order[t].second;
+        std::vector<int2> xq[kXcds];
+        for (const auto& g : order) {
+            int x = 0;
+            for (int k = 1; k < kXcds; k++)
+                if (xq[k].size() < xq[x].size()) x = k;
+            const bte_ctx* ctx = b->ctx[act[g.second.x]];
+            for (int bd = 0; bd < ctx->B; bd++) xq[x].push_back(make_int2(g.second.x, bd * ctx->D + g.second.y));
+        }
+        size_t len = 0;
+        for (int x = 0; x < kXcds; x++) len = std::max(len, xq[x].size());
+        items.assign(len * kXcds, make_int2(-1, 0));
+        for (int x = 0; x < kXcds; x++)
+            for (size_t j = 0; j < xq[x].size(); j++) items[j * kXcds + x] = xq[x][j];
         b->n_items = (int)items.size();
         BCHK(b->items.ensure(sizeof(int2) * items.size()));
         BCHK(b->mptr.ensure(sizeof(double*) * mp.size()));
