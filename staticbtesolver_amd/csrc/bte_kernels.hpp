@@ -1843,7 +1843,10 @@ __global__ __launch_bounds__(BLOCK) void k_solve_mixed(Ops o, Work w, SolveCfg c
 // (band, cell) order for the band-major intensity layout, in (cell, band) order (CB) for the
 // interleaved one, so that neighbouring threads read neighbouring intensities either way; a
 // band temperature sums its directions in the same order in both.
-template <int BLOCK, bool CB = false>
+// U > 1: a work-group takes U*BLOCK consecutive pairs at a time and a thread U of them (BLOCK
+// apart), loading all U in each direction step, so that each direction's plane is read in runs of
+// U*BLOCK values; the band temperatures are the same, the margin partials add in another order.
+template <int BLOCK, bool CB = false, int U = 1>
 __device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, const EeIdx& ei,
                                             const double* __restrict__ omega, const double* __restrict__ ee_curr,
                                             const double* __restrict__ ee_prev, double* __restrict__ Tband,
@@ -1851,6 +1854,45 @@ __device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, co
     __shared__ double lds[BLOCK / 64];
     double mp[1] = {0};
     const long long total = (long long)bands.B * N;
+    if constexpr (U > 1) {
+        for (long long t0 = blk * (long long)(U * BLOCK) + threadIdx.x; t0 < total;
+             t0 += (long long)nblk * (U * BLOCK)) {
+            size_t base[U];
+            double e0[U];
+            bool on[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const long long t = t0 + (long long)u * BLOCK;
+                on[u] = t < total;
+                const long long tt = on[u] ? t : t0;
+                const int b = CB ? (int)(tt % bands.B) : (int)(tt / N), c = CB ? (int)(tt / bands.B) : (int)(tt % N);
+                base[u] = ei.at(b, 0, c);
+                e0[u] = 0;
+            }
+#pragma unroll 2
+            for (int d = 0; d < D; d++) {
+                const double w = omega[d];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (on[u]) {
+                        const double ec = ee_curr[base[u] + (size_t)d * ei.sd];
+                        const double df = ec - ee_prev[base[u] + (size_t)d * ei.sd];
+                        e0[u] += ec * w;
+                        mp[0] += df * df;
+                    }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (on[u]) {
+                    const long long t = t0 + (long long)u * BLOCK;
+                    const int b = CB ? (int)(t % bands.B) : (int)(t / N), c = CB ? (int)(t / bands.B) : (int)(t % N);
+                    Tband[(size_t)b * N + c] = e0[u] / bands.C[b] + bands.T_ref;
+                }
+        }
+        block_sum<BLOCK, 1>(mp, lds);
+        if (threadIdx.x == 0) partial[blk] = mp[0];
+        return;
+    }
     for (long long t = blk * (long long)BLOCK + threadIdx.x; t < total; t += (long long)nblk * BLOCK) {
         const int b = CB ? (int)(t % bands.B) : (int)(t / N), c = CB ? (int)(t / bands.B) : (int)(t % N);
         const size_t base = ei.at(b, 0, c);
@@ -1868,11 +1910,11 @@ __device__ __forceinline__ void tband_block(int N, int D, const Bands& bands, co
     if (threadIdx.x == 0) partial[blk] = mp[0];
 }
 
-template <int BLOCK, bool CB = false>
+template <int BLOCK, bool CB = false, int U = 1>
 __global__ __launch_bounds__(BLOCK) void k_tband(int N, int D, Bands bands, EeIdx ei, const double* __restrict__ omega,
                                                 const double* __restrict__ ee_curr, const double* __restrict__ ee_prev,
                                                 double* __restrict__ Tband, double* __restrict__ partial) {
-    tband_block<BLOCK, CB>(N, D, bands, ei, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x, gridDim.x);
+    tband_block<BLOCK, CB, U>(N, D, bands, ei, omega, ee_curr, ee_prev, Tband, partial, blockIdx.x, gridDim.x);
 }
 
 // Equilibrium temperature and emission density per cell (StaticBTESolver.cpp:106-120).

@@ -1354,11 +1354,14 @@ int enqueue_iteration(bte_ctx* ctx, const bte_opts* o, Shape sh, const double* e
         // solve launch (beside the pipelined sweep's right-hand-side pass, or just before)
         launch_solve(ctx, o, sh, ee_prev, ee_curr, ctx->dens.as<double>(), ctx->has_diffuse);
     }
-    const int nblk = grid_for((long long)ctx->B * ctx->N, 256);
+    // interleaved intensities: two (cell, band) pairs per thread, so that a work-group reads each
+    // direction's plane in runs of 4 KB (3D 200k: 1.55 -> 1.30 ms)
+    const int pairs_per_thread = ctx->ee_inter ? 2 : 1;
+    const int nblk = grid_for(((long long)ctx->B * ctx->N + pairs_per_thread - 1) / pairs_per_thread, 256);
     {
         Timer tm(ctx, KA_TEMP, timing);
         if (ctx->ee_inter)
-            hipLaunchKernelGGL((k_tband<256, true>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
+            hipLaunchKernelGGL((k_tband<256, true, 2>), dim3(nblk), dim3(256), 0, ctx->stream, ctx->N, ctx->D, bn,
                                ctx->eei, ctx->omega.as<double>(), ee_curr, ee_prev, ctx->Tband.as<double>(),
                                partial_of(ctx));
         else
