@@ -90,15 +90,17 @@ def _strip_geometry(nx, ny, L):
     return vol, area.ravel(), normal.ravel(), nbr.ravel()
 
 
-def test_band_temperatures_beyond_one_launch():
-    """5 bands x 3.4M cells = 17M (band, cell) pairs, more than the 65536 x 256 threads of a capped
-    launch: after two iterations every band temperature lies between the wall temperatures."""
+@pytest.mark.parametrize("layout,bands", [(1, 5), (2, 10)])
+def test_band_temperatures_beyond_one_launch(layout, bands):
+    """Band-major: 5 bands x 3.4M cells = 17M (band, cell) pairs, more than the 65536 x 256 threads
+    of a capped launch.  Bands interleaved (two pairs per thread): 10 bands, 34M pairs, more than
+    65536 x 512.  After two iterations every band temperature lies between the wall temperatures."""
     nx, ny = 1300, 1300
-    say("band temperatures over", 5 * 2 * nx * ny, "pairs")
+    say("band temperatures over", bands * 2 * nx * ny, "pairs, layout", layout)
     L = 1e-6
     vol, area, normal, nbr = _strip_geometry(nx, ny, L)
     N = len(vol)
-    assert 5 * N > 65536 * 256
+    assert bands * N > 65536 * 256 * layout
     ref = capi.Case(**case_kwargs("2DM2DG"))  # DM 2, t = p = 8 quadrature of the same kind
     om, d, S = ref.quadrature()
     L_ = capi.lib()
@@ -109,9 +111,10 @@ def test_band_temperatures_beyond_one_launch():
         om, d, S = [np.ascontiguousarray(a) for a in (om, d.ravel(), S.ravel())]
         assert L_.bte_set_quadrature(ctx, 2, len(om), capi._dp(om), capi._dp(d), capi._dp(S), 8, 8,
                                      float(ref.solid_angle), 300.0) == 0
-        capi.ctx_set_bands(ctx, *meshgen.random_bands(5, 4).T)
+        capi.ctx_set_bands(ctx, *meshgen.random_bands(bands, 4).T)
         capi.ctx_set_bcs(ctx, [-1, -2, -3, -4], [1, 2, 2, 1], [301.0, 300.0, 300.0, 299.0])
-        r = capi.ctx_solve(ctx, N, 5, 4, max_iter=2, stop_margin=0.0, block=256, sweep_variant=4)
+        r = capi.ctx_solve(ctx, N, bands, 4, max_iter=2, stop_margin=0.0, block=256, sweep_variant=4,
+                           intensity_layout=layout)
         assert r["iters"] == 2
         Tb = r["T_band"]
         assert np.all(np.isfinite(Tb))
